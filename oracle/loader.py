@@ -1,0 +1,302 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+Array conventions follow the reference's MATLAB call surface
+(``MultiD_bspline/visual code/CPUmleFit_LM/mexFunction.cpp:129-311``):
+column-major (Fortran-order) float32 arrays.
+
+* ``imstack``: shape (S, S, N); element (row ii, column jj) of ROI n.
+* ``coeff``:   shape (nx+3, ny+3, nz+3[, nw+3[, nu+3]]), bsarray coefficients.
+* outputs:     ``P`` (N, NV+1) with the iteration count in the last column,
+               ``CRLB`` (N, NV) variances, ``LL`` (N,).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+REF_SPLINE = os.path.join(HERE, "_ref", "libref_bspline.so")
+REF_MATINV = os.path.join(HERE, "_ref", "libref_matinv.so")
+
+NV_OF = {6: 5, 7: 6, 8: 8}
+D_OF = {6: 3, 7: 4, 8: 5}
+
+_lib = None
+_ref = None
+_refinv = None
+
+fp = C.POINTER(C.c_float)
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+def build(with_ref=None):
+    """Compile liborc.so (and _ref/ when the reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liborc.so"], check=True)
+    if with_ref is None:
+        with_ref = os.path.isdir("/root/reference")
+    if with_ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def _L():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(with_ref=False)
+        _lib = C.CDLL(LIB)
+        for sfx, rp in (("", fp), ("_f64", dp)):
+            f = getattr(_lib, "orc_fit" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.c_int, fp, C.c_int, C.c_int, fp, ip, fp, C.c_int, fp, rp, rp, rp, C.c_int]
+            g = getattr(_lib, "orc_bspline" + sfx)
+            g.restype = C.c_float if sfx == "" else C.c_double
+            g.argtypes = [C.c_float if sfx == "" else C.c_double, C.c_int]
+    return _lib
+
+
+def _f32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float32))
+
+
+def _ptr(a, t=fp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _fit(sfx, rdt, fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1):
+    lib = _L()
+    imstack = _f32(imstack)
+    if imstack.ndim == 2:
+        imstack = imstack[:, :, None]
+    S, S2, N = imstack.shape
+    assert S == S2, "ROIs must be square"
+    coeff = _f32(coeff)
+    NV = NV_OF[fittype]
+    assert coeff.ndim == D_OF[fittype]
+    cd = np.zeros(5, np.int32)
+    cd[: coeff.ndim] = coeff.shape
+    c3 = _f32(coeff3) if coeff3 is not None else None
+    ip_ = _f32(np.ravel(initP)) if initP is not None else None
+    P = np.zeros((N, NV + 1), rdt, order="F")
+    CR = np.zeros((N, NV), rdt, order="F")
+    LL = np.zeros(N, rdt)
+    t = dp if rdt == np.float64 else fp
+    rc = getattr(lib, "orc_fit" + sfx)(
+        fittype, _ptr(imstack), S, N, _ptr(coeff), cd.ctypes.data_as(ip), _ptr(c3),
+        int(iterations), _ptr(ip_), _ptr(P, t), _ptr(CR, t), _ptr(LL, t), int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle: unsupported fittype %r" % fittype)
+    return P, CR, LL
+
+
+def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1):
+    """Parity (float) restatement of the reference CPU fitter."""
+    return _fit("", np.float32, fittype, imstack, iterations, coeff, initP, coeff3, nthreads)
+
+
+def fit_f64(fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1):
+    """The same fit with every quantity in double (FP64 oracle)."""
+    return _fit("_f64", np.float64, fittype, imstack, iterations, coeff, initP, coeff3, nthreads)
+
+
+# ---------------------------------------------------------------- primitives
+def bspline(x, deg, f64=False):
+    lib = _L()
+    return (lib.orc_bspline_f64 if f64 else lib.orc_bspline)(x, deg)
+
+
+def delta_tables(D, frac):
+    lib = _L()
+    fr = np.zeros(5, np.float32)
+    fr[:D] = frac
+    tab = np.zeros((D + 1) * 4 ** D, np.float32)
+    lib.orc_delta_tables(C.c_int(D), _ptr(fr), _ptr(tab))
+    return tab.reshape(D + 1, 4 ** D)
+
+
+def deriv_eval(D, coord, nd, tab, coeff, theta):
+    lib = _L()
+    co = np.zeros(5, np.float32)
+    co[:D] = coord
+    n = np.zeros(5, np.int32)
+    n[:D] = nd
+    tab = np.ascontiguousarray(tab, np.float32).ravel()
+    c = np.asarray(coeff, np.float32).ravel(order="F")
+    th = np.zeros(9, np.float32)
+    th[: len(theta)] = theta
+    dudt = np.zeros(8, np.float32)
+    model = np.zeros(1, np.float32)
+    lib.orc_deriv_eval(C.c_int(D), _ptr(co), n.ctypes.data_as(ip), _ptr(tab), _ptr(c),
+                       _ptr(th), _ptr(dudt), _ptr(model))
+    return dudt[: D + 2], float(model[0])
+
+
+def cholesky(A):
+    lib = _L()
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, np.float32).ravel()
+    L = np.zeros(n * n, np.float32)
+    U = np.zeros(n * n, np.float32)
+    info = lib.orc_cholesky(_ptr(A), C.c_int(n), _ptr(L), _ptr(U))
+    return info, L.reshape(n, n), U.reshape(n, n)
+
+
+def lu_eval(L, U, b):
+    lib = _L()
+    n = len(b)
+    x = np.zeros(n, np.float32)
+    lib.orc_lu_eval(_ptr(np.ascontiguousarray(L, np.float32).ravel()),
+                    _ptr(np.ascontiguousarray(U, np.float32).ravel()),
+                    _ptr(np.ascontiguousarray(b, np.float32)), C.c_int(n), _ptr(x))
+    return x
+
+
+def matinv(M):
+    lib = _L()
+    n = M.shape[0]
+    Mc = np.asfortranarray(M, np.float32).ravel(order="F").copy()
+    Minv = np.zeros(n * n, np.float32)
+    D = np.zeros(n, np.float32)
+    lib.orc_matinv(_ptr(Mc), _ptr(Minv), _ptr(D), C.c_int(n))
+    return D, Minv.reshape(n, n, order="F")
+
+
+def com2d(roi):
+    lib = _L()
+    r = _f32(roi)
+    x = C.c_float()
+    y = C.c_float()
+    lib.orc_com2d(C.c_int(r.shape[0]), _ptr(r), C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def maxmin2d(roi):
+    lib = _L()
+    r = _f32(roi)
+    a = C.c_float()
+    b = C.c_float()
+    lib.orc_maxmin2d(C.c_int(r.shape[0]), _ptr(r), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+# ------------------------------------------------- the reference's own code
+def ref_available():
+    return os.path.exists(REF_SPLINE) and os.path.exists(REF_MATINV)
+
+
+def _R():
+    global _ref, _refinv
+    if _ref is None:
+        _ref = C.CDLL(REF_SPLINE)
+        _refinv = C.CDLL(REF_MATINV)
+    return _ref, _refinv
+
+
+# mangled C++ names of the reference's functions (CPUsplineLib.h, MatInvLib.h)
+_M = {
+    "eval": "_Z18kernel_evalBSplinefiPf",
+    "d3": "_Z29kernel_computeDelta3D_bSplinefffPfS_S_S_",
+    "d4": "_Z29kernel_computeDelta4D_bSplineffffPfS_S_S_S_",
+    "d5": "_Z29kernel_computeDelta5D_bSplinefffffPfS_S_S_S_S_",
+    "v3": "_Z26kernel_Derivative_bSpline1fffiiiPfS_S_S_PKfS_S_S_",
+    "v4": "_Z28kernel_Derivative_bSpline14DffffiiiiPfS_S_S_S_PKfS_S_S_",
+    "v5": "_Z28kernel_Derivative_bSpline15DfffffiiiiiPfS_S_S_S_S_PKfS_S_S_",
+    "chol": "_Z15kernel_choleskyPfiS_S_",
+    "lu": "_Z17kernel_luEvaluatePfS_S_iS_",
+    "lu5": "_Z19kernel_luEvaluate5DPfS_S_iS_",
+    "inv": "_Z14kernel_MatInvNPfS_S_i",
+}
+
+
+def ref_bspline(x, deg):
+    r, _ = _R()
+    out = C.c_float()
+    getattr(r, _M["eval"])(C.c_float(x), C.c_int(deg), C.byref(out))
+    return out.value
+
+
+def ref_delta_tables(D, frac):
+    r, _ = _R()
+    T = 4 ** D
+    tabs = [np.zeros(T, np.float32) for _ in range(D + 1)]
+    args = [C.c_float(float(v)) for v in frac[:D]] + [_ptr(t) for t in tabs]
+    getattr(r, _M["d%d" % D])(*args)
+    return np.stack(tabs)
+
+
+def ref_deriv_eval(D, coord, nd, tab, coeff, theta):
+    r, _ = _R()
+    T = 4 ** D
+    tab = np.ascontiguousarray(tab, np.float32).reshape(D + 1, T)
+    tabs = [np.ascontiguousarray(tab[k]) for k in range(D + 1)]
+    c = np.asarray(coeff, np.float32).ravel(order="F")
+    th = np.zeros(9, np.float32)
+    th[: len(theta)] = theta
+    dudt = np.zeros(8, np.float32)
+    model = C.c_float()
+    args = ([C.c_float(float(v)) for v in coord[:D]] + [C.c_int(int(v)) for v in nd[:D]]
+            + [_ptr(t) for t in tabs] + [_ptr(c), _ptr(th), _ptr(dudt), C.byref(model)])
+    getattr(r, _M["v%d" % D])(*args)
+    return dudt[: D + 2], model.value
+
+
+def ref_cholesky(A):
+    r, _ = _R()
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, np.float32).ravel().copy()
+    L = np.zeros(n * n, np.float32)
+    U = np.zeros(n * n, np.float32)
+    info = getattr(r, _M["chol"])(_ptr(A), C.c_int(n), _ptr(L), _ptr(U))
+    return info, L.reshape(n, n), U.reshape(n, n)
+
+
+def ref_lu_eval(L, U, b, five_d=False):
+    r, _ = _R()
+    n = len(b)
+    x = np.zeros(n, np.float32)
+    getattr(r, _M["lu5" if five_d else "lu"])(
+        _ptr(np.ascontiguousarray(L, np.float32).ravel().copy()),
+        _ptr(np.ascontiguousarray(U, np.float32).ravel().copy()),
+        _ptr(np.ascontiguousarray(b, np.float32).copy()), C.c_int(n), _ptr(x))
+    return x
+
+
+def ref_matinv(M):
+    _, ri = _R()
+    n = M.shape[0]
+    Mc = np.asfortranarray(M, np.float32).ravel(order="F").copy()
+    Minv = np.zeros(n * n, np.float32)
+    D = np.zeros(n, np.float32)
+    getattr(ri, _M["inv"])(_ptr(Mc), _ptr(Minv), _ptr(D), C.c_int(n))
+    return D, Minv.reshape(n, n, order="F")
+
+
+def eval_pass(fittype, roi, theta, coeff, coeff3=None, f64=False):
+    """One deviance / gradient / Hessian pass at theta (no LM step).
+    Returns (err, J, H, mu) with mu indexed [row, column]."""
+    lib = _L()
+    roi = _f32(roi)
+    S = roi.shape[0]
+    coeff = _f32(coeff)
+    cd = np.zeros(5, np.int32)
+    cd[: coeff.ndim] = coeff.shape
+    c3 = _f32(coeff3) if coeff3 is not None else None
+    NV = NV_OF[fittype]
+    rdt = np.float64 if f64 else np.float32
+    t = dp if f64 else fp
+    th = np.zeros(9, rdt)
+    th[:NV] = theta[:NV]
+    J = np.zeros(NV, rdt)
+    H = np.zeros(NV * NV, rdt)
+    mu = np.zeros(S * S, rdt)
+    f = lib.orc_eval_pass_f64 if f64 else lib.orc_eval_pass
+    f.restype = C.c_double if f64 else C.c_float
+    err = f(C.c_int(fittype), _ptr(roi), C.c_int(S), _ptr(coeff), cd.ctypes.data_as(ip),
+            _ptr(c3), _ptr(th, t), _ptr(J, t), _ptr(H, t), _ptr(mu, t))
+    return err, J, H.reshape(NV, NV), mu.reshape(S, S, order="F")
