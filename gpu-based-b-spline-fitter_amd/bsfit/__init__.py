@@ -1,0 +1,16 @@
+"""bsfit -- MI355X B-spline PSF MLE fitter, Python side of the C-ABI.
+
+``GPUmleFit_LM_multiD`` mirrors the reference's MATLAB entry point
+(``MultiD_bspline/visual code/GPUmleFit_LM/mexFunction.cpp:14-55``):
+
+    P, CRLB, LogL = GPUmleFit_LM_multiD(imstack, fittype, iterations, coeff,
+                                        varmap=None, silent=1, initP=None, coeff3=None)
+
+with ``imstack`` (S, S, N) and ``coeff`` in MATLAB (Fortran) order.  All work
+runs in ``libbsfit.so`` (gfx950 HIP kernels); there is no CPU path: if the
+library or a gfx950 device is missing, calls raise ``BsfError``.
+"""
+from .api import (  # noqa: F401
+    BsfError, GPUmleFit_LM_multiD, fit, fit_device, lib_path, load, num_params,
+    PREC_MIX, PREC_F64,
+)

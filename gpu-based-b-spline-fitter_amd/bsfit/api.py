@@ -1,0 +1,151 @@
+"""ctypes binding of include/bsfit.h (libbsfit.so)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libbsfit.so")
+
+PREC_MIX = 0
+PREC_F64 = 1
+_NV = {6: 5, 7: 6, 8: 8}
+_D = {6: 3, 7: 4, 8: 5}
+
+_STATUS = {0: "BSF_OK", 1: "BSF_ERR_ARG", 2: "BSF_ERR_FITTYPE", 3: "BSF_ERR_ROI_SIZE",
+           4: "BSF_ERR_SHAPE", 5: "BSF_ERR_NOMEM", 6: "BSF_ERR_DEVICE"}
+
+
+class BsfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (_STATUS.get(code, code), msg))
+        self.code = code
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("fittype", C.c_int),
+        ("imstack", C.c_void_p),
+        ("sz", C.c_int),
+        ("nfits", C.c_int),
+        ("iterations", C.c_int),
+        ("coeff", C.c_void_p),
+        ("coeff_ndim", C.c_int),
+        ("coeff_dims", C.c_int * 5),
+        ("coeff3", C.c_void_p),
+        ("initP", C.c_void_p),
+        ("n_initP", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [("P", C.c_void_p), ("CRLB", C.c_void_p), ("LL", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib_path():
+    return LIB_PATH
+
+
+def load():
+    """Load libbsfit.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BsfError(6, "libbsfit.so is not built (run __graft_entry__.build() or make -C %s)" % PKG_DIR)
+        lib = C.CDLL(LIB_PATH)
+        lib.bsf_fit_host.argtypes = [C.POINTER(Request), C.POINTER(Result)]
+        lib.bsf_fit_host.restype = C.c_int
+        lib.bsf_fit_device.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_void_p]
+        lib.bsf_fit_device.restype = C.c_int
+        lib.bsf_last_error.restype = C.c_char_p
+        lib.bsf_num_params.argtypes = [C.c_int]
+        lib.bsf_num_params.restype = C.c_int
+        lib.bsf_GPUmleFit_LM_multiD.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def num_params(fittype):
+    return load().bsf_num_params(int(fittype))
+
+
+def _check(rc):
+    if rc != 0:
+        raise BsfError(rc, load().bsf_last_error().decode())
+
+
+def _f32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float32))
+
+
+def _request(fittype, S, N, iterations, coeff_ptr, coeff_shape, coeff3_ptr, initP, precision, data_ptr):
+    r = Request()
+    r.fittype = int(fittype)
+    r.imstack = data_ptr
+    r.sz = int(S)
+    r.nfits = int(N)
+    r.iterations = int(iterations)
+    r.coeff = coeff_ptr
+    r.coeff_ndim = len(coeff_shape)
+    for a, d in enumerate(coeff_shape):
+        r.coeff_dims[a] = int(d)
+    r.coeff3 = coeff3_ptr
+    r.initP = initP.ctypes.data if initP is not None else None
+    r.n_initP = 0 if initP is None else int(initP.size)
+    r.precision = int(precision)
+    return r
+
+
+def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=PREC_MIX):
+    """Fit an ROI stack (S, S, N) with host arrays; returns (P, CRLB, LL) with
+    P (N, NV+1) and CRLB (N, NV) in Fortran order, as the MEX returns them."""
+    lib = load()
+    imstack = _f32(imstack)
+    if imstack.ndim == 2:
+        imstack = imstack[:, :, None]
+    if imstack.ndim != 3:
+        raise BsfError(1, "imstack must be 2-D or 3-D")
+    S, S2, N = imstack.shape
+    if S != S2:
+        raise BsfError(4, "Fit Box must be square")
+    coeff = _f32(coeff)
+    c3 = _f32(coeff3) if coeff3 is not None else None
+    ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
+    NV = _NV.get(int(fittype), 1)
+    P = np.zeros((N, NV + 1), np.float32, order="F")
+    CR = np.zeros((N, NV), np.float32, order="F")
+    LL = np.zeros(N, np.float32)
+    req = _request(fittype, S, N, iterations, coeff.ctypes.data, coeff.shape,
+                   c3.ctypes.data if c3 is not None else None, ip, precision, imstack.ctypes.data)
+    res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
+    _check(lib.bsf_fit_host(C.byref(req), C.byref(res)))
+    return P, CR, LL
+
+
+def fit_device(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_ptr, CRLB_ptr, LL_ptr,
+               initP=None, coeff3_ptr=None, precision=PREC_MIX, stream=None):
+    """Queue a fit on device-resident buffers (raw device addresses, e.g.
+    torch tensor .data_ptr()) on a HIP stream handle; returns immediately."""
+    lib = load()
+    ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
+    req = _request(fittype, S, N, iterations, coeff_ptr, coeff_shape, coeff3_ptr, ip, precision, data_ptr)
+    res = Result(P_ptr, CRLB_ptr, LL_ptr)
+    _check(lib.bsf_fit_device(C.byref(req), C.byref(res), stream))
+
+
+def GPUmleFit_LM_multiD(imstack, fittype, iterations=50, coeff=None, varmap=None, silent=1,
+                        initP=None, coeff3=None):
+    """The reference MEX's call surface (GPUmleFit_LM/mexFunction.cpp:14-55).
+    varmap is ignored for fittypes 6-8 as in the MEX (:363-373)."""
+    if coeff is None:
+        raise BsfError(1, "coeff is required for fittypes 6-8")
+    fittype = int(fittype)
+    P, CR, LL = fit(fittype, imstack, int(iterations), coeff,
+                    initP=initP if fittype > 6 else None, coeff3=coeff3)
+    if not silent:
+        print("subregion size: %d fittype: %d\nNumber of fits: %d" % (np.shape(imstack)[0], fittype, P.shape[0]))
+    return P, CR, LL

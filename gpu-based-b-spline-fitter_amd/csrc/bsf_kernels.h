@@ -1,0 +1,26 @@
+// Internal launch interface between the C-ABI (bsfit_api.cpp) and the
+// gfx950 kernels (bsf_fit_kernels.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace bsf {
+
+struct FitLaunch {
+    int fittype;            // 6, 7, 8
+    int f64;                // 0: FP32 evaluation + FP64 solve, 1: all FP64
+    const float *data;      // device, column-major [S, S, N]
+    const float *coeff;     // device, bsarray layout, dims cdims[0..ndim)
+    const float *coeff3;    // device, fittype 8 only
+    float *P, *CRLB, *LL;   // device outputs, column-major with stride N
+    int N, S, iterations;
+    int roi_offset, count;  // ROIs [roi_offset, roi_offset + count)
+    int cdims[5], ndim;
+    float initP[4];
+    int has_initP;
+};
+
+hipError_t launch_fit(const FitLaunch &L, hipStream_t st);
+size_t lds_bytes(int fittype, int f64, int S);
+
+}  // namespace bsf

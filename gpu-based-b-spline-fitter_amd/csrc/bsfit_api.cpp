@@ -1,0 +1,230 @@
+// bsfit_api.cpp -- the C-ABI (include/bsfit.h): argument checks with the
+// reference MEX's semantics, device buffers, launches, error reporting.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/bsfit.h"
+#include "bsf_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int nv_of(int ft) { return ft == 6 ? 5 : ft == 7 ? 6 : ft == 8 ? 8 : -1; }
+int dim_of(int ft) { return ft == 6 ? 3 : ft == 7 ? 4 : ft == 8 ? 5 : -1; }
+
+size_t coeff_numel(const bsf_request *r, int nd)
+{
+    size_t n = 1;
+    for (int a = 0; a < nd; a++) n *= (size_t)r->coeff_dims[a];
+    return n;
+}
+
+// Argument checks mirroring GPUmleFit_LM/mexFunction.cpp:247-385 for
+// fittypes 6-8, with explicit errors where the MEX would crash.
+int validate(const bsf_request *r, const bsf_result *res)
+{
+    if (!r || !res) return fail(BSF_ERR_ARG, "bsf: NULL request or result");
+    const int D = dim_of(r->fittype);
+    if (D < 0) return fail(BSF_ERR_FITTYPE, "bsf: fittype %d is not a B-spline fittype (6, 7, 8)", r->fittype);
+    if (r->nfits < 0) return fail(BSF_ERR_ARG, "bsf: nfits must be >= 0");
+    if (r->sz < 1 || r->sz > BSF_MAX_ROI)
+        return fail(BSF_ERR_ROI_SIZE, "bsf: X,Y dimension of data must be between 1 and %d (got %d)", BSF_MAX_ROI,
+                    r->sz);
+    if (r->iterations < 0) return fail(BSF_ERR_ARG, "bsf: iterations must be >= 0");
+    if (r->nfits > 0 && (!r->imstack || !res->P || !res->CRLB || !res->LL))
+        return fail(BSF_ERR_ARG, "bsf: NULL data or output pointer");
+    if (!r->coeff) return fail(BSF_ERR_ARG, "bsf: coeff must not be NULL");
+    if (r->coeff_ndim != D)
+        return fail(BSF_ERR_SHAPE, "bsf: fittype %d needs a %d-D coefficient array (got %d-D)", r->fittype, D,
+                    r->coeff_ndim);
+    for (int a = 0; a < D; a++)
+        if (r->coeff_dims[a] < 5)
+            return fail(BSF_ERR_SHAPE, "bsf: coefficient dim %d is %d; each axis needs >= 2 data samples (+3)", a,
+                        r->coeff_dims[a]);
+    if (r->fittype == 8 && !r->coeff3)
+        return fail(BSF_ERR_ARG, "bsf: fittype 8 needs coeff3 (the 3D coefficient array)");
+    if (r->initP) {
+        int need = r->fittype == 7 ? 2 : r->fittype == 8 ? 4 : 0;
+        if (r->n_initP < need)
+            return fail(BSF_ERR_ARG, "bsf: initP for fittype %d needs %d values (got %d)", r->fittype, need,
+                        r->n_initP);
+    }
+    if (r->precision != BSF_PREC_MIX && r->precision != BSF_PREC_F64)
+        return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", r->precision);
+    return BSF_OK;
+}
+
+int check_device(int S, int ft, int f64)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: no HIP device (%s)", hipGetErrorString(e));
+    hipDeviceProp_t p;
+    e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: hipGetDeviceProperties: %s", hipGetErrorString(e));
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(BSF_ERR_DEVICE, "bsf: device %d is %s; this build targets gfx950 (MI355X)", dev, p.gcnArchName);
+    size_t lds = bsf::lds_bytes(ft, f64, S);
+    if (lds > p.sharedMemPerBlock)
+        return fail(BSF_ERR_ROI_SIZE, "bsf: ROI size %d with fittype %d needs %zu B of LDS (device limit %zu B)", S,
+                    ft, lds, (size_t)p.sharedMemPerBlock);
+    return BSF_OK;
+}
+
+bsf::FitLaunch make_launch(const bsf_request *r, const float *data, const float *coeff, const float *coeff3,
+                           float *P, float *CRLB, float *LL)
+{
+    bsf::FitLaunch L;
+    std::memset(&L, 0, sizeof L);
+    L.fittype = r->fittype;
+    L.f64 = r->precision == BSF_PREC_F64;
+    L.data = data;
+    L.coeff = coeff;
+    L.coeff3 = coeff3;
+    L.P = P;
+    L.CRLB = CRLB;
+    L.LL = LL;
+    L.N = r->nfits;
+    L.S = r->sz;
+    L.iterations = r->iterations;
+    L.ndim = r->coeff_ndim;
+    for (int a = 0; a < 5; a++) L.cdims[a] = a < r->coeff_ndim ? r->coeff_dims[a] : 1;
+    L.has_initP = r->initP != nullptr;
+    if (r->initP) {
+        int n = r->fittype == 7 ? 2 : 4;
+        for (int i = 0; i < n; i++) L.initP[i] = r->initP[i];
+    }
+    return L;
+}
+
+// Launch in grid chunks (one workgroup per ROI).
+hipError_t launch_all(bsf::FitLaunch L, hipStream_t st)
+{
+    const int chunk = 1 << 24;
+    for (int off = 0; off < L.N; off += chunk) {
+        L.roi_offset = off;
+        L.count = (L.N - off) < chunk ? (L.N - off) : chunk;
+        hipError_t e = bsf::launch_fit(L, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsf_num_params(int fittype) { return nv_of(fittype); }
+
+const char *bsf_last_error(void) { return g_err.c_str(); }
+
+int bsf_fit_device(const bsf_request *r, bsf_result *res, void *stream)
+{
+    g_err.clear();
+    int rc = validate(r, res);
+    if (rc) return rc;
+    if ((rc = check_device(r->sz, r->fittype, r->precision == BSF_PREC_F64))) return rc;
+    if (r->nfits == 0) return BSF_OK;
+    bsf::FitLaunch L = make_launch(r, r->imstack, r->coeff, r->coeff3, res->P, res->CRLB, res->LL);
+    hipError_t e = launch_all(L, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
+    return BSF_OK;
+}
+
+int bsf_fit_host(const bsf_request *r, bsf_result *res)
+{
+    g_err.clear();
+    int rc = validate(r, res);
+    if (rc) return rc;
+    if ((rc = check_device(r->sz, r->fittype, r->precision == BSF_PREC_F64))) return rc;
+    if (r->nfits == 0) return BSF_OK;
+    const int D = dim_of(r->fittype), NV = nv_of(r->fittype);
+    const size_t N = (size_t)r->nfits, S2 = (size_t)r->sz * r->sz;
+    const size_t nc = coeff_numel(r, D);
+    const size_t nc3 = r->fittype == 8 ? (size_t)r->coeff_dims[0] * r->coeff_dims[1] * r->coeff_dims[2] : 0;
+    const size_t bytes_data = N * S2 * 4, bytes_c = nc * 4, bytes_c3 = nc3 * 4;
+    const size_t bytes_out = N * (size_t)(NV + 1 + NV + 1) * 4;
+    char *buf = nullptr;
+    const size_t a = 256;
+    auto up = [&](size_t x) { return (x + a - 1) / a * a; };
+    const size_t total = up(bytes_data) + up(bytes_c) + up(bytes_c3) + up(bytes_out);
+    hipError_t e = hipMalloc((void **)&buf, total);
+    if (e != hipSuccess) return fail(BSF_ERR_NOMEM, "bsf: could not allocate %zu bytes on the device", total);
+    float *d_data = (float *)buf;
+    float *d_c = (float *)(buf + up(bytes_data));
+    float *d_c3 = (float *)(buf + up(bytes_data) + up(bytes_c));
+    float *d_out = (float *)(buf + up(bytes_data) + up(bytes_c) + up(bytes_c3));
+    float *d_P = d_out, *d_CR = d_out + N * (NV + 1), *d_LL = d_CR + N * NV;
+    hipStream_t st = hipStreamPerThread;
+    rc = BSF_OK;
+    e = hipMemcpyAsync(d_data, r->imstack, bytes_data, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_c, r->coeff, bytes_c, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nc3) e = hipMemcpyAsync(d_c3, r->coeff3, bytes_c3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        bsf::FitLaunch L = make_launch(r, d_data, d_c, nc3 ? d_c3 : nullptr, d_P, d_CR, d_LL);
+        e = launch_all(L, st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(res->P, d_P, N * (NV + 1) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res->CRLB, d_CR, N * NV * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res->LL, d_LL, N * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
+    hipFree(buf);
+    return rc;
+}
+
+int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_ndim, int fittype, int iterations,
+                            const float *coeff, const size_t *coeff_dims, int coeff_ndim, const float *varmap,
+                            size_t varmap_numel, int silent, const float *initP, size_t initP_numel,
+                            const float *coeff3, float *P, float *CRLB, float *LL)
+{
+    (void)varmap;
+    (void)varmap_numel;
+    g_err.clear();
+    if (!imdims || im_ndim < 2 || im_ndim > 3) return fail(BSF_ERR_ARG, "bsf: imstack must be 2-D or 3-D");
+    if (imdims[0] > BSF_MAX_ROI)
+        return fail(BSF_ERR_ROI_SIZE, "bsf: X,Y dimension of data must be smaller than %d", BSF_MAX_ROI);
+    if (imdims[1] != imdims[0]) return fail(BSF_ERR_SHAPE, "bsf: Fit Box must be square");
+    if (!coeff_dims || coeff_ndim < 1 || coeff_ndim > 5) return fail(BSF_ERR_SHAPE, "bsf: bad coefficient dims");
+    bsf_request r;
+    std::memset(&r, 0, sizeof r);
+    r.fittype = fittype;
+    r.imstack = imstack;
+    r.sz = (int)imdims[0];
+    r.nfits = im_ndim == 2 ? 1 : (int)imdims[2];
+    r.iterations = iterations;
+    r.coeff = coeff;
+    r.coeff_ndim = coeff_ndim;
+    for (int a = 0; a < coeff_ndim; a++) r.coeff_dims[a] = (int)coeff_dims[a];
+    r.coeff3 = coeff3;
+    r.initP = (fittype > 6) ? initP : nullptr;
+    r.n_initP = (int)initP_numel;
+    r.precision = BSF_PREC_MIX;
+    bsf_result res = {P, CRLB, LL};
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = bsf_fit_host(&r, &res);
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == BSF_OK && !silent)
+        std::printf("subregion size: %d fittype: %d\nNumber of fits: %d\nfit time %f s, fits per second %f\n", r.sz,
+                    fittype, r.nfits, dt, dt > 0 ? r.nfits / dt : 0.0);
+    return rc;
+}
+
+}  // extern "C"

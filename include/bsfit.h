@@ -1,0 +1,113 @@
+/*
+ * bsfit.h -- C-ABI of the MI355X (gfx950) multi-D B-spline PSF Poisson-MLE
+ * Levenberg-Marquardt fitter.  Plain pointers and sizes only.
+ *
+ * The entry points replace the reference's MATLAB MEX boundary:
+ *   MultiD_bspline/visual code/GPUmleFit_LM/mexFunction.cpp:208
+ *       [P, CRLB, LogL] = GPUmleFit_LM_multiD(imstack, fittype, iterations,
+ *                                             coeff, varmap, silent, initP, coeff3)
+ *   and its per-ROI kernels / launch wrappers
+ *   MultiD_bspline/visual code/GPUmleFit_LM/wrapper.cu:256-279 (fittypes 6-8).
+ *
+ * Conventions (MATLAB column-major, as the MEX):
+ *   imstack  float [sz, sz, nfits]; element (row ii, col jj) of ROI n at
+ *            n*sz*sz + jj*sz + ii.
+ *   coeff    float bsarray coefficients, dims = data grid + 3 per axis,
+ *            3 axes (fittype 6), 4 (fittype 7) or 5 (fittype 8).
+ *   coeff3   fittype 8 only: 3D coefficients with coeff's first three dims.
+ *   initP    fittype 7: {z, w}; fittype 8: {z, w, u, g2}; one start shared
+ *            by all ROIs (the MEX passes one vector).  NULL selects
+ *            nz/2, nw/2 (, nu/2, g2 = 0.9).
+ *   P        float [nfits, NV+1]: parameters, last column = iterations used.
+ *            fittype 6: x, y, N, bg, z;  7: + w;  8: x, y, N, bg, z, w, u, g2
+ *            (x is the column position theta0 as the reference reports it).
+ *   CRLB     float [nfits, NV]: variances (diagonal of the inverse Fisher).
+ *   LL       float [nfits]: log-likelihood (data-normalised, as the MEX).
+ *
+ * Every function returns a bsf_status; bsf_last_error() describes the last
+ * failure on the calling thread.  Nothing here exits or resets the device.
+ */
+#ifndef BSFIT_H
+#define BSFIT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bsf_status {
+    BSF_OK = 0,
+    BSF_ERR_ARG = 1,        /* NULL pointer, bad count, bad dims        */
+    BSF_ERR_FITTYPE = 2,    /* fittype not in {6, 7, 8}                 */
+    BSF_ERR_ROI_SIZE = 3,   /* sz < 1, sz > 51, or window beyond LDS    */
+    BSF_ERR_SHAPE = 4,      /* non-square ROIs / coeff rank mismatch    */
+    BSF_ERR_NOMEM = 5,      /* device allocation failed                 */
+    BSF_ERR_DEVICE = 6      /* no gfx950 device / HIP runtime error     */
+};
+
+enum bsf_precision {
+    BSF_PREC_MIX = 0,       /* FP32 spline evaluation and pixel sums, FP64 solve and Fisher inverse */
+    BSF_PREC_F64 = 1        /* everything in FP64 */
+};
+
+#define BSF_MAX_ROI 51      /* IMSZBIG, MultiD_bspline/visual code/GPUmleFit_LM/definitions.h:13 */
+
+typedef struct bsf_request {
+    int fittype;            /* 6, 7 or 8 */
+    const float *imstack;   /* [sz, sz, nfits] */
+    int sz;
+    int nfits;
+    int iterations;         /* the MEX default is 50 */
+    const float *coeff;
+    int coeff_ndim;         /* 3, 4 or 5 */
+    int coeff_dims[5];      /* stored dims (data grid + 3) */
+    const float *coeff3;    /* fittype 8 */
+    const float *initP;     /* may be NULL */
+    int n_initP;
+    int precision;          /* enum bsf_precision */
+} bsf_request;
+
+typedef struct bsf_result {
+    float *P;
+    float *CRLB;
+    float *LL;
+} bsf_result;
+
+/* Number of fitted parameters NV for a fittype (5, 6, 8), or -1.
+ * Replaces the MEX's output sizing, mexFunction.cpp:528-586. */
+int bsf_num_params(int fittype);
+
+/* Synchronous fit, host pointers in and out (copies to and from the device).
+ * Replaces mexFunction (GPUmleFit_LM/mexFunction.cpp:208) for fittypes 6-8:
+ * H2D copies (:466-518), the kernel launch (:661-674), D2H copies (:725-734). */
+int bsf_fit_host(const bsf_request *req, bsf_result *res);
+
+/* Asynchronous fit on device-resident buffers: imstack, coeff, coeff3 and the
+ * outputs are device pointers (initP stays a host pointer: a few scalars
+ * read at launch); the work is queued on `stream` (a hipStream_t, NULL = the
+ * null stream) and nothing is copied or allocated.
+ * Replaces the launch wrappers kernel_bsplineMLEFit_z_EMCCD_wrapper,
+ * kernel_bsplineMLEFit_4D_EMCCD_wrapper, kernel_splineMLEFit_5n3D_EMCCD_wrapper
+ * (GPUmleFit_LM/wrapper.cu:256-279). */
+int bsf_fit_device(const bsf_request *req, bsf_result *res, void *stream);
+
+/* The MEX argument list as plain C (GPUmleFit_LM/mexFunction.cpp:14-55):
+ * imdims = size(imstack) (2 dims: one ROI), coeff_dims = size(coeff).
+ * varmap is accepted and ignored as the MEX ignores it for fittypes 6-8
+ * (:363-373, 661-674); silent = 0 prints the fit rate to stdout (:753-760).
+ * P, CRLB and LL are caller-owned host arrays sized as bsf_fit_host's. */
+int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_ndim, int fittype,
+                            int iterations, const float *coeff, const size_t *coeff_dims, int coeff_ndim,
+                            const float *varmap, size_t varmap_numel, int silent, const float *initP,
+                            size_t initP_numel, const float *coeff3, float *P, float *CRLB, float *LL);
+
+/* Text of the last error on this thread ("" if none).
+ * Replaces mexErrMsgIdAndTxt (GPUmleFit_LM/mexFunction.cpp:140-176). */
+const char *bsf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BSFIT_H */
