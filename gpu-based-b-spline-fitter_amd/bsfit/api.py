@@ -149,3 +149,23 @@ def GPUmleFit_LM_multiD(imstack, fittype, iterations=50, coeff=None, varmap=None
     if not silent:
         print("subregion size: %d fittype: %d\nNumber of fits: %d" % (np.shape(imstack)[0], fittype, P.shape[0]))
     return P, CR, LL
+
+
+def test_eval_pass(fittype, imstack, theta, coeff, coeff3=None, precision=PREC_MIX):
+    """Test hook (bsf_test_eval_pass): one deviance / gradient / Hessian
+    evaluation per ROI at theta (N, NV).  Returns err (N,), J (N, NV),
+    H (N, NV, NV)."""
+    lib = load()
+    lib.bsf_test_eval_pass.argtypes = [C.POINTER(Request), C.c_void_p, C.c_void_p]
+    lib.bsf_test_eval_pass.restype = C.c_int
+    imstack = _f32(imstack)
+    S, _, N = imstack.shape
+    coeff = _f32(coeff)
+    c3 = _f32(coeff3) if coeff3 is not None else None
+    NV = _NV[int(fittype)]
+    th = np.ascontiguousarray(theta, np.float32).reshape(N, NV)
+    out = np.zeros((N, 1 + NV + NV * NV), np.float32)
+    req = _request(fittype, S, N, 0, coeff.ctypes.data, coeff.shape,
+                   c3.ctypes.data if c3 is not None else None, None, precision, imstack.ctypes.data)
+    _check(lib.bsf_test_eval_pass(C.byref(req), th.ctypes.data, out.ctypes.data))
+    return out[:, 0], out[:, 1:1 + NV], out[:, 1 + NV:].reshape(N, NV, NV)

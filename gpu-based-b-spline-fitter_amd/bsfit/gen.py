@@ -223,6 +223,61 @@ def psf_dipole3d(nx=21, nz=21, dz=60.0, nphi=24, npol=24):
     return out
 
 
+def phase_axis(nphi=16):
+    """Phase samples of the 4Pi model: phi_k = 2*pi*(k-1)/nphi, k = 0..nphi+1,
+    so the samples repeat with period nphi (w units) and the fitter's wrap
+    (period nw-2) is exact.  Spline coordinate w = 2 + phi*nphi/(2*pi)."""
+    return 2 * math.pi * (np.arange(nphi + 2) - 1) / nphi
+
+
+def psf_4pi(nx=27, nz=61, nphi=16, visibility=0.8):
+    """4Pi interference model sampled on (x, y, z, phi): I(x,y,z) * (1 + v cos phi)
+    with I the astigmatic 3D PSF of psf_astig3d.  Channel k of a 4Pi ROI sees
+    phi + offset_k.  Shape (nx, nx, nz, nphi+2)."""
+    I = psf_astig3d(nx, nz)
+    ph = phase_axis(nphi)
+    return I[:, :, :, None] * (1 + visibility * np.cos(ph))[None, None, None, :]
+
+
+PHASE_OFFSETS = (0.0, math.pi / 2, math.pi, 3 * math.pi / 2)
+
+
+def make_rois_4pi(N, S, seed, coeff, shared=(1, 1, 1, 1, 1, 1), noise=True, iterations=50,
+                  offsets=PHASE_OFFSETS):
+    """Linked 4Pi ROIs: shared x, y, N, z, phi; per-channel background when
+    shared[3] == 0.  imstack (S, S, N, nch) Fortran-order."""
+    rng = np.random.default_rng(seed)
+    nd = [s - 3 for s in coeff.shape]
+    nphi = nd[3] - 2
+    nch = len(offsets)
+    c0 = (S - 1) / 2.0
+    x = c0 + rng.uniform(-1.5, 1.5, N)
+    y = c0 + rng.uniform(-1.5, 1.5, N)
+    photons = np.exp(rng.uniform(math.log(1e3), math.log(1e4), N))
+    bg = rng.uniform(5, 50, (N, nch))
+    if shared[3]:
+        bg[:] = bg[:, :1]
+    zc = 1 + (rng.uniform(-600, 600, N) + 600) / 20.0
+    phi = rng.uniform(0, 2 * math.pi, N)
+    wc = 2 + phi * nphi / (2 * math.pi)
+    shift = np.array([o * nphi / (2 * math.pi) for o in offsets], np.float32)
+    data = np.empty((S, S, N, nch), np.float32, order="F")
+    mu_all = []
+    for k in range(nch):
+        wk = wc + shift[k]
+        wk = np.where(wk > nd[3] - 1, wk - (nd[3] - 2), wk)
+        f = eval_spline_rois(coeff, S, np.stack([x, y], 1), np.stack([zc, wk], 1))
+        mu = bg[:, k, None, None] + photons[:, None, None] * np.maximum(f, 0.0)
+        mu_all.append(mu)
+        d = rng.poisson(mu).astype(np.float32) if noise else mu.astype(np.float32)
+        data[:, :, :, k] = np.transpose(d, (1, 2, 0))
+    truth = dict(x=x, y=y, N=photons, bg=bg, z=zc, w=wc)
+    return Workload(fittype="4pi", imstack=data, coeff=coeff, S=S, truth=truth, shared=np.array(shared, np.int32),
+                    phaseShift=shift, initZ=np.full(N, nd[2] / 2.0, np.float32),
+                    initPhi=np.full(N, 2 + nphi / 2.0, np.float32), iterations=iterations, nch=nch,
+                    mu=np.stack(mu_all, -1))
+
+
 # ------------------------------------------------------------------ configs
 class Workload:
     """One fit call: ROI stack plus everything the fitter needs."""
@@ -246,6 +301,8 @@ def model_coeffs(kind):
             _MODEL_CACHE[kind] = bspline_coeffs(psf_astig3d())
         elif kind == "color4d":
             _MODEL_CACHE[kind] = bspline_coeffs(psf_color4d())
+        elif kind == "4pi":
+            _MODEL_CACHE[kind] = bspline_coeffs(psf_4pi())
         elif kind == "dipole5d":
             _MODEL_CACHE[kind] = (bspline_coeffs(psf_dipole5d()), bspline_coeffs(psf_dipole3d()))
         else:
@@ -314,6 +371,8 @@ def config(idx, N=None, S=None, noise=True, seed=None):
 
     0: 3D astigmatic, 27x27x61 grid, S=13, N=10,000, fittype 6
     1: colour 4D, 27x27x61x3 grid, S=15, N=100,000, fittype 7
+    2: 4Pi, four 13x13 channels (phase offsets 0, pi/2, pi, 3pi/2), 27x27x61x18
+       grid (16 phase samples + 2 wrap samples), N=200,000, all shared (NV=6)
     3: vortex 5D, 21x21x21x17x9 grid (+21^3 averaged model), S in
        {11,13,15,17}, N=100,000, fittype 8 (one Workload per S group)
     """
@@ -322,6 +381,8 @@ def config(idx, N=None, S=None, noise=True, seed=None):
         return make_rois(6, N or 10000, S or 13, seed, model_coeffs("astig3d"), noise=noise)
     if idx == 1:
         return make_rois(7, N or 100000, S or 15, seed, model_coeffs("color4d"), noise=noise)
+    if idx == 2:
+        return make_rois_4pi(N or 200000, S or 13, seed, model_coeffs("4pi"), noise=noise)
     if idx == 3:
         c5, c3 = model_coeffs("dipole5d")
         Ntot = N or 100000

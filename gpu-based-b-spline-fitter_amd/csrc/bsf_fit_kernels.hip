@@ -104,33 +104,6 @@ __device__ __forceinline__ int base_tap(E c, int n)
     return kc + 1;
 }
 
-// ------------------------------------------------------------- reductions
-// Sum V per-lane values across the wave through LDS; every lane gets the
-// totals.  scratch needs (V+1)*65 elements.
-template <typename E, int V>
-__device__ __forceinline__ void wave_sum(E (&v)[V], E *scratch, int lane)
-{
-#pragma unroll
-    for (int k = 0; k < V; k++) scratch[k * 65 + lane] = v[k];
-    __syncthreads();
-    if (lane < V) {
-        E s0 = E(0), s1 = E(0), s2 = E(0), s3 = E(0);
-        const E *row = scratch + lane * 65;
-#pragma unroll 4
-        for (int j = 0; j < 64; j += 4) {
-            s0 += row[j];
-            s1 += row[j + 1];
-            s2 += row[j + 2];
-            s3 += row[j + 3];
-        }
-        scratch[V * 65 + lane] = (s0 + s1) + (s2 + s3);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < V; k++) v[k] = scratch[V * 65 + k];
-    __syncthreads();
-}
-
 // -------------------------------------------------------------- geometry
 // Everything one evaluation needs that depends on the ROI-level spline
 // coordinates (and so on theta): per-axis factors, per-pixel taps, windows.
@@ -148,6 +121,9 @@ struct Dev {
     int N, S, iterations, roi_offset;
     int nd[5];
     long s1, s2, s3, s4;   // coefficient strides of dims 1..4
+    int cbytes, c3bytes;   // coefficient array sizes (buffer record counts)
+    const float *theta_in; // test hook: evaluate once at these parameters
+    float *dbg;            // test hook output: err, J[NV], H[NV*NV] per ROI
     float initP[4];
     int has_initP;
 };
@@ -184,21 +160,35 @@ __device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> 
 }
 
 // ------------------------------------------------------------ stage A
+// Coefficient gathers go through buffer loads: one per-lane byte offset for
+// the window point (VGPR) plus a wave-uniform tap offset (SGPR soffset), so
+// the 4^K taps of a point cost no per-tap address registers; the descriptor's
+// record count also makes any out-of-range tap read 0.
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
 // Contract the non-xy axes over the window; planes[p][b*PR + a].
 template <typename E, int FT>
 __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *planes, int plane_stride, int lane)
 {
     constexpr int K = Ft<FT>::K;
     const int PR = g.PR, npts = g.PR * g.PC;
+    const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void *)A.coeff, (short)0, A.cbytes, 0x00020000);
+    const int s2 = (int)A.s2 * 4, s3 = (int)A.s3 * 4, s4 = (int)A.s4 * 4;
+    int tapb = (g.knx[0] - 2) * s2;
+    if constexpr (K >= 2) tapb += (g.knx[1] - 2) * s3;
+    if constexpr (K >= 3) tapb += (g.knx[2] - 2) * s4;
+    tapb = __builtin_amdgcn_readfirstlane(tapb);
     for (int q = lane; q < npts; q += 64) {
         const int a = q % PR, b = q / PR;
-        const long base = (long)(g.row0 + a) + (long)(g.col0 + b) * A.s1;
+        const int vo = ((g.row0 + a) + (g.col0 + b) * (int)A.s1) * 4;
         if constexpr (K == 1) {
             const Axis<E> &z = g.ax[2];
-            const float *c = A.coeff + base + (long)(g.knx[0] - 2) * A.s2;
             E cv[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) cv[t] = (E)c[t * A.s2];
+            for (int t = 0; t < 4; t++) cv[t] = (E)bload(rc, vo, tapb + t * s2);
             E v = E(0), dz = E(0);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
@@ -209,13 +199,12 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
             planes[plane_stride + q] = dz;
         } else if constexpr (K == 2) {
             const Axis<E> &z = g.ax[2], &w = g.ax[3];
-            const float *c = A.coeff + base + (long)(g.knx[0] - 2) * A.s2 + (long)(g.knx[1] - 2) * A.s3;
             E v0[4], v1[4];
 #pragma unroll
             for (int tz = 0; tz < 4; tz++) {
                 E cv[4];
 #pragma unroll
-                for (int tw = 0; tw < 4; tw++) cv[tw] = (E)c[tz * A.s2 + tw * A.s3];
+                for (int tw = 0; tw < 4; tw++) cv[tw] = (E)bload(rc, vo, tapb + tz * s2 + tw * s3);
                 E s0 = E(0), s1 = E(0);
 #pragma unroll
                 for (int tw = 0; tw < 4; tw++) {
@@ -237,15 +226,13 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
             planes[2 * plane_stride + q] = dw;
         } else {
             const Axis<E> &z = g.ax[2], &w = g.ax[3], &u = g.ax[4];
-            const float *c = A.coeff + base + (long)(g.knx[0] - 2) * A.s2 + (long)(g.knx[1] - 2) * A.s3 +
-                             (long)(g.knx[2] - 2) * A.s4;
             E cv[4][4][4];
 #pragma unroll
             for (int tz = 0; tz < 4; tz++)
 #pragma unroll
                 for (int tw = 0; tw < 4; tw++)
 #pragma unroll
-                    for (int tu = 0; tu < 4; tu++) cv[tz][tw][tu] = (E)c[tz * A.s2 + tw * A.s3 + tu * A.s4];
+                    for (int tu = 0; tu < 4; tu++) cv[tz][tw][tu] = (E)bload(rc, vo, tapb + tz * s2 + tw * s3 + tu * s4);
             E v = E(0), dz = E(0), dw = E(0), du = E(0);
 #pragma unroll
             for (int tz = 0; tz < 4; tz++) {
@@ -307,11 +294,13 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
             planes[4 * plane_stride + q] = zb;
             planes[5 * plane_stride + q] = ub;
             // the 3D model (coeff3) shares the xy window and the z taps
-            const float *c3 = A.coeff3 + base + (long)(g.knx[0] - 2) * A.s2;
+            const __amdgpu_buffer_rsrc_t r3 =
+                __builtin_amdgcn_make_buffer_rsrc((void *)A.coeff3, (short)0, A.c3bytes, 0x00020000);
+            const int tap3 = __builtin_amdgcn_readfirstlane((g.knx[0] - 2) * s2);
             E v3 = E(0), dz3 = E(0);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                E cc = (E)c3[t * A.s2];
+                E cc = (E)bload(r3, vo, tap3 + t * s2);
                 v3 += z.W[t] * cc;
                 dz3 += z.Dv[t] * cc;
             }
@@ -420,88 +409,128 @@ __device__ __forceinline__ void pixel_model(const Dev &A, const Geo<E, FT> &g, c
 }
 
 // ------------------------------------------------------------- LM solve
-// Cholesky of the damped Hessian in FP64 (fails on a negative radicand, as
-// the reference's kernel_cholesky, CPUsplineLib.cpp:1363-1388) and the two
-// triangular solves.  Returns 0 on success.
-template <int NV>
-__device__ __forceinline__ int chol_solve(const double *H, const double *b, double *x)
+// The NV x NV systems are spread over the wave: lane i holds row i, pivots
+// and columns travel by v_readlane (uniform lane index), so the solve costs a
+// few registers per lane instead of NV^2 doubles.
+__device__ __forceinline__ double readlane_d(double v, int l)
 {
-    double U[NV * NV];
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <typename E>
+__device__ __forceinline__ E readlane_e(E v, int l)
+{
+    if constexpr (sizeof(E) == 8) return readlane_d(v, l);
+    else return readlane_f(v, l);
+}
+
+// Damped normal equations H x = J in FP64 by Cholesky (right-looking), the
+// two triangular solves; lane i holds H row i (a[]) and J_i (b).  Fails, as
+// the reference's kernel_cholesky (CPUsplineLib.cpp:1363-1388), when a pivot
+// radicand is negative.  On success every lane returns x_lane in xi.
+template <int NV>
+__device__ __forceinline__ int chol_solve_rows(double (&a)[NV], double b, int lane, double &xi)
+{
+    double colL[NV], rinv[NV];
 #pragma unroll
-    for (int i = 0; i < NV; i++)
+    for (int j = 0; j < NV; j++) colL[j] = 0.0;
 #pragma unroll
-        for (int j = 0; j <= i; j++) {
-            double s = 0;
+    for (int k = 0; k < NV; k++) {
+        const double akk = readlane_d(a[k], k);
+        if (!(akk >= 0.0)) return 1;
+        const double lkk = sqrt(akk);
+        rinv[k] = 1.0 / lkk;
+        const double lik = lane > k ? a[k] * rinv[k] : (lane == k ? lkk : 0.0);
+        a[k] = lik;
 #pragma unroll
-            for (int k = 0; k < j; k++) s += U[i * NV + k] * U[j * NV + k];
-            if (i == j) {
-                double rad = H[i * NV + i] - s;
-                if (!(rad >= 0)) return 1;
-                U[i * NV + i] = sqrt(rad);
-            } else {
-                U[i * NV + j] = (1.0 / U[j * NV + j]) * (H[i * NV + j] - s);
-            }
+        for (int j = k + 1; j < NV; j++) {
+            const double ljk = readlane_d(lik, j);
+            a[j] -= lik * ljk;
+            colL[j] = (lane == k) ? ljk : colL[j];   // lane k keeps column k of L
         }
-    double y[NV];
-#pragma unroll
-    for (int i = 0; i < NV; i++) {
-        double v = b[i];
-#pragma unroll
-        for (int j = 0; j < i; j++) v -= U[i * NV + j] * y[j];
-        y[i] = v / U[i * NV + i];
     }
+    double acc = 0.0, yv = 0.0;
 #pragma unroll
-    for (int i = NV - 1; i >= 0; i--) {
-        double v = y[i];
+    for (int k = 0; k < NV; k++) {                   // L y = b
+        const double yk = readlane_d((b - acc) * rinv[k], k);
+        yv = (lane == k) ? yk : yv;
+        acc += a[k] * yk;
+    }
+    double acc2 = 0.0;
+    xi = 0.0;
 #pragma unroll
-        for (int j = i + 1; j < NV; j++) v -= U[j * NV + i] * x[j];
-        x[i] = v / U[i * NV + i];
+    for (int k = NV - 1; k >= 0; k--) {              // L^T x = y
+        const double xk = readlane_d((yv - acc2) * rinv[k], k);
+        xi = (lane == k) ? xk : xi;
+        acc2 += colL[k] * xk;
     }
     return 0;
 }
 
-// Unpivoted LU inverse diagonal (MatInvLib.h:17-77), FP64.
+// Diagonal of the inverse Fisher matrix by unpivoted Gauss-Jordan elimination
+// over rows held one per lane, in FP64: the reference's kernel_MatInvN
+// (MatInvLib.h:17-77) is the same unpivoted elimination written as LU.
 template <int NV>
-__device__ __forceinline__ void matinv_diag(double *M, double *diag)
+__device__ __forceinline__ double inverse_diag_rows(double (&m)[NV], int lane)
+{
+    double v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) v[j] = (j == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double mkk = readlane_d(m[k], k);
+        const double r = 1.0 / mkk;
+        const double fac = m[k] * r;
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            const double mk = readlane_d(m[j], k), vk = readlane_d(v[j], k);
+            m[j] = (lane == k) ? mk * r : m[j] - fac * mk;
+            v[j] = (lane == k) ? vk * r : v[j] - fac * vk;
+        }
+    }
+    double d = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; j++) d = (j == lane) ? v[j] : d;
+    return d;
+}
+
+// Index of (row i, column j) in the packed upper triangle used by the sums.
+template <int NV>
+__device__ __forceinline__ int packed_index(int i, int j)
+{
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    return a * NV - (a * (a - 1)) / 2 + (b - a);
+}
+
+// Sum V per-lane values across the wave; the totals stay in LDS at
+// scratch[V*65 + v] for every lane to read.
+template <typename E, int V>
+__device__ __forceinline__ void wave_sum_lds(const E (&v)[V], E *scratch, int lane)
 {
 #pragma unroll
-    for (int jj = 0; jj < NV; jj++) {
-#pragma unroll
-        for (int ii = 1; ii <= jj; ii++) {
-            double t = 0;
-#pragma unroll
-            for (int kk = 0; kk < ii; kk++) t += M[ii + kk * NV] * M[kk + jj * NV];
-            M[ii + jj * NV] -= t;
+    for (int k = 0; k < V; k++) scratch[k * 65 + lane] = v[k];
+    __syncthreads();
+    if (lane < V) {
+        E s0 = E(0), s1 = E(0), s2 = E(0), s3 = E(0);
+        const E *row = scratch + lane * 65;
+#pragma unroll 4
+        for (int j = 0; j < 64; j += 4) {
+            s0 += row[j];
+            s1 += row[j + 1];
+            s2 += row[j + 2];
+            s3 += row[j + 3];
         }
-#pragma unroll
-        for (int ii = jj + 1; ii < NV; ii++) {
-            double t = 0;
-#pragma unroll
-            for (int kk = 0; kk < jj; kk++) t += M[ii + kk * NV] * M[kk + jj * NV];
-            M[ii + jj * NV] = (1.0 / M[jj + jj * NV]) * (M[ii + jj * NV] - t);
-        }
+        scratch[V * 65 + lane] = (s0 + s1) + (s2 + s3);
     }
-#pragma unroll
-    for (int num = 0; num < NV; num++) {
-        double yy[NV], xinv[NV];
-        yy[0] = (num == 0) ? 1.0 : 0.0;
-#pragma unroll
-        for (int ii = 1; ii < NV; ii++) {
-            double t = 0;
-#pragma unroll
-            for (int jj = 0; jj < ii; jj++) t += M[ii + jj * NV] * yy[jj];
-            yy[ii] = (ii == num ? 1.0 : 0.0) - t;
-        }
-        xinv[NV - 1] = yy[NV - 1] / M[(NV - 1) + (NV - 1) * NV];
-#pragma unroll
-        for (int ii = NV - 2; ii >= 0; ii--) {
-            double t = 0;
-#pragma unroll
-            for (int jj = ii + 1; jj < NV; jj++) t += M[ii + jj * NV] * xinv[jj];
-            xinv[ii] = (1.0 / M[ii + ii * NV]) * (yy[ii] - t);
-        }
-        diag[num] = xinv[num];
-    }
+    __syncthreads();
 }
 
 template <typename E>
@@ -510,7 +539,7 @@ template <typename E>
 __device__ __forceinline__ E dminr(E a, double b) { return ((double)a < b) ? a : (E)b; }
 
 // ----------------------------------------------------------------- kernel
-template <int FT, typename E, int PPL>
+template <int FT, typename E>
 __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 {
     constexpr int NV = Ft<FT>::NV;
@@ -529,27 +558,13 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
 
     const float *dptr = A.data + (size_t)n * S2;
-    float dv[PPL];
-    int pii[PPL], pjj[PPL];
-#pragma unroll
-    for (int i = 0; i < PPL; i++) {
-        int p = lane + 64 * i;
-        if (p < S2) {
-            dv[i] = dptr[p];
-            s_data[p] = dv[i];
-            pii[i] = p % S;
-            pjj[i] = p / S;
-        } else {
-            dv[i] = 0.f;
-            pii[i] = -1;
-            pjj[i] = 0;
-        }
-    }
+    for (int p = lane; p < S2; p += 64) s_data[p] = dptr[p];
     __syncthreads();
 
     // ---- initial values: centre of mass, max / min (CPUgaussLib.cpp:172-223)
     // summed in the reference's sequential order so they match it bit for bit.
     E th[NV], oth[NV], sgn[NV], osgn[NV], clampv[NV], oclamp[NV], fade[NV], upd[NV];
+    (void)NPL;
     {
         E tx = E(0), ty = E(0), ts = E(0), mx = E(0), mn = E(10e10);
         for (int ii = 0; ii < S; ii++)
@@ -566,7 +581,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         }
         th[0] = tx / ts;
         th[1] = ty / ts;
-        double t = (double)(((mx - mn) * E(2)) * E(3.141592f)) * 1.5 * 1.5;
+        double t = (double)(((mx - mn) * E(2)) * (E)3.141592) * 1.5 * 1.5;   // PI, definitions.h:15
         th[2] = (E)((0.0 > t) ? 0.0 : t);
         th[3] = dmaxr<E>(mn, 0.01);
         if constexpr (FT == 6) {
@@ -587,7 +602,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
                 th[4] = (E)A.nd[2] / E(2);
                 th[5] = (E)A.nd[3] / E(2);
                 th[6] = (E)A.nd[4] / E(2);
-                th[7] = E(0.9f);
+                th[7] = (E)0.9;
             }
         }
     }
@@ -612,29 +627,33 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     E cth[9];
     Geo<E, FT> g;
     E g2 = E(0);
-    E J[NV], H[NV * NV];
+    // lane i (< NV) holds row i of the Hessian and J_i; the reference keeps
+    // the Hessian between iterations (a rejected step reuses it, a failed
+    // Cholesky leaves its damped diagonal), so does this.
+    E hrow[NV], jv = E(0);
+    const int myrow = lane < NV ? lane : NV - 1;
 
-    // one ROI evaluation at th: fills J, H and returns the deviance
+    // one ROI evaluation at th: fills hrow / jv and returns the deviance
     auto evaluate = [&](bool first) __attribute__((always_inline)) -> E {
 #pragma unroll
         for (int i = 0; i < NV; i++) cth[i] = th[i];
         if constexpr (FT == 8) g2 = th[7];
         make_geo<E, FT>(A, cth, g);
+        __syncthreads();   // the scratch of the last reduction is read out
         stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
         __syncthreads();
         E acc[NRED];
 #pragma unroll
         for (int k = 0; k < NRED; k++) acc[k] = E(0);
-#pragma unroll
-        for (int i = 0; i < PPL; i++) {
-            if (pii[i] < 0) continue;
+        for (int p = lane; p < S2; p += 64) {
+            const int pjj = p / S, pii = p - pjj * S;
             E dudt[8], model;
-            pixel_model<E, FT>(A, g, s_pl, plane_stride, pii[i], pjj[i], cth[0], cth[1], th[2], th[3], g2, dudt,
+            pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], th[2], th[3], g2, dudt,
                                model);
             if constexpr (FT == 7) {
                 if (first) dudt[5] = dudt[4];
             }
-            E data = (E)dv[i];
+            E data = (E)s_data[p];
             if (data > E(0)) {
                 acc[0] += E(2) * ((model - data) - data * log(model / data));
             } else {
@@ -652,27 +671,34 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
                 for (int m = l; m < NV; m++) acc[h++] += t2 * dudt[l] * dudt[m];
         }
         __syncthreads();   // planes are dead; the scratch reuses them
-        wave_sum<E, NRED>(acc, s_red, lane);
+        wave_sum_lds<E, NRED>(acc, s_red, lane);
+        const E *tot = s_red + NRED * 65;
+        jv = tot[1 + myrow];
 #pragma unroll
-        for (int l = 0; l < NV; l++) J[l] = acc[1 + l];
-        int h = 1 + NV;
-#pragma unroll
-        for (int l = 0; l < NV; l++)
-#pragma unroll
-            for (int m = l; m < NV; m++) {
-                H[l * NV + m] = acc[h];
-                H[m * NV + l] = acc[h];
-                h++;
-            }
-        return acc[0];
+        for (int j = 0; j < NV; j++) hrow[j] = tot[1 + NV + packed_index<NV>(myrow, j)];
+        return tot[0];
     };
 
+    if (A.theta_in) {
+        // test hook: one evaluation at a given theta; write err, J, H
+#pragma unroll
+        for (int i = 0; i < NV; i++) th[i] = (E)A.theta_in[(size_t)n * NV + i];
+        const E e = evaluate(false);
+        float *o = A.dbg + (size_t)n * (1 + NV + NV * NV);
+        if (lane == 0) o[0] = (float)e;
+        if (lane < NV) {
+            o[1 + lane] = (float)jv;
+#pragma unroll
+            for (int j = 0; j < NV; j++) o[1 + NV + lane * NV + j] = (float)hrow[j];
+        }
+        return;
+    }
     E newErr = evaluate(true);
-    E oldErr = (E)1e13f;
-    E newLambda = E(1), oldLambda = E(1), sumUpdate = (E)1e10f;
+    E oldErr = (E)1e13;
+    E newLambda = E(1), oldLambda = E(1), sumUpdate = (E)1e10;
     int turn = 1, kk;
     for (kk = 0; kk < A.iterations; kk++) {
-        if (fabs((newErr - oldErr) / newErr) < (E)1e-6f || (FT == 8 && sumUpdate < (E)1e-5f)) break;
+        if (fabs((newErr - oldErr) / newErr) < (E)1e-6 || (FT == 8 && sumUpdate < (E)1e-5)) break;   // TOLERANCE, TOLERANCE1
         if ((double)newErr > 1.5 * (double)oldErr) {
 #pragma unroll
             for (int i = 0; i < NV; i++) {
@@ -687,17 +713,16 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             if (newLambda > E(1)) newLambda = (E)((double)newLambda * 0.8);
             else if (newLambda < E(1)) newLambda = E(1);
         }
+        double arow[NV], xi;
 #pragma unroll
-        for (int i = 0; i < NV; i++) H[i * NV + i] = H[i * NV + i] * newLambda;
-        double Hd[NV * NV], Jd[NV], xd[NV];
-#pragma unroll
-        for (int i = 0; i < NV * NV; i++) Hd[i] = (double)H[i];
-#pragma unroll
-        for (int i = 0; i < NV; i++) Jd[i] = (double)J[i];
-        int info = chol_solve<NV>(Hd, Jd, xd);
+        for (int j = 0; j < NV; j++) {
+            hrow[j] = (j == lane) ? hrow[j] * newLambda : hrow[j];
+            arow[j] = (double)hrow[j];
+        }
+        int info = chol_solve_rows<NV>(arow, (double)jv, lane, xi);
         if (info == 0) {
 #pragma unroll
-            for (int l = 0; l < NV; l++) upd[l] = (E)xd[l];
+            for (int l = 0; l < NV; l++) upd[l] = (E)readlane_d(xi, l);
             if constexpr (FT != 6) {
 #pragma unroll
                 for (int l = 0; l < NV; l++)
@@ -772,6 +797,16 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             th[7] = (th[7] > E(0)) ? th[7] : E(0);
         }
         newErr = evaluate(false);
+#ifdef BSF_TRACE
+        if (lane == 0 && A.dbg) {
+            float *o = A.dbg + ((size_t)n * 64 + kk) * (3 + 2 * NV);
+            o[0] = (float)kk;
+            o[1] = (float)newErr;
+            o[2] = (float)newLambda;
+            for (int i = 0; i < NV; i++) o[3 + i] = (float)th[i];
+            for (int i = 0; i < NV; i++) o[3 + NV + i] = (float)upd[i];
+        }
+#endif
     }
 
     // ---- Fisher information, CRLB, log-likelihood.  Fittypes 6/7 evaluate
@@ -782,18 +817,18 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         for (int i = 0; i < NV; i++) cth[i] = th[i];
         make_geo<E, FT>(A, cth, g);
     }
+    __syncthreads();
     stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
     __syncthreads();
     constexpr int NF = 1 + NH;
     E acc[NF];
 #pragma unroll
     for (int k = 0; k < NF; k++) acc[k] = E(0);
-#pragma unroll
-    for (int i = 0; i < PPL; i++) {
-        if (pii[i] < 0) continue;
+    for (int p = lane; p < S2; p += 64) {
+        const int pjj = p / S, pii = p - pjj * S;
         E dudt[8], model;
-        pixel_model<E, FT>(A, g, s_pl, plane_stride, pii[i], pjj[i], cth[0], cth[1], th[2], th[3], g2, dudt, model);
-        E data = (E)dv[i];
+        pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], th[2], th[3], g2, dudt, model);
+        E data = (E)s_data[p];
         int h = 1;
 #pragma unroll
         for (int a = 0; a < NV; a++)
@@ -805,31 +840,23 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         }
     }
     __syncthreads();
-    wave_sum<E, NF>(acc, s_red, lane);
+    wave_sum_lds<E, NF>(acc, s_red, lane);
+    const E *tot = s_red + NF * 65;
+    double mrow[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) mrow[j] = (double)tot[1 + packed_index<NV>(myrow, j)];
+    const double crlb = inverse_diag_rows<NV>(mrow, lane);
+    if (lane < NV) A.CRLB[(size_t)A.N * lane + n] = (float)crlb;
     if (lane == 0) {
-        double M[NV * NV], diag[NV];
-        int h = 1;
 #pragma unroll
-        for (int a = 0; a < NV; a++)
-#pragma unroll
-            for (int b = a; b < NV; b++) {
-                M[a * NV + b] = (double)acc[h];
-                M[b * NV + a] = (double)acc[h];
-                h++;
-            }
-        matinv_diag<NV>(M, diag);
-#pragma unroll
-        for (int a = 0; a < NV; a++) {
-            A.P[(size_t)A.N * a + n] = (float)th[a];
-            A.CRLB[(size_t)A.N * a + n] = (float)diag[a];
-        }
+        for (int a = 0; a < NV; a++) A.P[(size_t)A.N * a + n] = (float)th[a];
         A.P[(size_t)A.N * NV + n] = (float)kk;
-        A.LL[n] = (float)acc[0];
+        A.LL[n] = (float)tot[0];
     }
 }
 
 // ------------------------------------------------------------ launchers
-template <int FT, typename E, int PPL>
+template <int FT, typename E>
 static hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
 {
     const int S = A.S;
@@ -839,7 +866,7 @@ static hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
     size_t pl = (size_t)Ft<FT>::NPL * W * W;
     size_t red = (size_t)(NRED + 1) * 65;
     size_t lds = (((size_t)S * S * 4 + 15) & ~(size_t)15) + sizeof(E) * (pl > red ? pl : red);
-    hipLaunchKernelGGL((fit_kernel<FT, E, PPL>), dim3(nblocks), dim3(64), lds, st, A);
+    hipLaunchKernelGGL((fit_kernel<FT, E>), dim3(nblocks), dim3(64), lds, st, A);
     return hipGetLastError();
 }
 
@@ -855,10 +882,7 @@ size_t lds_bytes(int fittype, int f64, int S)
 template <int FT, typename E>
 static hipError_t launch_ft(const Dev &A, int nblocks, hipStream_t st)
 {
-    const int S2 = A.S * A.S;
-    if (S2 <= 64 * 5) return launch_t<FT, E, 5>(A, nblocks, st);
-    if (S2 <= 64 * 16) return launch_t<FT, E, 16>(A, nblocks, st);
-    return launch_t<FT, E, 41>(A, nblocks, st);
+    return launch_t<FT, E>(A, nblocks, st);
 }
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
@@ -879,8 +903,16 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     A.s2 = A.s1 * L.cdims[1];
     A.s3 = A.s2 * (L.ndim > 2 ? L.cdims[2] : 1);
     A.s4 = A.s3 * (L.ndim > 3 ? L.cdims[3] : 1);
+    {
+        long nc = 1;
+        for (int a = 0; a < L.ndim; a++) nc *= L.cdims[a];
+        A.cbytes = (int)(nc * 4);
+        A.c3bytes = (int)(A.s3 * 4);
+    }
     for (int i = 0; i < 4; i++) A.initP[i] = L.initP[i];
     A.has_initP = L.has_initP;
+    A.theta_in = L.theta_in;
+    A.dbg = L.dbg;
     const int nb = L.count;
     switch (L.fittype * 2 + (L.f64 ? 1 : 0)) {
     case 12: return launch_ft<6, float>(A, nb, st);

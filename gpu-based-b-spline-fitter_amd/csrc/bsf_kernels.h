@@ -18,6 +18,8 @@ struct FitLaunch {
     int cdims[5], ndim;
     float initP[4];
     int has_initP;
+    const float *theta_in;  // test hook (device): NV parameters per ROI, or NULL
+    float *dbg;             // test hook (device): err, J, H per ROI
 };
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st);

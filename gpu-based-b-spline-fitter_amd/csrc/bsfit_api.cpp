@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <cstdlib>
 
 #include "../../include/bsfit.h"
 #include "bsf_kernels.h"
@@ -177,16 +178,36 @@ int bsf_fit_host(const bsf_request *r, bsf_result *res)
     e = hipMemcpyAsync(d_data, r->imstack, bytes_data, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_c, r->coeff, bytes_c, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && nc3) e = hipMemcpyAsync(d_c3, r->coeff3, bytes_c3, hipMemcpyHostToDevice, st);
+#ifdef BSF_TRACE
+    float *d_tr = nullptr;
+    const size_t ntr = N * 64 * (3 + 2 * NV);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_tr, ntr * 4);
+    if (e == hipSuccess) e = hipMemset(d_tr, 0, ntr * 4);
+#endif
     if (e == hipSuccess) {
         bsf::FitLaunch L = make_launch(r, d_data, d_c, nc3 ? d_c3 : nullptr, d_P, d_CR, d_LL);
+#ifdef BSF_TRACE
+        L.dbg = d_tr;
+#endif
         e = launch_all(L, st);
     }
+#ifdef BSF_TRACE
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && getenv("BSF_TRACE_FILE")) {
+        float *h = (float *)malloc(ntr * 4);
+        e = hipMemcpy(h, d_tr, ntr * 4, hipMemcpyDeviceToHost);
+        FILE *f = fopen(getenv("BSF_TRACE_FILE"), "wb");
+        if (f) { fwrite(h, 4, ntr, f); fclose(f); }
+        free(h);
+    }
+    (void)hipFree(d_tr);
+#endif
     if (e == hipSuccess) e = hipMemcpyAsync(res->P, d_P, N * (NV + 1) * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->CRLB, d_CR, N * NV * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->LL, d_LL, N * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
-    hipFree(buf);
+    (void)hipFree(buf);
     return rc;
 }
 
@@ -224,6 +245,52 @@ int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_n
     if (rc == BSF_OK && !silent)
         std::printf("subregion size: %d fittype: %d\nNumber of fits: %d\nfit time %f s, fits per second %f\n", r.sz,
                     fittype, r.nfits, dt, dt > 0 ? r.nfits / dt : 0.0);
+    return rc;
+}
+
+// Test hook, not part of include/bsfit.h: one deviance / gradient / Hessian
+// evaluation per ROI at the given parameters (theta: host [nfits][NV]);
+// out (host): [nfits][1 + NV + NV*NV] = err, J, H.  Lets tests compare the
+// GPU's model, derivatives and pixel sums with the oracle's directly.
+int bsf_test_eval_pass(const bsf_request *r, const float *theta, float *out)
+{
+    g_err.clear();
+    float dummy = 0.f;
+    bsf_result res = {&dummy, &dummy, &dummy};
+    int rc = validate(r, &res);
+    if (rc) return rc;
+    if ((rc = check_device(r->sz, r->fittype, r->precision == BSF_PREC_F64))) return rc;
+    if (r->nfits == 0) return BSF_OK;
+    const int D = dim_of(r->fittype), NV = nv_of(r->fittype);
+    const size_t N = (size_t)r->nfits, S2 = (size_t)r->sz * r->sz;
+    const size_t nc = coeff_numel(r, D);
+    const size_t nc3 = r->fittype == 8 ? (size_t)r->coeff_dims[0] * r->coeff_dims[1] * r->coeff_dims[2] : 0;
+    const size_t no = N * (1 + NV + NV * NV);
+    float *d_data = nullptr, *d_c = nullptr, *d_c3 = nullptr, *d_th = nullptr, *d_o = nullptr, *d_out = nullptr;
+    hipError_t e = hipMalloc((void **)&d_data, N * S2 * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_c, nc * 4);
+    if (e == hipSuccess && nc3) e = hipMalloc((void **)&d_c3, nc3 * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_th, N * NV * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_o, no * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_out, N * (2 * NV + 2) * 4);
+    if (e == hipSuccess) e = hipMemcpy(d_data, r->imstack, N * S2 * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_c, r->coeff, nc * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nc3) e = hipMemcpy(d_c3, r->coeff3, nc3 * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_th, theta, N * NV * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        bsf::FitLaunch L = make_launch(r, d_data, d_c, d_c3, d_out, d_out + N * (NV + 1), d_out + N * (2 * NV + 1));
+        L.theta_in = d_th;
+        L.dbg = d_o;
+        e = launch_all(L, nullptr);
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d_o, no * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
+    (void)hipFree(d_data);
+    (void)hipFree(d_c);
+    (void)hipFree(d_c3);
+    (void)hipFree(d_th);
+    (void)hipFree(d_o);
+    (void)hipFree(d_out);
     return rc;
 }
 

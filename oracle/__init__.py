@@ -17,7 +17,7 @@ is not buildable here.
 """
 from .loader import (  # noqa: F401
     available, build, fit, fit_f64, bspline, delta_tables, deriv_eval,
-    cholesky, lu_eval, matinv, com2d, maxmin2d, eval_pass,
+    cholesky, lu_eval, matinv, com2d, maxmin2d, eval_pass, fit_4pi, nv_4pi,
     ref_available, ref_bspline, ref_delta_tables, ref_deriv_eval,
     ref_cholesky, ref_lu_eval, ref_matinv, NV_OF, D_OF,
 )

