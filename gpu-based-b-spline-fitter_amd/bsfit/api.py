@@ -169,3 +169,79 @@ def test_eval_pass(fittype, imstack, theta, coeff, coeff3=None, precision=PREC_M
                    c3.ctypes.data if c3 is not None else None, None, precision, imstack.ctypes.data)
     _check(lib.bsf_test_eval_pass(C.byref(req), th.ctypes.data, out.ctypes.data))
     return out[:, 0], out[:, 1:1 + NV], out[:, 1 + NV:].reshape(N, NV, NV)
+
+
+class Request4Pi(C.Structure):
+    _fields_ = [
+        ("imstack", C.c_void_p),
+        ("sz", C.c_int),
+        ("nfits", C.c_int),
+        ("nch", C.c_int),
+        ("shared", C.c_void_p),
+        ("iterations", C.c_int),
+        ("coeff", C.c_void_p),
+        ("coeff_dims", C.c_int * 4),
+        ("dT", C.c_void_p),
+        ("initZ", C.c_void_p),
+        ("initPhi", C.c_void_p),
+        ("phaseShift", C.c_void_p),
+        ("precision", C.c_int),
+    ]
+
+
+def num_params_4pi(shared, nch=4):
+    lib = load()
+    sh = np.ascontiguousarray(shared, np.uint32)
+    lib.bsf_num_params_4pi.argtypes = [C.c_void_p, C.c_int]
+    return lib.bsf_num_params_4pi(sh.ctypes.data, int(nch))
+
+
+def fit_4pi(imstack, shared, iterations, coeff, phaseShift, dT=None, initZ=None, initPhi=None,
+            precision=PREC_MIX):
+    """4Pi multi-channel fit on host arrays: imstack (S, S, N, nch) Fortran
+    order, shared (6,), coeff 4D, phaseShift (nch,) in w units, optional
+    dT (6, nch, N), initZ (N,), initPhi (N,).  Returns P (N, NV+1),
+    CRLB (N, NV), LL (N,) -- GPUmleFit_LM_4Pi_bspline's outputs."""
+    lib = load()
+    lib.bsf_fit_4pi_host.argtypes = [C.POINTER(Request4Pi), C.POINTER(Result)]
+    lib.bsf_fit_4pi_host.restype = C.c_int
+    im = _f32(imstack)
+    if im.ndim != 4:
+        raise BsfError(1, "4Pi imstack must be (S, S, N, nch)")
+    S, S2, N, nch = im.shape
+    if S != S2:
+        raise BsfError(4, "Fit Box must be square")
+    sh = np.ascontiguousarray(shared, np.uint32)
+    coeff = _f32(coeff)
+    ps = np.ascontiguousarray(phaseShift, np.float32)
+    dT_ = _f32(dT) if dT is not None else None
+    iz = np.ascontiguousarray(initZ, np.float32) if initZ is not None else None
+    ip = np.ascontiguousarray(initPhi, np.float32) if initPhi is not None else None
+    NV = num_params_4pi(sh, nch)
+    if NV < 0:
+        raise BsfError(1, "bad shared mask or channel count")
+    P = np.zeros((N, NV + 1), np.float32, order="F")
+    CR = np.zeros((N, NV), np.float32, order="F")
+    LL = np.zeros(N, np.float32)
+    r = Request4Pi()
+    r.imstack = im.ctypes.data
+    r.sz, r.nfits, r.nch = int(S), int(N), int(nch)
+    r.shared = sh.ctypes.data
+    r.iterations = int(iterations)
+    r.coeff = coeff.ctypes.data
+    for a in range(4):
+        r.coeff_dims[a] = int(coeff.shape[a])
+    r.dT = dT_.ctypes.data if dT_ is not None else None
+    r.initZ = iz.ctypes.data if iz is not None else None
+    r.initPhi = ip.ctypes.data if ip is not None else None
+    r.phaseShift = ps.ctypes.data
+    r.precision = int(precision)
+    res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
+    _check(lib.bsf_fit_4pi_host(C.byref(r), C.byref(res)))
+    return P, CR, LL
+
+
+def GPUmleFit_LM_4Pi_bspline(imstack, shared, iterations, coeffs, dT=None, initZ=None, initPhi=None,
+                             phaseShift=None):
+    """The reference 4Pi demo's call surface (demo_4pi/bspline_based_4piloc_v4.m:196-206)."""
+    return fit_4pi(imstack, shared, iterations, coeffs, phaseShift, dT=dT, initZ=initZ, initPhi=initPhi)

@@ -1,0 +1,602 @@
+// bsf_device.h -- gfx950 device building blocks shared by the fit kernels:
+// B-spline basis and per-axis stencils, the ROI window geometry, the
+// non-xy contraction (stage A), the per-pixel model (stage B), wave
+// reductions and the lane-distributed FP64 solves.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bsf {
+
+template <int FT>
+struct Ft;
+template <>
+struct Ft<6> { static constexpr int D = 3, K = 1, NV = 5, NPL = 2; };
+template <>
+struct Ft<7> { static constexpr int D = 4, K = 2, NV = 6, NPL = 3; };
+// planes: value, dz, dw, du, z-irregular, u-irregular, 3D value, 3D dz
+template <>
+struct Ft<8> { static constexpr int D = 5, K = 3, NV = 8, NPL = 8; };
+
+// ---------------------------------------------------------------- basis
+// Centred cubic / quadratic B-spline, the reference's piecewise formulas in
+// the same operation order (CPUsplineLib.cpp:1289-1326).
+template <typename E>
+__device__ __forceinline__ E bsp3(E x)
+{
+    const E c43 = E(4) / E(3), c23 = E(2) / E(3), c16 = E(1) / E(6);
+    E x2 = x * x, x3 = x2 * x, b = E(0);
+    if (x >= E(1) && x < E(2)) b = c43 - E(2) * x + x2 - c16 * x3;
+    else if (x >= E(0) && x < E(1)) b = c23 - x2 + E(0.5) * x3;
+    else if (x >= E(-1) && x < E(0)) b = c23 - x2 - E(0.5) * x3;
+    else if (x >= E(-2) && x < E(-1)) b = c43 + E(2) * x + x2 + c16 * x3;
+    return b;
+}
+
+template <typename E>
+__device__ __forceinline__ E bsp2(E x)
+{
+    E x2 = x * x, b = E(0);
+    if (x >= E(0.5) && x < E(1.5)) b = E(1.125) - E(1.5) * x + E(0.5) * x2;
+    else if (x >= E(-0.5) && x < E(0.5)) b = E(0.75) - x2;
+    else if (x >= E(-1.5) && x < E(-0.5)) b = (E)((double)E(1.125) + 1.5 * (double)x + (double)(E(0.5) * x2));
+    return b;
+}
+
+// Per-axis factors for fractional part d (the reference's B1/B2/dB1/dB2 at
+// idx 1, 2) and the 4-tap stencils over taps k-2..k+1:
+//   W  = value weights, Dv = derivative weights (forward differences with
+//   quadratic weights rewritten as a stencil on the coefficients).
+template <typename E>
+struct Axis {
+    E B1[3], B2[3], dB1[3], dB2[3];
+    E W[4], Dv[4];
+};
+
+template <typename E>
+__device__ __forceinline__ void axis_factors(E d, Axis<E> &a)
+{
+    E dd = d - E(1);
+#pragma unroll
+    for (int idx = 1; idx <= 2; idx++) {
+        a.B1[idx] = bsp3<E>(d + (E)idx - E(1));
+        a.B2[idx] = bsp3<E>(d - (E)idx);
+        a.dB1[idx] = bsp2<E>(dd + (E)idx - E(0.5));
+        a.dB2[idx] = bsp2<E>(dd - (E)idx + E(0.5));
+    }
+    a.W[0] = a.B1[2];
+    a.W[1] = a.B1[1];
+    a.W[2] = a.B2[1];
+    a.W[3] = a.B2[2];
+    // sum_t (c[t+1]-c[t]) dW[t], dW = (dB1[2], dB1[1], dB2[1], dB2[2]);
+    // dB2[2] = beta2(d - 2.5) = 0 for d in [0,1), so tap k+2 drops out.
+    a.Dv[0] = -a.dB1[2];
+    a.Dv[1] = a.dB1[2] - a.dB1[1];
+    a.Dv[2] = a.dB1[1] - a.dB2[1];
+    a.Dv[3] = a.dB2[1] - a.dB2[2];
+}
+
+// Clamped base tap index for one coordinate (CPUsplineLib.cpp:1203-1225):
+// floor, clamp to [1, n-1], coordinate == n selects n; returns k = kc + 1.
+template <typename E>
+__device__ __forceinline__ int base_tap(E c, int n)
+{
+    int kc = (int)floor(c);
+    kc = kc < 1 ? 1 : kc;
+    kc = kc > n - 1 ? n - 1 : kc;
+    if (c == (E)n) kc = n;
+    return kc + 1;
+}
+
+// -------------------------------------------------------------- geometry
+// Everything one evaluation needs that depends on the ROI-level spline
+// coordinates (and so on theta): per-axis factors, per-pixel taps, windows.
+template <typename E, int FT>
+struct Geo {
+    Axis<E> ax[5];     // 0: row (dim 0), 1: column (dim 1), 2..: z, w, u
+    int knx[3];        // base taps of the non-xy axes
+    int row0, col0;    // window origin (0-based coefficient index)
+    int PR, PC;        // window extent
+};
+
+struct Dev {
+    const float *data, *coeff, *coeff3;
+    float *P, *CRLB, *LL;
+    int N, S, iterations, roi_offset;
+    int nd[5];
+    long s1, s2, s3, s4;   // coefficient strides of dims 1..4
+    int cbytes, c3bytes;   // coefficient array sizes (buffer record counts)
+    const float *theta_in; // test hook: evaluate once at these parameters
+    float *dbg;            // test hook output: err, J[NV], H[NV*NV] per ROI
+    float initP[4];
+    int has_initP;
+};
+
+template <typename E, int FT>
+__device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> &g)
+{
+    constexpr int D = Ft<FT>::D;
+    const int S = A.S;
+    const E off = (E)(A.nd[0] - S) / E(2);
+    const E colc = (E)((S + 1) / 2) - cth[0];
+    const E rowc = (E)((S + 1) / 2) - cth[1];
+    axis_factors<E>(rowc - floor(rowc), g.ax[0]);
+    axis_factors<E>(colc - floor(colc), g.ax[1]);
+#pragma unroll
+    for (int a = 2; a < D; a++) {
+        E c = cth[2 + a];
+        axis_factors<E>(c - floor(c), g.ax[a]);
+        g.knx[a - 2] = base_tap<E>(c, A.nd[a]);
+    }
+    int rmin = 1 << 30, rmax = -1, cmin = 1 << 30, cmax = -1;
+    for (int i = 0; i < S; i++) {
+        int kr = base_tap<E>(rowc + (E)i + off, A.nd[1]);
+        int kq = base_tap<E>(colc + (E)i + off, A.nd[0]);
+        rmin = min(rmin, kr);
+        rmax = max(rmax, kr);
+        cmin = min(cmin, kq);
+        cmax = max(cmax, kq);
+    }
+    g.row0 = rmin - 2;
+    g.col0 = cmin - 2;
+    g.PR = rmax - rmin + 4;
+    g.PC = cmax - cmin + 4;
+}
+
+// ------------------------------------------------------------ stage A
+// Coefficient gathers go through buffer loads: one per-lane byte offset for
+// the window point (VGPR) plus a wave-uniform tap offset (SGPR soffset), so
+// the 4^K taps of a point cost no per-tap address registers; the descriptor's
+// record count also makes any out-of-range tap read 0.
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// Contract the non-xy axes over the window; planes[p][b*PR + a].
+template <typename E, int FT>
+__device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *planes, int plane_stride, int lane)
+{
+    constexpr int K = Ft<FT>::K;
+    const int PR = g.PR, npts = g.PR * g.PC;
+    const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void *)A.coeff, (short)0, A.cbytes, 0x00020000);
+    const int s2 = (int)A.s2 * 4, s3 = (int)A.s3 * 4, s4 = (int)A.s4 * 4;
+    int tapb = (g.knx[0] - 2) * s2;
+    if constexpr (K >= 2) tapb += (g.knx[1] - 2) * s3;
+    if constexpr (K >= 3) tapb += (g.knx[2] - 2) * s4;
+    tapb = __builtin_amdgcn_readfirstlane(tapb);
+    for (int q = lane; q < npts; q += 64) {
+        const int a = q % PR, b = q / PR;
+        const int vo = ((g.row0 + a) + (g.col0 + b) * (int)A.s1) * 4;
+        if constexpr (K == 1) {
+            const Axis<E> &z = g.ax[2];
+            E cv[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) cv[t] = (E)bload(rc, vo, tapb + t * s2);
+            E v = E(0), dz = E(0);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                v += z.W[t] * cv[t];
+                dz += z.Dv[t] * cv[t];
+            }
+            planes[q] = v;
+            planes[plane_stride + q] = dz;
+        } else if constexpr (K == 2) {
+            const Axis<E> &z = g.ax[2], &w = g.ax[3];
+            E v0[4], v1[4];
+#pragma unroll
+            for (int tz = 0; tz < 4; tz++) {
+                E cv[4];
+#pragma unroll
+                for (int tw = 0; tw < 4; tw++) cv[tw] = (E)bload(rc, vo, tapb + tz * s2 + tw * s3);
+                E s0 = E(0), s1 = E(0);
+#pragma unroll
+                for (int tw = 0; tw < 4; tw++) {
+                    s0 += w.W[tw] * cv[tw];
+                    s1 += w.Dv[tw] * cv[tw];
+                }
+                v0[tz] = s0;
+                v1[tz] = s1;
+            }
+            E v = E(0), dz = E(0), dw = E(0);
+#pragma unroll
+            for (int tz = 0; tz < 4; tz++) {
+                v += z.W[tz] * v0[tz];
+                dz += z.Dv[tz] * v0[tz];
+                dw += z.W[tz] * v1[tz];
+            }
+            planes[q] = v;
+            planes[plane_stride + q] = dz;
+            planes[2 * plane_stride + q] = dw;
+        } else {
+            const Axis<E> &z = g.ax[2], &w = g.ax[3], &u = g.ax[4];
+            E cv[4][4][4];
+#pragma unroll
+            for (int tz = 0; tz < 4; tz++)
+#pragma unroll
+                for (int tw = 0; tw < 4; tw++)
+#pragma unroll
+                    for (int tu = 0; tu < 4; tu++) cv[tz][tw][tu] = (E)bload(rc, vo, tapb + tz * s2 + tw * s3 + tu * s4);
+            E v = E(0), dz = E(0), dw = E(0), du = E(0);
+#pragma unroll
+            for (int tz = 0; tz < 4; tz++) {
+                E a00 = E(0), a01 = E(0), a10 = E(0);
+#pragma unroll
+                for (int tw = 0; tw < 4; tw++) {
+                    E u0 = E(0), u1 = E(0);
+#pragma unroll
+                    for (int tu = 0; tu < 4; tu++) {
+                        u0 += u.W[tu] * cv[tz][tw][tu];
+                        u1 += u.Dv[tu] * cv[tz][tw][tu];
+                    }
+                    a00 += w.W[tw] * u0;
+                    a01 += w.Dv[tw] * u0;
+                    a10 += w.W[tw] * u1;
+                }
+                v += z.W[tz] * a00;
+                dz += z.Dv[tz] * a00;
+                dw += z.W[tz] * a01;
+                du += z.W[tz] * a10;
+            }
+            // The reference's 5D contraction has two irregular terms in
+            // both its CPU and GPU code (CPUsplineLib.cpp:313 / 420-421,
+            // GPUsplineLib.cuh:573 / 681); they are reproduced as two extra
+            // planes applied later with the upper / lower 2x2 xy stencils.
+            //  z: term 27 of block (q,r,s) weighs the z-difference at tap
+            //     kz-q (w tap kw+r-1, u tap ku+s-1) with a weight taken from
+            //     block +8, term 15 (past the z table: the w table).
+            E zb = E(0);
+#pragma unroll
+            for (int q2 = 1; q2 <= 2; q2++)
+#pragma unroll
+                for (int r = 1; r <= 2; r++)
+#pragma unroll
+                    for (int s = 1; s <= 2; s++) {
+                        E bug;
+                        if (r == 1) bug = z.dB2[q2] * w.B2[2] * u.B1[s];
+                        else if (s == 1) bug = z.dB2[q2] * w.B2[1] * u.B1[2];
+                        else bug = z.B2[q2] * w.dB2[1] * u.B1[1];
+                        E ok = z.dB1[q2] * w.B2[r] * u.B2[s];
+                        const int tz = 2 - q2, tw = 1 + r, tu = 1 + s;
+                        zb += (bug - ok) * (cv[tz + 1][tw][tu] - cv[tz][tw][tu]);
+                    }
+            //  u: term 16 differences u tap ku-s instead of ku+s-1; with
+            //     dB2[2] = 0 only s = 1 survives: weight * (2c[ku]-c[ku-1]-c[ku+1]).
+            E ub = E(0);
+#pragma unroll
+            for (int q2 = 1; q2 <= 2; q2++)
+#pragma unroll
+                for (int r = 1; r <= 2; r++) {
+                    const int tz = 2 - q2, tw = 2 - r;
+                    ub += z.B1[q2] * w.B1[r] * (E(2) * cv[tz][tw][2] - cv[tz][tw][1] - cv[tz][tw][3]);
+                }
+            ub *= u.dB2[1];
+            planes[q] = v;
+            planes[plane_stride + q] = dz;
+            planes[2 * plane_stride + q] = dw;
+            planes[3 * plane_stride + q] = du;
+            planes[4 * plane_stride + q] = zb;
+            planes[5 * plane_stride + q] = ub;
+            // the 3D model (coeff3) shares the xy window and the z taps
+            const __amdgpu_buffer_rsrc_t r3 =
+                __builtin_amdgcn_make_buffer_rsrc((void *)A.coeff3, (short)0, A.c3bytes, 0x00020000);
+            const int tap3 = __builtin_amdgcn_readfirstlane((g.knx[0] - 2) * s2);
+            E v3 = E(0), dz3 = E(0);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                E cc = (E)bload(r3, vo, tap3 + t * s2);
+                v3 += z.W[t] * cc;
+                dz3 += z.Dv[t] * cc;
+            }
+            planes[6 * plane_stride + q] = v3;
+            planes[7 * plane_stride + q] = dz3;
+        }
+    }
+}
+
+// --------------------------------------------------------- stage B + C
+// Model and dudt at one pixel from the planes.
+template <typename E, int FT>
+__device__ __forceinline__ void pixel_model(const Dev &A, const Geo<E, FT> &g, const E *planes, int plane_stride,
+                                            int ii, int jj, E cth0, E cth1, E Nph, E bg, E g2, E *dudt, E &model)
+{
+    constexpr int D = Ft<FT>::D;
+    const int S = A.S;
+    const E off = (E)(A.nd[0] - S) / E(2);
+    const E colc = (E)((S + 1) / 2) - cth0;
+    const E rowc = (E)((S + 1) / 2) - cth1;
+    const int r = base_tap<E>(rowc + (E)ii + off, A.nd[1]) - 2 - g.row0;
+    const int c = base_tap<E>(colc + (E)jj + off, A.nd[0]) - 2 - g.col0;
+    const int PR = g.PR;
+    const Axis<E> &ay = g.ax[0], &ax = g.ax[1];
+    const E *p0 = planes + c * PR + r;
+    // value plane: f, d/dcol, d/drow
+    E f = E(0), dcol = E(0), drow = E(0);
+#pragma unroll
+    for (int ty = 0; ty < 4; ty++) {
+        E sw = E(0), sd = E(0);
+#pragma unroll
+        for (int tx = 0; tx < 4; tx++) {
+            E gv = p0[tx * PR + ty];
+            sw += ax.W[tx] * gv;
+            sd += ax.Dv[tx] * gv;
+        }
+        f += ay.W[ty] * sw;
+        dcol += ay.W[ty] * sd;
+        drow += ay.Dv[ty] * sw;
+    }
+    E dn[3] = {E(0), E(0), E(0)};
+#pragma unroll
+    for (int k = 0; k < D - 2; k++) {
+        const E *pk = p0 + (k + 1) * plane_stride;
+        E s = E(0);
+#pragma unroll
+        for (int ty = 0; ty < 4; ty++) {
+            E sw = E(0);
+#pragma unroll
+            for (int tx = 0; tx < 4; tx++) sw += ax.W[tx] * pk[tx * PR + ty];
+            s += ay.W[ty] * sw;
+        }
+        dn[k] = s;
+    }
+    if constexpr (FT == 8) {
+        const E *pz = p0 + 4 * plane_stride, *pu = p0 + 5 * plane_stride;
+        E zb = E(0), ub = E(0);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                zb += ay.W[2 + i] * ax.W[2 + j] * pz[(2 + j) * PR + 2 + i];
+                ub += ay.W[i] * ax.W[j] * pu[j * PR + i];
+            }
+        dn[0] += zb;
+        dn[2] += ub;
+    }
+    dudt[0] = E(-1) * Nph * dcol;
+    dudt[1] = E(-1) * Nph * drow;
+    dudt[2] = f;
+    dudt[3] = E(1);
+#pragma unroll
+    for (int k = 0; k < D - 2; k++) dudt[4 + k] = Nph * dn[k];
+    model = bg + Nph * f;
+    if constexpr (FT == 8) {
+        const E *q0 = p0 + 6 * plane_stride, *q1 = p0 + 7 * plane_stride;
+        E f3 = E(0), dc3 = E(0), dr3 = E(0), dz3 = E(0);
+#pragma unroll
+        for (int ty = 0; ty < 4; ty++) {
+            E sw = E(0), sd = E(0), sz = E(0);
+#pragma unroll
+            for (int tx = 0; tx < 4; tx++) {
+                E gv = q0[tx * PR + ty];
+                sw += ax.W[tx] * gv;
+                sd += ax.Dv[tx] * gv;
+                sz += ax.W[tx] * q1[tx * PR + ty];
+            }
+            f3 += ay.W[ty] * sw;
+            dc3 += ay.W[ty] * sd;
+            dr3 += ay.Dv[ty] * sw;
+            dz3 += ay.W[ty] * sz;
+        }
+        E d3[5];
+        d3[0] = E(-1) * Nph * dc3;
+        d3[1] = E(-1) * Nph * dr3;
+        d3[2] = f3;
+        d3[3] = E(1);
+        d3[4] = Nph * dz3;
+        E m3 = bg + Nph * f3;
+        E m5 = model;
+#pragma unroll
+        for (int l = 0; l < 5; l++) dudt[l] = dudt[l] * g2 + (E(1) - g2) * d3[l];
+        model = m3 * (E(1) - g2) + m5 * g2;
+        dudt[7] = m5 - m3;
+    }
+}
+
+// ------------------------------------------------------------- LM solve
+// The NV x NV systems are spread over the wave: lane i holds row i, pivots
+// and columns travel by v_readlane (uniform lane index), so the solve costs a
+// few registers per lane instead of NV^2 doubles.
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <typename E>
+__device__ __forceinline__ E readlane_e(E v, int l)
+{
+    if constexpr (sizeof(E) == 8) return readlane_d(v, l);
+    else return readlane_f(v, l);
+}
+
+// Damped normal equations H x = J in FP64 by Cholesky (right-looking), the
+// two triangular solves; lane i holds H row i (a[]) and J_i (b).  Fails, as
+// the reference's kernel_cholesky (CPUsplineLib.cpp:1363-1388), when a pivot
+// radicand is negative.  On success every lane returns x_lane in xi.
+template <int NV>
+__device__ __forceinline__ int chol_solve_rows(double (&a)[NV], double b, int lane, double &xi)
+{
+    double colL[NV], rinv[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) colL[j] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double akk = readlane_d(a[k], k);
+        if (!(akk >= 0.0)) return 1;
+        const double lkk = sqrt(akk);
+        rinv[k] = 1.0 / lkk;
+        const double lik = lane > k ? a[k] * rinv[k] : (lane == k ? lkk : 0.0);
+        a[k] = lik;
+#pragma unroll
+        for (int j = k + 1; j < NV; j++) {
+            const double ljk = readlane_d(lik, j);
+            a[j] -= lik * ljk;
+            colL[j] = (lane == k) ? ljk : colL[j];   // lane k keeps column k of L
+        }
+    }
+    double acc = 0.0, yv = 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {                   // L y = b
+        const double yk = readlane_d((b - acc) * rinv[k], k);
+        yv = (lane == k) ? yk : yv;
+        acc += a[k] * yk;
+    }
+    double acc2 = 0.0;
+    xi = 0.0;
+#pragma unroll
+    for (int k = NV - 1; k >= 0; k--) {              // L^T x = y
+        const double xk = readlane_d((yv - acc2) * rinv[k], k);
+        xi = (lane == k) ? xk : xi;
+        acc2 += colL[k] * xk;
+    }
+    return 0;
+}
+
+// Diagonal of the inverse Fisher matrix by unpivoted Gauss-Jordan elimination
+// over rows held one per lane, in FP64: the reference's kernel_MatInvN
+// (MatInvLib.h:17-77) is the same unpivoted elimination written as LU.
+template <int NV>
+__device__ __forceinline__ double inverse_diag_rows(double (&m)[NV], int lane)
+{
+    double v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) v[j] = (j == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double mkk = readlane_d(m[k], k);
+        const double r = 1.0 / mkk;
+        const double fac = m[k] * r;
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            const double mk = readlane_d(m[j], k), vk = readlane_d(v[j], k);
+            m[j] = (lane == k) ? mk * r : m[j] - fac * mk;
+            v[j] = (lane == k) ? vk * r : v[j] - fac * vk;
+        }
+    }
+    double d = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; j++) d = (j == lane) ? v[j] : d;
+    return d;
+}
+
+// Index of (row i, column j) in the packed upper triangle used by the sums.
+template <int NV>
+__device__ __forceinline__ int packed_index(int i, int j)
+{
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    return a * NV - (a * (a - 1)) / 2 + (b - a);
+}
+
+// Sum V per-lane values across the wave; the totals stay in LDS at
+// scratch[V*65 + v] for every lane to read.
+template <typename E, int V>
+__device__ __forceinline__ void wave_sum_lds(const E (&v)[V], E *scratch, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < V; k++) scratch[k * 65 + lane] = v[k];
+    __syncthreads();
+    if (lane < V) {
+        E s0 = E(0), s1 = E(0), s2 = E(0), s3 = E(0);
+        const E *row = scratch + lane * 65;
+#pragma unroll 4
+        for (int j = 0; j < 64; j += 4) {
+            s0 += row[j];
+            s1 += row[j + 1];
+            s2 += row[j + 2];
+            s3 += row[j + 3];
+        }
+        scratch[V * 65 + lane] = (s0 + s1) + (s2 + s3);
+    }
+    __syncthreads();
+}
+
+template <typename E>
+__device__ __forceinline__ E dmaxr(E a, double b) { return ((double)a > b) ? a : (E)b; }
+template <typename E>
+__device__ __forceinline__ E dminr(E a, double b) { return ((double)a < b) ? a : (E)b; }
+
+// Runtime-size variants (nv <= NVMAX) of the lane-distributed solves, for
+// fitters whose parameter count depends on the call (the 4Pi shared mask).
+template <int NVMAX>
+__device__ __forceinline__ int chol_solve_rows_rt(double (&a)[NVMAX], double b, int lane, int nv, double &xi)
+{
+    double colL[NVMAX], rinv[NVMAX];
+#pragma unroll
+    for (int j = 0; j < NVMAX; j++) colL[j] = 0.0, rinv[j] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NVMAX; k++) {
+        if (k < nv) {
+            const double akk = readlane_d(a[k], k);
+            if (!(akk >= 0.0)) return 1;
+            const double lkk = sqrt(akk);
+            rinv[k] = 1.0 / lkk;
+            const double lik = lane > k ? a[k] * rinv[k] : (lane == k ? lkk : 0.0);
+            a[k] = lik;
+#pragma unroll
+            for (int j = k + 1; j < NVMAX; j++) {
+                if (j < nv) {
+                    const double ljk = readlane_d(lik, j);
+                    a[j] -= lik * ljk;
+                    colL[j] = (lane == k) ? ljk : colL[j];
+                }
+            }
+        }
+    }
+    double acc = 0.0, yv = 0.0;
+#pragma unroll
+    for (int k = 0; k < NVMAX; k++) {
+        if (k < nv) {
+            const double yk = readlane_d((b - acc) * rinv[k], k);
+            yv = (lane == k) ? yk : yv;
+            acc += a[k] * yk;
+        }
+    }
+    double acc2 = 0.0;
+    xi = 0.0;
+#pragma unroll
+    for (int k = NVMAX - 1; k >= 0; k--) {
+        if (k < nv) {
+            const double xk = readlane_d((yv - acc2) * rinv[k], k);
+            xi = (lane == k) ? xk : xi;
+            acc2 += colL[k] * xk;
+        }
+    }
+    return 0;
+}
+
+template <int NVMAX>
+__device__ __forceinline__ double inverse_diag_rows_rt(double (&m)[NVMAX], int lane, int nv)
+{
+    double v[NVMAX];
+#pragma unroll
+    for (int j = 0; j < NVMAX; j++) v[j] = (j == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < NVMAX; k++) {
+        if (k < nv) {
+            const double mkk = readlane_d(m[k], k);
+            const double r = 1.0 / mkk;
+            const double fac = m[k] * r;
+#pragma unroll
+            for (int j = 0; j < NVMAX; j++) {
+                if (j < nv) {
+                    const double mk = readlane_d(m[j], k), vk = readlane_d(v[j], k);
+                    m[j] = (lane == k) ? mk * r : m[j] - fac * mk;
+                    v[j] = (lane == k) ? vk * r : v[j] - fac * vk;
+                }
+            }
+        }
+    }
+    double d = 0.0;
+#pragma unroll
+    for (int j = 0; j < NVMAX; j++) d = (j == lane) ? v[j] : d;
+    return d;
+}
+
+}  // namespace bsf

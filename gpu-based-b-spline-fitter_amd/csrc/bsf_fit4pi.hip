@@ -1,0 +1,368 @@
+// bsf_fit4pi.hip -- gfx950 kernel of the 4Pi multi-channel B-spline fit.
+//
+// The reference calls GPUmleFit_LM_4Pi_bspline (demo_4pi/bspline_based_4piloc_v4.m:
+// 196-206) but ships no source for it.  This fitter is specified from that
+// call site, the channel model of demo_4pi/store/simBSplinePSF.m:38-53 (channel
+// k evaluates the 4D (x, y, z, phi) spline at its phase shifted by the
+// channel offset) and the shared-parameter layout of
+// demo_4pi/store/calculate_CRLB_YL_shared.m:11-13, 98-124 (base parameters x, y,
+// N, bg, z, phi; a shared one is one slot, an unshared one nch slots).  The
+// LM control is fittype 7's (CPUmleFit_LM.cpp:2763-3158) applied slot by
+// slot, with phase slots wrapped (period nw-2) instead of clamped.
+//
+// One wave fits one ROI of nch channels.  Lane l < NV owns parameter slot l
+// (its value, sign, clamp, update and their saved copies) and row l of the
+// Hessian; the per-channel contraction and pixel sums reuse the fittype-7
+// stages, and the channel blocks are scattered into the NV x NV system in LDS.
+#include <hip/hip_runtime.h>
+
+#include "bsf_device.h"
+#include "bsf_kernels.h"
+
+namespace bsf {
+
+struct Dev4 {
+    Dev b;
+    int nch, nv;
+    int slot[6][8];
+    int base_of[24], chan_of[24];   // slot -> base parameter, channel (-1: shared)
+    float shift[8];                 // channel phase offsets, w units
+    const float *dT;                // [6, nch, N] x/y offsets, or NULL
+    const float *initZ, *initPhi;   // [N] starts, or NULL
+};
+
+template <typename E>
+__device__ __forceinline__ E wrap_phase(E w, int nw)
+{
+    if (w <= E(1)) w = (E)(nw - 2) + w;
+    else if (w > (E)(nw - 1)) w = (E(2) + w) - (E)nw;
+    return w;
+}
+
+__device__ __forceinline__ int packed6(int p, int q)
+{
+    const int a = p < q ? p : q, b = p < q ? q : p;
+    return a * 6 - (a * (a - 1)) / 2 + (b - a);
+}
+
+template <typename E, int NVMAX>
+__global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
+{
+    const Dev &A = A4.b;
+    const int n = blockIdx.x + A.roi_offset;
+    if (n >= A.N) return;
+    const int lane = threadIdx.x;
+    const int S = A.S, S2 = S * S, nch = A4.nch, nv = A4.nv;
+    const int W = S + 4, plane_stride = W * W;
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *s_data = (float *)smem;
+    size_t off = (((size_t)nch * S2 * 4) + 15) & ~(size_t)15;
+    E *s_pl = (E *)(smem + off);
+    const size_t plred = (size_t)3 * plane_stride > (size_t)29 * 65 ? (size_t)3 * plane_stride : (size_t)29 * 65;
+    E *s_red = s_pl;
+    E *s_H = s_pl + plred;          // nv * nv
+    E *s_J = s_H + nv * nv;         // nv
+    E *s_err = s_J + nv;            // 1
+
+    for (int p = lane; p < nch * S2; p += 64) {
+        const int k = p / S2, q = p - k * S2;
+        s_data[p] = A.data[((size_t)k * A.N + n) * S2 + q];
+    }
+    __syncthreads();
+
+    const int myrow = lane < nv ? lane : nv - 1;
+    const int myb = A4.base_of[myrow], myc = A4.chan_of[myrow];
+
+    // ---- initial values: fittype 7's per channel, channel mean for a shared
+    // slot (CPUmleFit_LM.cpp:2831-2845, CPUgaussLib.cpp:172-223)
+    E th;
+    {
+        E sum = E(0);
+        for (int k = 0; k < nch; k++) {
+            const float *d = s_data + k * S2;
+            E v;
+            if (myb <= 1) {
+                E tx = E(0), ty = E(0), ts = E(0);
+                for (int ii = 0; ii < S; ii++)
+                    for (int jj = 0; jj < S; jj++) {
+                        E dd = (E)d[S * jj + ii];
+                        tx += dd * (E)ii;
+                        ty += dd * (E)jj;
+                        ts += dd;
+                    }
+                v = (myb == 0) ? tx / ts : ty / ts;
+            } else if (myb <= 3) {
+                E mx = E(0), mn = E(10e10);
+                for (int p = 0; p < S2; p++) {
+                    E dd = (E)d[p];
+                    mx = (mx > dd) ? mx : dd;
+                    mn = (mn < dd) ? mn : dd;
+                }
+                if (myb == 2) {
+                    double t = (double)(((mx - mn) * E(2)) * (E)3.141592) * 1.5 * 1.5;
+                    v = (E)((0.0 > t) ? 0.0 : t);
+                } else {
+                    v = dmaxr<E>(mn, 0.01);
+                }
+            } else if (myb == 4) {
+                v = A4.initZ ? (E)A4.initZ[n] : (E)A.nd[2] / E(2);
+            } else {
+                v = A4.initPhi ? (E)A4.initPhi[n] : (E)A.nd[3] / E(2);
+            }
+            if (myc < 0 || myc == k) sum += v;
+        }
+        th = (myc < 0) ? sum / (E)nch : sum;
+    }
+    const float c7[6] = {1.0f, 1.0f, 10000.f, 20.f, 2.f, 0.00001f};
+    const float f7[6] = {0.5f, 0.5f, 0.4f, 0.5f, 0.7f, 0.8f};
+    E clampv = (E)c7[myb], oclamp = clampv, fade = (E)f7[myb];
+    E oth = th, sgn = E(0), osgn = E(0), upd = E(0);
+
+    Geo<E, 7> g;
+    E hrow[NVMAX], jv = E(0);
+
+    auto channel_params = [&](int k, E *cth) __attribute__((always_inline)) {
+#pragma unroll
+        for (int p = 0; p < 6; p++) cth[p] = readlane_e<E>(th, A4.slot[p][k]);
+        if (A4.dT) {
+            cth[0] += (E)A4.dT[((size_t)n * nch + k) * 6 + 0];
+            cth[1] += (E)A4.dT[((size_t)n * nch + k) * 6 + 1];
+        }
+        cth[5] = wrap_phase<E>(cth[5] + (E)A4.shift[k], A.nd[3]);
+    };
+
+    // one evaluation over all channels: the NV x NV Hessian, J and deviance
+    // assembled in LDS from the per-channel 6 x 6 blocks
+    auto evaluate = [&](bool first) __attribute__((always_inline)) -> E {
+        __syncthreads();
+        for (int i = lane; i < nv * nv + nv + 1; i += 64) s_H[i] = E(0);
+        for (int k = 0; k < nch; k++) {
+            E cth[6];
+            channel_params(k, cth);
+            make_geo<E, 7>(A, cth, g);
+            __syncthreads();
+            stage_a<E, 7>(A, g, s_pl, plane_stride, lane);
+            __syncthreads();
+            E acc[28];
+#pragma unroll
+            for (int q = 0; q < 28; q++) acc[q] = E(0);
+            for (int p = lane; p < S2; p += 64) {
+                const int pjj = p / S, pii = p - pjj * S;
+                E dudt[8], model;
+                pixel_model<E, 7>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], E(0), dudt,
+                                  model);
+                if (first) dudt[5] = dudt[4];
+                E data = (E)s_data[k * S2 + p];
+                if (data > E(0)) {
+                    acc[0] += E(2) * ((model - data) - data * log(model / data));
+                } else {
+                    acc[0] += E(2) * model;
+                    data = E(0);
+                }
+                E t1 = E(1) - data / model;
+                E t2 = data / (model * model);
+#pragma unroll
+                for (int l = 0; l < 6; l++) acc[1 + l] += t1 * dudt[l];
+                int h = 7;
+#pragma unroll
+                for (int l = 0; l < 6; l++)
+#pragma unroll
+                    for (int m = l; m < 6; m++) acc[h++] += t2 * dudt[l] * dudt[m];
+            }
+            __syncthreads();
+            wave_sum_lds<E, 28>(acc, s_red, lane);
+            const E *tot = s_red + 28 * 65;
+            if (lane < 36) {
+                const int p = lane / 6, q = lane - p * 6;
+                s_H[A4.slot[p][k] * nv + A4.slot[q][k]] += tot[7 + packed6(p, q)];
+            } else if (lane < 42) {
+                s_J[A4.slot[lane - 36][k]] += tot[1 + lane - 36];
+            } else if (lane == 42) {
+                s_err[0] += tot[0];
+            }
+            __syncthreads();
+        }
+        jv = s_J[myrow];
+#pragma unroll
+        for (int j = 0; j < NVMAX; j++) hrow[j] = j < nv ? s_H[myrow * nv + j] : E(0);
+        return s_err[0];
+    };
+
+    E newErr = evaluate(true);
+    E oldErr = (E)1e13, newLambda = E(1), oldLambda = E(1);
+    int turn = 1, kk;
+    const int zslot = A4.slot[4][0];
+    for (kk = 0; kk < A.iterations; kk++) {
+        if (fabs((newErr - oldErr) / newErr) < (E)1e-6) break;
+        if ((double)newErr > 1.5 * (double)oldErr) {
+            sgn = osgn;
+            clampv = oclamp;
+            th = oth;
+            newLambda = oldLambda;
+            newErr = oldErr;
+            newLambda = E(10) * newLambda;
+        } else if (newErr < oldErr) {
+            if (newLambda > E(1)) newLambda = (E)((double)newLambda * 0.8);
+            else if (newLambda < E(1)) newLambda = E(1);
+        }
+        double arow[NVMAX], xi;
+#pragma unroll
+        for (int j = 0; j < NVMAX; j++) {
+            hrow[j] = (j == lane) ? hrow[j] * newLambda : hrow[j];
+            arow[j] = (double)hrow[j];
+        }
+        int info = chol_solve_rows_rt<NVMAX>(arow, (double)jv, lane, nv, xi);
+        if (info == 0) {
+            upd = (E)xi;
+            if (__any(lane < nv && isnan(upd))) info = 1;
+        }
+        if (info != 0) {
+            newLambda = E(10) * newLambda;
+            continue;
+        }
+        if (myb == 5) upd = upd * E(10);
+        osgn = sgn;
+        oclamp = clampv;
+        oth = th;
+        oldLambda = newLambda;
+        oldErr = newErr;
+        const E uz = readlane_e<E>(upd, zslot);
+        if ((double)fabs(uz) < 0.005 && turn) {
+            if (myb == 5) clampv = oclamp = (E)A.nd[3] / E(10);
+            if (myb == 4) clampv = oclamp = E(2);
+            turn = 0;
+        }
+        if (sgn != E(0)) {
+            if ((sgn == E(1) && upd < E(0)) || (sgn == E(-1) && upd > E(0))) clampv = clampv * fade;
+        }
+        sgn = (upd > E(0)) ? E(1) : E(-1);
+        th = th - upd / (E(1) + fabs(upd / clampv));
+        if (myb == 2) th = dmaxr<E>(th, 1.0);
+        else if (myb == 3) th = dmaxr<E>(th, 0.01);
+        else if (myb == 4) th = dminr<E>(dmaxr<E>(th, 0.0), (double)(E)A.nd[2] - 0.5);
+        else if (myb == 5) th = wrap_phase<E>(th, A.nd[3]);
+        newErr = evaluate(false);
+    }
+
+    // ---- Fisher information over all channels, CRLB, log-likelihood
+    __syncthreads();
+    for (int i = lane; i < nv * nv + nv + 1; i += 64) s_H[i] = E(0);
+    for (int k = 0; k < nch; k++) {
+        E cth[6];
+        channel_params(k, cth);
+        make_geo<E, 7>(A, cth, g);
+        __syncthreads();
+        stage_a<E, 7>(A, g, s_pl, plane_stride, lane);
+        __syncthreads();
+        E acc[22];
+#pragma unroll
+        for (int q = 0; q < 22; q++) acc[q] = E(0);
+        for (int p = lane; p < S2; p += 64) {
+            const int pjj = p / S, pii = p - pjj * S;
+            E dudt[8], model;
+            pixel_model<E, 7>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], E(0), dudt, model);
+            E data = (E)s_data[k * S2 + p];
+            int h = 1;
+#pragma unroll
+            for (int a = 0; a < 6; a++)
+#pragma unroll
+                for (int b = a; b < 6; b++) acc[h++] += dudt[b] * dudt[a] / model;
+            if (model > E(0)) {
+                if (data > E(0)) acc[0] += data * log(model) - model - data * log(data) + data;
+                else acc[0] += -model;
+            }
+        }
+        __syncthreads();
+        wave_sum_lds<E, 22>(acc, s_red, lane);
+        const E *tot = s_red + 22 * 65;
+        if (lane < 36) {
+            const int p = lane / 6, q = lane - p * 6;
+            s_H[A4.slot[p][k] * nv + A4.slot[q][k]] += tot[1 + packed6(p, q)];
+        } else if (lane == 42) {
+            s_err[0] += tot[0];
+        }
+        __syncthreads();
+    }
+    double mrow[NVMAX];
+#pragma unroll
+    for (int j = 0; j < NVMAX; j++) mrow[j] = j < nv ? (double)s_H[myrow * nv + j] : 0.0;
+    const double crlb = inverse_diag_rows_rt<NVMAX>(mrow, lane, nv);
+    if (lane < nv) {
+        A.P[(size_t)A.N * lane + n] = (float)th;
+        A.CRLB[(size_t)A.N * lane + n] = (float)crlb;
+    }
+    if (lane == 0) {
+        A.P[(size_t)A.N * nv + n] = (float)kk;
+        A.LL[n] = (float)s_err[0];
+    }
+}
+
+size_t lds_bytes_4pi(int f64, int S, int nch, int nv)
+{
+    const size_t W = S + 4;
+    size_t plred = 3 * W * W > 29 * 65 ? 3 * W * W : 29 * 65;
+    return ((((size_t)nch * S * S * 4) + 15) & ~(size_t)15) + (f64 ? 8 : 4) * (plred + nv * nv + nv + 1);
+}
+
+template <typename E, int NVMAX>
+static hipError_t launch4(const Dev4 &A4, int nblocks, hipStream_t st, size_t lds)
+{
+    hipLaunchKernelGGL((fit4pi_kernel<E, NVMAX>), dim3(nblocks), dim3(64), lds, st, A4);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st)
+{
+    Dev4 A4;
+    Dev &A = A4.b;
+    A.data = L.data;
+    A.coeff = L.coeff;
+    A.coeff3 = nullptr;
+    A.P = L.P;
+    A.CRLB = L.CRLB;
+    A.LL = L.LL;
+    A.N = L.N;
+    A.S = L.S;
+    A.iterations = L.iterations;
+    A.roi_offset = L.roi_offset;
+    for (int a = 0; a < 4; a++) A.nd[a] = L.cdims[a] - 3;
+    A.nd[4] = 0;
+    A.s1 = L.cdims[0];
+    A.s2 = A.s1 * L.cdims[1];
+    A.s3 = A.s2 * L.cdims[2];
+    A.s4 = A.s3 * L.cdims[3];
+    A.cbytes = (int)(A.s4 * 4);
+    A.c3bytes = 0;
+    A.has_initP = 0;
+    A.theta_in = nullptr;
+    A.dbg = nullptr;
+    A4.nch = L.nch;
+    int n = 0;
+    for (int p = 0; p < 6; p++) {
+        for (int k = 0; k < L.nch; k++) A4.slot[p][k] = L.shared[p] ? n : n + k;
+        for (int k = L.nch; k < 8; k++) A4.slot[p][k] = 0;
+        for (int k = 0; k < (L.shared[p] ? 1 : L.nch); k++) {
+            A4.base_of[n + k] = p;
+            A4.chan_of[n + k] = L.shared[p] ? -1 : k;
+        }
+        n += L.shared[p] ? 1 : L.nch;
+    }
+    A4.nv = n;
+    for (int i = n; i < 24; i++) A4.base_of[i] = 0, A4.chan_of[i] = -1;
+    for (int k = 0; k < 8; k++) A4.shift[k] = k < L.nch ? L.phaseShift[k] : 0.f;
+    A4.dT = L.dT;
+    A4.initZ = L.initZ;
+    A4.initPhi = L.initPhi;
+    const size_t lds = lds_bytes_4pi(L.f64, L.S, L.nch, n);
+    const int nb = L.count;
+    if (L.f64) {
+        if (n <= 8) return launch4<double, 8>(A4, nb, st, lds);
+        if (n <= 12) return launch4<double, 12>(A4, nb, st, lds);
+        return launch4<double, 24>(A4, nb, st, lds);
+    }
+    if (n <= 8) return launch4<float, 8>(A4, nb, st, lds);
+    if (n <= 12) return launch4<float, 12>(A4, nb, st, lds);
+    return launch4<float, 24>(A4, nb, st, lds);
+}
+
+}  // namespace bsf

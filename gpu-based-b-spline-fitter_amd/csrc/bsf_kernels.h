@@ -25,4 +25,21 @@ struct FitLaunch {
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st);
 size_t lds_bytes(int fittype, int f64, int S);
 
+struct Fit4PiLaunch {
+    int f64;
+    const float *data;        // device, [S, S, N, nch]
+    const float *coeff;       // device, 4D (x, y, z, phase) coefficients
+    float *P, *CRLB, *LL;     // device outputs
+    int N, S, iterations, nch;
+    int roi_offset, count;
+    int cdims[4];
+    int shared[6];
+    float phaseShift[8];      // w units
+    const float *dT;          // device [6, nch, N] or NULL
+    const float *initZ, *initPhi;  // device [N] or NULL
+};
+
+hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st);
+size_t lds_bytes_4pi(int f64, int S, int nch, int nv);
+
 }  // namespace bsf

@@ -248,6 +248,144 @@ int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_n
     return rc;
 }
 
+// ------------------------------------------------------------------ 4Pi
+static int nv_4pi(const unsigned int *shared, int nch)
+{
+    if (!shared || nch < 1 || nch > 8) return -1;
+    int n = 0;
+    for (int p = 0; p < 6; p++) n += shared[p] ? 1 : nch;
+    return n;
+}
+
+int bsf_num_params_4pi(const unsigned int *shared, int nch) { return nv_4pi(shared, nch); }
+
+static int validate_4pi(const bsf_request_4pi *r, const bsf_result *res)
+{
+    if (!r || !res) return fail(BSF_ERR_ARG, "bsf: NULL request or result");
+    const int nv = nv_4pi(r->shared, r->nch);
+    if (nv < 0) return fail(BSF_ERR_ARG, "bsf: 4Pi needs a shared[6] mask and 1..8 channels (got %d)", r->nch);
+    if (nv > 24) return fail(BSF_ERR_ARG, "bsf: 4Pi parameter count %d exceeds 24", nv);
+    if (r->nfits < 0 || r->iterations < 0) return fail(BSF_ERR_ARG, "bsf: nfits and iterations must be >= 0");
+    if (r->sz < 1 || r->sz > BSF_MAX_ROI)
+        return fail(BSF_ERR_ROI_SIZE, "bsf: X,Y dimension of data must be between 1 and %d (got %d)", BSF_MAX_ROI,
+                    r->sz);
+    if (r->nfits > 0 && (!r->imstack || !res->P || !res->CRLB || !res->LL))
+        return fail(BSF_ERR_ARG, "bsf: NULL data or output pointer");
+    if (!r->coeff || !r->phaseShift) return fail(BSF_ERR_ARG, "bsf: coeff and phaseShift must not be NULL");
+    for (int a = 0; a < 4; a++)
+        if (r->coeff_dims[a] < 5) return fail(BSF_ERR_SHAPE, "bsf: coefficient dim %d is %d (needs >= 5)", a, r->coeff_dims[a]);
+    if (r->precision != BSF_PREC_MIX && r->precision != BSF_PREC_F64)
+        return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", r->precision);
+    return BSF_OK;
+}
+
+static int check_device_4pi(const bsf_request_4pi *r)
+{
+    int rc = check_device(1, 6, 0);
+    if (rc) return rc;
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
+        return fail(BSF_ERR_DEVICE, "bsf: device query failed");
+    size_t lds = bsf::lds_bytes_4pi(r->precision == BSF_PREC_F64, r->sz, r->nch, nv_4pi(r->shared, r->nch));
+    if (lds > p.sharedMemPerBlock)
+        return fail(BSF_ERR_ROI_SIZE, "bsf: 4Pi ROI size %d x %d channels needs %zu B of LDS (limit %zu B)", r->sz,
+                    r->nch, lds, (size_t)p.sharedMemPerBlock);
+    return BSF_OK;
+}
+
+static bsf::Fit4PiLaunch make_launch_4pi(const bsf_request_4pi *r, const float *data, const float *coeff,
+                                         const float *dT, const float *iz, const float *ip, float *P, float *CR,
+                                         float *LL)
+{
+    bsf::Fit4PiLaunch L;
+    std::memset(&L, 0, sizeof L);
+    L.f64 = r->precision == BSF_PREC_F64;
+    L.data = data;
+    L.coeff = coeff;
+    L.P = P;
+    L.CRLB = CR;
+    L.LL = LL;
+    L.N = r->nfits;
+    L.S = r->sz;
+    L.iterations = r->iterations;
+    L.nch = r->nch;
+    for (int a = 0; a < 4; a++) L.cdims[a] = r->coeff_dims[a];
+    for (int p = 0; p < 6; p++) L.shared[p] = r->shared[p] ? 1 : 0;
+    for (int k = 0; k < r->nch; k++) L.phaseShift[k] = r->phaseShift[k];
+    L.dT = dT;
+    L.initZ = iz;
+    L.initPhi = ip;
+    return L;
+}
+
+static hipError_t launch_all_4pi(bsf::Fit4PiLaunch L, hipStream_t st)
+{
+    const int chunk = 1 << 24;
+    for (int off = 0; off < L.N; off += chunk) {
+        L.roi_offset = off;
+        L.count = (L.N - off) < chunk ? (L.N - off) : chunk;
+        hipError_t e = bsf::launch_fit_4pi(L, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int bsf_fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, void *stream)
+{
+    g_err.clear();
+    int rc = validate_4pi(r, res);
+    if (rc) return rc;
+    if ((rc = check_device_4pi(r))) return rc;
+    if (r->nfits == 0) return BSF_OK;
+    bsf::Fit4PiLaunch L = make_launch_4pi(r, r->imstack, r->coeff, r->dT, r->initZ, r->initPhi, res->P, res->CRLB,
+                                          res->LL);
+    hipError_t e = launch_all_4pi(L, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
+    return BSF_OK;
+}
+
+int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
+{
+    g_err.clear();
+    int rc = validate_4pi(r, res);
+    if (rc) return rc;
+    if ((rc = check_device_4pi(r))) return rc;
+    if (r->nfits == 0) return BSF_OK;
+    const int nv = nv_4pi(r->shared, r->nch);
+    const size_t N = (size_t)r->nfits, S2 = (size_t)r->sz * r->sz;
+    size_t nc = 1;
+    for (int a = 0; a < 4; a++) nc *= (size_t)r->coeff_dims[a];
+    const size_t bd = N * S2 * r->nch * 4, bc = nc * 4, bt = r->dT ? N * 6 * r->nch * 4 : 0;
+    const size_t bz = r->initZ ? N * 4 : 0, bp = r->initPhi ? N * 4 : 0, bo = N * (2 * nv + 2) * 4;
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    char *buf = nullptr;
+    hipError_t e = hipMalloc((void **)&buf, up(bd) + up(bc) + up(bt) + up(bz) + up(bp) + up(bo));
+    if (e != hipSuccess) return fail(BSF_ERR_NOMEM, "bsf: device allocation failed");
+    char *q = buf;
+    float *d_data = (float *)q; q += up(bd);
+    float *d_c = (float *)q; q += up(bc);
+    float *d_dT = bt ? (float *)q : nullptr; q += up(bt);
+    float *d_iz = bz ? (float *)q : nullptr; q += up(bz);
+    float *d_ip = bp ? (float *)q : nullptr; q += up(bp);
+    float *d_P = (float *)q, *d_CR = d_P + N * (nv + 1), *d_LL = d_CR + N * nv;
+    hipStream_t st = hipStreamPerThread;
+    e = hipMemcpyAsync(d_data, r->imstack, bd, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_c, r->coeff, bc, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && bt) e = hipMemcpyAsync(d_dT, r->dT, bt, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && bz) e = hipMemcpyAsync(d_iz, r->initZ, bz, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && bp) e = hipMemcpyAsync(d_ip, r->initPhi, bp, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_all_4pi(make_launch_4pi(r, d_data, d_c, d_dT, d_iz, d_ip, d_P, d_CR, d_LL), st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res->P, d_P, N * (nv + 1) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res->CRLB, d_CR, N * nv * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res->LL, d_LL, N * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    rc = BSF_OK;
+    if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
+    (void)hipFree(buf);
+    return rc;
+}
+
 // Test hook, not part of include/bsfit.h: one deviance / gradient / Hessian
 // evaluation per ROI at the given parameters (theta: host [nfits][NV]);
 // out (host): [nfits][1 + NV + NV*NV] = err, J, H.  Lets tests compare the

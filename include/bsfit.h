@@ -102,6 +102,48 @@ int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_n
                             const float *varmap, size_t varmap_numel, int silent, const float *initP,
                             size_t initP_numel, const float *coeff3, float *P, float *CRLB, float *LL);
 
+/* ------------------------------------------------------------------ 4Pi
+ * Multi-channel 4Pi fit.  Replaces the MEX the reference's 4Pi demo calls,
+ *   [P, CRLB, LogL] = GPUmleFit_LM_4Pi_bspline(imstack, uint32(shared), iterations,
+ *                         coeffs, dT, initZ, initPhi, phaseShift)
+ * (demo_4pi/bspline_based_4piloc_v4.m:196-206), whose source the reference
+ * does not ship.
+ *   imstack    float [sz, sz, nfits, nch], nch <= 8 channels
+ *   shared     6 flags for x, y, N, bg, z, phi; NV = sum(shared ? 1 : nch) <= 24
+ *              (calculate_CRLB_YL_shared.m:11-13 layout: an unshared
+ *              parameter takes nch consecutive slots)
+ *   coeff      4D bsarray coefficients (x, y, z, phase); the phase axis has
+ *              period nw-2 samples (nw = coeff_dims[3]-3)
+ *   dT         float [6, nch, nfits] per-channel offsets (x, y used), or NULL
+ *   initZ, initPhi   float [nfits] starts (spline units), or NULL (nz/2, nw/2)
+ *   phaseShift float [nch] channel phase offsets in phase-axis (w) units
+ *   P [nfits, NV+1] (last column: iterations), CRLB [nfits, NV], LL [nfits]. */
+typedef struct bsf_request_4pi {
+    const float *imstack;
+    int sz;
+    int nfits;
+    int nch;
+    const unsigned int *shared;
+    int iterations;
+    const float *coeff;
+    int coeff_dims[4];
+    const float *dT;
+    const float *initZ;
+    const float *initPhi;
+    const float *phaseShift;
+    int precision;
+} bsf_request_4pi;
+
+/* NV for a shared mask and channel count, or -1. */
+int bsf_num_params_4pi(const unsigned int *shared, int nch);
+
+/* Synchronous 4Pi fit on host arrays (copies to and from the device). */
+int bsf_fit_4pi_host(const bsf_request_4pi *req, bsf_result *res);
+
+/* Asynchronous 4Pi fit on device buffers (imstack, coeff, dT, initZ, initPhi
+ * and the outputs are device pointers; shared and phaseShift are host). */
+int bsf_fit_4pi_device(const bsf_request_4pi *req, bsf_result *res, void *stream);
+
 /* Text of the last error on this thread ("" if none).
  * Replaces mexErrMsgIdAndTxt (GPUmleFit_LM/mexFunction.cpp:140-176). */
 const char *bsf_last_error(void);

@@ -1,0 +1,49 @@
+"""The C-ABI library (libbsfit.so) loads and exports every function that
+include/bsfit.h declares; host-only entry points answer without a GPU, and
+compute entry points fail with BSF_ERR_DEVICE (never crash) where no gfx950
+device is present."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "bsfit.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bsf_\w+)\s*\(", src)))
+
+
+def test_exports_every_declared_symbol():
+    import bsfit
+    lib = bsfit.load()
+    names = _declared()
+    assert "bsf_fit_host" in names and "bsf_fit_4pi_host" in names
+    for nm in names:
+        assert hasattr(lib, nm), "libbsfit.so does not export %s" % nm
+
+
+def test_host_only_entry_points():
+    import bsfit
+    assert [bsfit.num_params(f) for f in (6, 7, 8, 5)] == [5, 6, 8, -1]
+    assert bsfit.num_params_4pi([1] * 6, 4) == 6
+    assert bsfit.num_params_4pi([1, 1, 1, 0, 1, 1], 4) == 9
+
+
+def test_errors_without_device_or_bad_args():
+    import torch
+    import bsfit
+    from bsfit import gen
+    w = gen.config(0, N=2)
+    # argument errors are reported before any device work
+    with pytest.raises(bsfit.BsfError) as e:
+        bsfit.fit(9, w.imstack, 50, w.coeff)
+    assert e.value.code == 2
+    if not torch.cuda.is_available():
+        with pytest.raises(bsfit.BsfError) as e:
+            bsfit.fit(6, w.imstack, 50, w.coeff)
+        assert e.value.code == 6

@@ -1,0 +1,68 @@
+"""4Pi multi-channel fit on the GPU against the oracle's 4Pi restatement.
+
+The reference ships no source for its 4Pi B-spline fitter, so the oracle is
+specified from its call site and MATLAB helpers (see oracle/bsf_oracle.c,
+A4); CPU-side pins (reduction to fittype 7, truth recovery, NV layout) are in
+test_oracle_4pi.py.  Here: the GPU against that oracle, at the same
+agreement-rate bar as the fittype 6-8 fits."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cmp(P, CR, Po, CRo, NV):
+    coord = [0, 1] + [i for i in range(NV)]  # all slots compared
+    d = np.abs(P[:, :NV].astype(np.float64) - Po[:, :NV])
+    scale = np.maximum(np.abs(Po[:, :NV]), 1.0)
+    ok_p = (d / scale <= 1e-4).all(1)
+    ok_kk = P[:, NV] == Po[:, NV]
+    ok_cr = (np.abs(CR - CRo) / np.maximum(np.abs(CRo), 1e-30) <= 1e-4).all(1)
+    return float((ok_p & ok_kk & ok_cr).mean()), float(ok_p.mean())
+
+
+@pytest.mark.parametrize("shared", [(1, 1, 1, 1, 1, 1), (1, 1, 1, 0, 1, 1)])
+def test_4pi_matches_oracle(orc, shared):
+    import bsfit
+    from bsfit import gen
+    w = gen.make_rois_4pi(96, 13, 4242, gen.model_coeffs("4pi"), shared=shared)
+    print("case 4Pi shared=%s N=%d" % (shared, w.N), flush=True)
+    NV = bsfit.num_params_4pi(shared, 4)
+    assert NV == (6 if all(shared) else 9)
+    args = (w.imstack, shared, 50, w.coeff, w.phaseShift)
+    kw = dict(initZ=w.initZ, initPhi=w.initPhi)
+    gd = bsfit.fit_4pi(*args, precision=bsfit.PREC_F64, **kw)
+    gm = bsfit.fit_4pi(*args, **kw)
+    od = orc.fit_4pi(*args, nthreads=8, f64=True, **kw)
+    of = orc.fit_4pi(*args, nthreads=8, **kw)
+    a64, p64 = _cmp(gd[0], gd[1], od[0], od[1], NV)
+    amx, pmx = _cmp(gm[0], gm[1], of[0], of[1], NV)
+    aref, pref = _cmp(of[0], of[1], od[0], od[1], NV)
+    print("agree F64 %.3f (P %.3f)  MIX %.3f (P %.3f)  oracle F32 vs F64 %.3f (P %.3f)"
+          % (a64, p64, amx, pmx, aref, pref), flush=True)
+    assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0]))
+    assert a64 >= 0.3 and p64 >= 0.5
+    assert amx >= aref - 0.15
+    ll_ok = np.mean(gm[2] >= of[2] - 1e-3 * np.abs(of[2]) - 1e-3)
+    print("LL(gpu MIX) >= LL(oracle F32) on %.3f" % ll_ok, flush=True)
+    assert ll_ok >= 0.75
+
+
+@pytest.mark.parametrize("iters", [1, 2, 4])
+def test_4pi_short_trajectories_match_f64_oracle(orc, iters):
+    """The first LM steps (init, evaluation over channels, slot assembly,
+    solve, limiter, bounds and wrap) before rounding differences can grow:
+    the FP64 kernel against the FP64 oracle to 1e-5."""
+    import bsfit
+    from bsfit import gen
+    for shared in ((1, 1, 1, 1, 1, 1), (1, 1, 1, 0, 1, 1), (0, 1, 1, 0, 1, 1)):
+        w = gen.make_rois_4pi(32, 13, 11, gen.model_coeffs("4pi"), shared=shared)
+        args = (w.imstack, shared, iters, w.coeff, w.phaseShift)
+        kw = dict(initZ=w.initZ, initPhi=w.initPhi)
+        P, CR, LL = bsfit.fit_4pi(*args, precision=bsfit.PREC_F64, **kw)
+        Po, CRo, LLo = orc.fit_4pi(*args, nthreads=8, f64=True, **kw)
+        NV = CR.shape[1]
+        rel = np.abs(P[:, :NV] - Po[:, :NV]) / np.maximum(np.abs(Po[:, :NV]), 1.0)
+        ok = (rel.max(1) < 1e-5) & (P[:, NV] == Po[:, NV])
+        print("shared", shared, "iterations", iters, "agree", ok.mean(), flush=True)
+        assert ok.mean() >= 0.95

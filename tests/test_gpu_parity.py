@@ -100,3 +100,23 @@ def test_single_roi_and_iterations_zero(orc):
     assert np.allclose(P0[:, :5], Po[:, :5], rtol=1e-6, atol=1e-5)
     assert np.allclose(CR0, CRo, rtol=1e-4)
     assert np.allclose(LL0, LLo, rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.parametrize("ft,cfg,S", [(6, 0, None), (7, 1, None), (8, 3, 11), (8, 3, 17)])
+@pytest.mark.parametrize("iters", [1, 2, 4])
+def test_short_trajectories_match_f64_oracle(orc, ft, cfg, S, iters):
+    """The first LM steps (initial estimates, accept / reject, damping,
+    solve, limiter, clamp fading, bounds, wraps, the fittype-7 w scaling and
+    the fittype-8 extra convergence test) before rounding differences grow:
+    the FP64 kernel against the FP64 oracle to 1e-5."""
+    import bsfit
+    from bsfit import gen
+    w = gen.config(cfg, N=48, S=S)
+    w = w[0] if isinstance(w, list) else w
+    P, CR, LL = bsfit.fit(ft, w.imstack, iters, w.coeff, w.initP, w.coeff3, precision=bsfit.PREC_F64)
+    Po, CRo, LLo = orc.fit_f64(ft, w.imstack, iters, w.coeff, w.initP, w.coeff3, nthreads=8)
+    NV = CR.shape[1]
+    rel = np.abs(P[:, :NV] - Po[:, :NV]) / np.maximum(np.abs(Po[:, :NV]), 1.0)
+    ok = (rel.max(1) < 1e-5) & (P[:, NV] == Po[:, NV])
+    print("fittype", ft, "S", w.S, "iterations", iters, "agree", ok.mean(), flush=True)
+    assert ok.mean() >= 0.95
