@@ -209,25 +209,49 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
             planes[plane_stride + q] = dz;
             planes[2 * plane_stride + q] = dw;
         } else {
+            // Slice by slice along z (16 taps at a time) to bound live
+            // registers.  The reference's 5D contraction has two irregular
+            // terms in both its CPU and GPU code (CPUsplineLib.cpp:313 /
+            // 420-421, GPUsplineLib.cuh:573 / 681), reproduced as two extra
+            // planes applied later with the upper / lower 2x2 xy stencils:
+            //  z: term 27 of block (q,r,s) weighs the z-difference at tap
+            //     kz-q (w tap kw+r-1, u tap ku+s-1) with a weight taken from
+            //     block +8, term 15 (past the z table: the w table);
+            //  u: term 16 differences u tap ku-s instead of ku+s-1; with
+            //     dB2[2] = 0 only s = 1 survives: weight * (2c[ku]-c[ku-1]-c[ku+1]).
             const Axis<E> &z = g.ax[2], &w = g.ax[3], &u = g.ax[4];
-            E cv[4][4][4];
+            E Vz[3][3][3], Vu[3][3];
 #pragma unroll
-            for (int tz = 0; tz < 4; tz++)
+            for (int q2 = 1; q2 <= 2; q2++)
+#pragma unroll
+                for (int r = 1; r <= 2; r++) {
+#pragma unroll
+                    for (int s = 1; s <= 2; s++) {
+                        E bug;
+                        if (r == 1) bug = z.dB2[q2] * w.B2[2] * u.B1[s];
+                        else if (s == 1) bug = z.dB2[q2] * w.B2[1] * u.B1[2];
+                        else bug = z.B2[q2] * w.dB2[1] * u.B1[1];
+                        Vz[q2][r][s] = bug - z.dB1[q2] * w.B2[r] * u.B2[s];
+                    }
+                    Vu[q2][r] = z.B1[q2] * w.B1[r] * u.dB2[1];
+                }
+            E v = E(0), dz = E(0), dw = E(0), du = E(0), zb = E(0), ub = E(0);
+            E prev[2][2];
+#pragma unroll
+            for (int tz = 0; tz < 4; tz++) {
+                E cs[4][4];
 #pragma unroll
                 for (int tw = 0; tw < 4; tw++)
 #pragma unroll
-                    for (int tu = 0; tu < 4; tu++) cv[tz][tw][tu] = (E)bload(rc, vo, tapb + tz * s2 + tw * s3 + tu * s4);
-            E v = E(0), dz = E(0), dw = E(0), du = E(0);
-#pragma unroll
-            for (int tz = 0; tz < 4; tz++) {
+                    for (int tu = 0; tu < 4; tu++) cs[tw][tu] = (E)bload(rc, vo, tapb + tz * s2 + tw * s3 + tu * s4);
                 E a00 = E(0), a01 = E(0), a10 = E(0);
 #pragma unroll
                 for (int tw = 0; tw < 4; tw++) {
                     E u0 = E(0), u1 = E(0);
 #pragma unroll
                     for (int tu = 0; tu < 4; tu++) {
-                        u0 += u.W[tu] * cv[tz][tw][tu];
-                        u1 += u.Dv[tu] * cv[tz][tw][tu];
+                        u0 += u.W[tu] * cs[tw][tu];
+                        u1 += u.Dv[tu] * cs[tw][tu];
                     }
                     a00 += w.W[tw] * u0;
                     a01 += w.Dv[tw] * u0;
@@ -237,40 +261,26 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
                 dz += z.Dv[tz] * a00;
                 dw += z.W[tz] * a01;
                 du += z.W[tz] * a10;
-            }
-            // The reference's 5D contraction has two irregular terms in
-            // both its CPU and GPU code (CPUsplineLib.cpp:313 / 420-421,
-            // GPUsplineLib.cuh:573 / 681); they are reproduced as two extra
-            // planes applied later with the upper / lower 2x2 xy stencils.
-            //  z: term 27 of block (q,r,s) weighs the z-difference at tap
-            //     kz-q (w tap kw+r-1, u tap ku+s-1) with a weight taken from
-            //     block +8, term 15 (past the z table: the w table).
-            E zb = E(0);
+                if (tz == 1 || tz == 2) {
+                    const int q2 = 3 - tz;      // difference between slices tz-1 and tz
 #pragma unroll
-            for (int q2 = 1; q2 <= 2; q2++)
+                    for (int r = 1; r <= 2; r++)
 #pragma unroll
-                for (int r = 1; r <= 2; r++)
-#pragma unroll
-                    for (int s = 1; s <= 2; s++) {
-                        E bug;
-                        if (r == 1) bug = z.dB2[q2] * w.B2[2] * u.B1[s];
-                        else if (s == 1) bug = z.dB2[q2] * w.B2[1] * u.B1[2];
-                        else bug = z.B2[q2] * w.dB2[1] * u.B1[1];
-                        E ok = z.dB1[q2] * w.B2[r] * u.B2[s];
-                        const int tz = 2 - q2, tw = 1 + r, tu = 1 + s;
-                        zb += (bug - ok) * (cv[tz + 1][tw][tu] - cv[tz][tw][tu]);
-                    }
-            //  u: term 16 differences u tap ku-s instead of ku+s-1; with
-            //     dB2[2] = 0 only s = 1 survives: weight * (2c[ku]-c[ku-1]-c[ku+1]).
-            E ub = E(0);
-#pragma unroll
-            for (int q2 = 1; q2 <= 2; q2++)
-#pragma unroll
-                for (int r = 1; r <= 2; r++) {
-                    const int tz = 2 - q2, tw = 2 - r;
-                    ub += z.B1[q2] * w.B1[r] * (E(2) * cv[tz][tw][2] - cv[tz][tw][1] - cv[tz][tw][3]);
+                        for (int s = 1; s <= 2; s++) zb += Vz[q2][r][s] * (cs[1 + r][1 + s] - prev[r - 1][s - 1]);
                 }
-            ub *= u.dB2[1];
+                if (tz <= 1) {
+                    const int q2 = 2 - tz;
+#pragma unroll
+                    for (int r = 1; r <= 2; r++) {
+                        const int tw = 2 - r;
+                        ub += Vu[q2][r] * (E(2) * cs[tw][2] - cs[tw][1] - cs[tw][3]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+#pragma unroll
+                    for (int s = 0; s < 2; s++) prev[r][s] = cs[2 + r][2 + s];
+            }
             planes[q] = v;
             planes[plane_stride + q] = dz;
             planes[2 * plane_stride + q] = dw;

@@ -51,8 +51,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 
     // ---- initial values: centre of mass, max / min (CPUgaussLib.cpp:172-223)
     // summed in the reference's sequential order so they match it bit for bit.
-    E th[NV], oth[NV], sgn[NV], osgn[NV], clampv[NV], oclamp[NV], fade[NV], upd[NV];
-    (void)NPL;
+    // Lane l < NV owns parameter l: its value and LM state (sign, clamp,
+    // update and the copies restored on a rejected step) are per-lane scalars.
+    const int myl = lane < NV ? lane : NV - 1;
+    E th;
     {
         E tx = E(0), ty = E(0), ts = E(0), mx = E(0), mn = E(10e10);
         for (int ii = 0; ii < S; ii++)
@@ -67,65 +69,70 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             mx = (mx > d) ? mx : d;
             mn = (mn < d) ? mn : d;
         }
-        th[0] = tx / ts;
-        th[1] = ty / ts;
+        E init[8];
+        init[0] = tx / ts;
+        init[1] = ty / ts;
         double t = (double)(((mx - mn) * E(2)) * (E)3.141592) * 1.5 * 1.5;   // PI, definitions.h:15
-        th[2] = (E)((0.0 > t) ? 0.0 : t);
-        th[3] = dmaxr<E>(mn, 0.01);
+        init[2] = (E)((0.0 > t) ? 0.0 : t);
+        init[3] = dmaxr<E>(mn, 0.01);
+        init[4] = init[5] = init[6] = init[7] = E(0);
         if constexpr (FT == 6) {
-            th[4] = (E)A.nd[2] / E(2);
+            init[4] = (E)A.nd[2] / E(2);
         } else if constexpr (FT == 7) {
             if (A.has_initP && A.initP[0] != 0.f) {
-                th[4] = (E)A.initP[0];
-                th[5] = (E)A.initP[1];
+                init[4] = (E)A.initP[0];
+                init[5] = (E)A.initP[1];
             } else {
-                th[4] = (E)A.nd[2] / E(2);
-                th[5] = (E)A.nd[3] / E(2);
+                init[4] = (E)A.nd[2] / E(2);
+                init[5] = (E)A.nd[3] / E(2);
             }
         } else {
             if (A.has_initP) {
 #pragma unroll
-                for (int i = 4; i < 8; i++) th[i] = (E)A.initP[i - 4];
+                for (int i = 4; i < 8; i++) init[i] = (E)A.initP[i - 4];
             } else {
-                th[4] = (E)A.nd[2] / E(2);
-                th[5] = (E)A.nd[3] / E(2);
-                th[6] = (E)A.nd[4] / E(2);
-                th[7] = (E)0.9;
+                init[4] = (E)A.nd[2] / E(2);
+                init[5] = (E)A.nd[3] / E(2);
+                init[6] = (E)A.nd[4] / E(2);
+                init[7] = (E)0.9;
             }
         }
+        th = init[0];
+#pragma unroll
+        for (int i = 1; i < NV; i++) th = (myl == i) ? init[i] : th;
     }
+    E clampv, fade;
     {
         const float c6[5] = {2.0f, 2.0f, 10000.f, 20.f, 20.f};
         const float c7[6] = {1.0f, 1.0f, 10000.f, 20.f, 2.f, 0.00001f};
         const float f7[6] = {0.5f, 0.5f, 0.4f, 0.5f, 0.7f, 0.8f};
         const float c8[8] = {1.0f, 1.0f, 10000.f, 20.f, 2.f, 2.f, 2.f, 0.1f};
+        clampv = (E)(FT == 6 ? c6[0] : FT == 7 ? c7[0] : c8[0]);
+        fade = (E)(FT == 7 ? f7[0] : 0.5f);
 #pragma unroll
-        for (int i = 0; i < NV; i++) {
-            clampv[i] = (E)(FT == 6 ? c6[i] : FT == 7 ? c7[i] : c8[i]);
-            oclamp[i] = clampv[i];
-            fade[i] = (E)(FT == 7 ? f7[i] : 0.5f);
-            sgn[i] = E(0);
-            osgn[i] = E(0);
-            upd[i] = E(0);
-            oth[i] = th[i];
+        for (int i = 1; i < NV; i++) {
+            clampv = (myl == i) ? (E)(FT == 6 ? c6[i] : FT == 7 ? c7[i] : c8[i]) : clampv;
+            fade = (myl == i) ? (E)(FT == 7 ? f7[i] : 0.5f) : fade;
         }
     }
+    E oclamp = clampv, oth = th, sgn = E(0), osgn = E(0), upd = E(0);
 
-    // coordinate-bearing parameters of the last evaluation (theta 0,1,4..7)
-    E cth[9];
+    // coordinate-bearing parameters of the last evaluation (uniform)
+    E cth[NV];
     Geo<E, FT> g;
     E g2 = E(0);
     // lane i (< NV) holds row i of the Hessian and J_i; the reference keeps
     // the Hessian between iterations (a rejected step reuses it, a failed
     // Cholesky leaves its damped diagonal), so does this.
     E hrow[NV], jv = E(0);
-    const int myrow = lane < NV ? lane : NV - 1;
+    const int myrow = myl;
 
-    // one ROI evaluation at th: fills hrow / jv and returns the deviance
+    // one ROI evaluation at the current parameters: fills hrow / jv and
+    // returns the deviance
     auto evaluate = [&](bool first) __attribute__((always_inline)) -> E {
 #pragma unroll
-        for (int i = 0; i < NV; i++) cth[i] = th[i];
-        if constexpr (FT == 8) g2 = th[7];
+        for (int i = 0; i < NV; i++) cth[i] = readlane_e<E>(th, i);
+        if constexpr (FT == 8) g2 = cth[7];
         make_geo<E, FT>(A, cth, g);
         __syncthreads();   // the scratch of the last reduction is read out
         stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
@@ -136,7 +143,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         for (int p = lane; p < S2; p += 64) {
             const int pjj = p / S, pii = p - pjj * S;
             E dudt[8], model;
-            pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], th[2], th[3], g2, dudt,
+            pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], g2, dudt,
                                model);
             if constexpr (FT == 7) {
                 if (first) dudt[5] = dudt[4];
@@ -169,8 +176,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 
     if (A.theta_in) {
         // test hook: one evaluation at a given theta; write err, J, H
-#pragma unroll
-        for (int i = 0; i < NV; i++) th[i] = (E)A.theta_in[(size_t)n * NV + i];
+        th = (E)A.theta_in[(size_t)n * NV + myl];
         const E e = evaluate(false);
         float *o = A.dbg + (size_t)n * (1 + NV + NV * NV);
         if (lane == 0) o[0] = (float)e;
@@ -188,12 +194,9 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     for (kk = 0; kk < A.iterations; kk++) {
         if (fabs((newErr - oldErr) / newErr) < (E)1e-6 || (FT == 8 && sumUpdate < (E)1e-5)) break;   // TOLERANCE, TOLERANCE1
         if ((double)newErr > 1.5 * (double)oldErr) {
-#pragma unroll
-            for (int i = 0; i < NV; i++) {
-                sgn[i] = osgn[i];
-                clampv[i] = oclamp[i];
-                th[i] = oth[i];
-            }
+            sgn = osgn;
+            clampv = oclamp;
+            th = oth;
             newLambda = oldLambda;
             newErr = oldErr;
             newLambda = E(10) * newLambda;
@@ -209,90 +212,89 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         }
         int info = chol_solve_rows<NV>(arow, (double)jv, lane, xi);
         if (info == 0) {
-#pragma unroll
-            for (int l = 0; l < NV; l++) upd[l] = (E)readlane_d(xi, l);
+            upd = (E)xi;
             if constexpr (FT != 6) {
-#pragma unroll
-                for (int l = 0; l < NV; l++)
-                    if (isnan(upd[l])) info = 1;
+                if (__any(lane < NV && isnan(upd))) info = 1;
             }
         }
         if (info != 0) {
             newLambda = E(10) * newLambda;
             continue;
         }
-        if constexpr (FT == 7) upd[5] = upd[5] * E(10);
-#pragma unroll
-        for (int i = 0; i < NV; i++) {
-            osgn[i] = sgn[i];
-            oclamp[i] = clampv[i];
-            oth[i] = th[i];
-        }
+        if constexpr (FT == 7) upd = (myl == 5) ? upd * E(10) : upd;
+        osgn = sgn;
+        oclamp = clampv;
+        oth = th;
         oldLambda = newLambda;
         oldErr = newErr;
         if constexpr (FT == 7) {
-            if ((double)fabs(upd[4]) < 0.005 && turn) {
-                clampv[5] = (E)A.nd[3] / E(10);
-                oclamp[5] = (E)A.nd[3] / E(10);
+            const E u4 = readlane_e<E>(upd, 4);
+            if ((double)fabs(u4) < 0.005 && turn) {
+                if (myl == 5) clampv = oclamp = (E)A.nd[3] / E(10);
+                if (myl == 4) clampv = oclamp = E(2);
                 turn = 0;
-                clampv[4] = E(2);
-                oclamp[4] = E(2);
             }
         }
-        if constexpr (FT == 8) sumUpdate = E(0);
-#pragma unroll
-        for (int l = 0; l < NV; l++) {
-            if (sgn[l] != E(0)) {
-                if ((sgn[l] == E(1) && upd[l] < E(0)) || (sgn[l] == E(-1) && upd[l] > E(0)))
-                    clampv[l] = clampv[l] * fade[l];
-            }
-            sgn[l] = (upd[l] > E(0)) ? E(1) : E(-1);
-            if constexpr (FT == 8) {
-                upd[l] = upd[l] / (E(1) + fabs(upd[l] / clampv[l]));
-                th[l] = th[l] - upd[l];
-                sumUpdate += fabs(upd[l]);
-            } else {
-                th[l] = th[l] - upd[l] / (E(1) + fabs(upd[l] / clampv[l]));
-            }
+        if (sgn != E(0)) {
+            if ((sgn == E(1) && upd < E(0)) || (sgn == E(-1) && upd > E(0))) clampv = clampv * fade;
         }
+        sgn = (upd > E(0)) ? E(1) : E(-1);
         if constexpr (FT == 8) {
-            const double lo = (double)(((E)S - E(1)) / E(2)) - S / 4.0;
-            const double hi = (double)(((E)S - E(1)) / E(2)) + S / 4.0;
-            th[0] = dmaxr<E>(th[0], lo);
-            th[0] = dminr<E>(th[0], hi);
-            th[1] = dmaxr<E>(th[1], lo);
-            th[1] = dminr<E>(th[1], hi);
-        }
-        th[2] = dmaxr<E>(th[2], 1.0);
-        th[3] = dmaxr<E>(th[3], 0.01);
-        th[4] = dmaxr<E>(th[4], 0.0);
-        if constexpr (FT == 6) {
-            E zb = (E)(A.nd[2] - 4);
-            th[4] = (th[4] < zb) ? th[4] : zb;
+            upd = upd / (E(1) + fabs(upd / clampv));
+            th = th - upd;
+            sumUpdate = E(0);
+#pragma unroll
+            for (int l = 0; l < NV; l++) sumUpdate += fabs(readlane_e<E>(upd, l));
         } else {
-            th[4] = dminr<E>(th[4], (double)(E)A.nd[2] - 0.5);
+            th = th - upd / (E(1) + fabs(upd / clampv));
+        }
+        // bounds (CPUmleFit_LM.cpp:2631-2634, 3021-3026, 3449-3472), one per lane
+        if constexpr (FT == 8) {
+            if (myl <= 1) {
+                const double lo = (double)(((E)S - E(1)) / E(2)) - S / 4.0;
+                const double hi = (double)(((E)S - E(1)) / E(2)) + S / 4.0;
+                th = dminr<E>(dmaxr<E>(th, lo), hi);
+            }
+        }
+        if (myl == 2) th = dmaxr<E>(th, 1.0);
+        if (myl == 3) th = dmaxr<E>(th, 0.01);
+        if (myl == 4) {
+            th = dmaxr<E>(th, 0.0);
+            if constexpr (FT == 6) {
+                const E zb = (E)(A.nd[2] - 4);
+                th = (th < zb) ? th : zb;
+            } else {
+                th = dminr<E>(th, (double)(E)A.nd[2] - 0.5);
+            }
         }
         if constexpr (FT == 7) {
-            th[5] = dmaxr<E>(th[5], 0.0);
-            th[5] = dminr<E>(th[5], (double)(E)A.nd[3] - 0.5);
+            if (myl == 5) th = dminr<E>(dmaxr<E>(th, 0.0), (double)(E)A.nd[3] - 0.5);
         }
         if constexpr (FT == 8) {
-            if (th[5] <= E(1)) th[5] = (E)(A.nd[3] - 2) + th[5];
-            else if (th[5] > (E)(A.nd[3] - 1)) th[5] = (E(2) + th[5]) - (E)A.nd[3];
-            th[6] = (th[6] > E(1)) ? th[6] : E(1);
-            th[6] = dminr<E>(th[6], (double)(E)A.nd[4] - 0.5);
-            th[7] = (th[7] < E(1)) ? th[7] : E(1);
-            th[7] = (th[7] > E(0)) ? th[7] : E(0);
+            if (myl == 5) {
+                if (th <= E(1)) th = (E)(A.nd[3] - 2) + th;
+                else if (th > (E)(A.nd[3] - 1)) th = (E(2) + th) - (E)A.nd[3];
+            }
+            if (myl == 6) {
+                th = (th > E(1)) ? th : E(1);
+                th = dminr<E>(th, (double)(E)A.nd[4] - 0.5);
+            }
+            if (myl == 7) {
+                th = (th < E(1)) ? th : E(1);
+                th = (th > E(0)) ? th : E(0);
+            }
         }
         newErr = evaluate(false);
 #ifdef BSF_TRACE
-        if (lane == 0 && A.dbg) {
+        if (A.dbg && lane < NV) {
             float *o = A.dbg + ((size_t)n * 64 + kk) * (3 + 2 * NV);
-            o[0] = (float)kk;
-            o[1] = (float)newErr;
-            o[2] = (float)newLambda;
-            for (int i = 0; i < NV; i++) o[3 + i] = (float)th[i];
-            for (int i = 0; i < NV; i++) o[3 + NV + i] = (float)upd[i];
+            if (lane == 0) {
+                o[0] = (float)kk;
+                o[1] = (float)newErr;
+                o[2] = (float)newLambda;
+            }
+            o[3 + lane] = (float)th;
+            o[3 + NV + lane] = (float)upd;
         }
 #endif
     }
@@ -302,8 +304,11 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     // and g2 with the final N, bg (CPUmleFit_LM.cpp:3566-3583).
     if constexpr (FT != 8) {
 #pragma unroll
-        for (int i = 0; i < NV; i++) cth[i] = th[i];
+        for (int i = 0; i < NV; i++) cth[i] = readlane_e<E>(th, i);
         make_geo<E, FT>(A, cth, g);
+    } else {
+        cth[2] = readlane_e<E>(th, 2);
+        cth[3] = readlane_e<E>(th, 3);
     }
     __syncthreads();
     stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     for (int p = lane; p < S2; p += 64) {
         const int pjj = p / S, pii = p - pjj * S;
         E dudt[8], model;
-        pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], th[2], th[3], g2, dudt, model);
+        pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], g2, dudt, model);
         E data = (E)s_data[p];
         int h = 1;
 #pragma unroll
@@ -334,10 +339,11 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 #pragma unroll
     for (int j = 0; j < NV; j++) mrow[j] = (double)tot[1 + packed_index<NV>(myrow, j)];
     const double crlb = inverse_diag_rows<NV>(mrow, lane);
-    if (lane < NV) A.CRLB[(size_t)A.N * lane + n] = (float)crlb;
+    if (lane < NV) {
+        A.P[(size_t)A.N * lane + n] = (float)th;
+        A.CRLB[(size_t)A.N * lane + n] = (float)crlb;
+    }
     if (lane == 0) {
-#pragma unroll
-        for (int a = 0; a < NV; a++) A.P[(size_t)A.N * a + n] = (float)th[a];
         A.P[(size_t)A.N * NV + n] = (float)kk;
         A.LL[n] = (float)tot[0];
     }

@@ -919,7 +919,7 @@ int SYM(orc_fit_4pi)(const float *data, int sz, int Nfits, int nch, const int *s
  * (mu[sz*jj+ii], column-major like the data) are returned. */
 real SYM(orc_eval_pass)(int fittype, const float *roi, int sz, const float *coeff,
                         const int *cdims, const float *coeff3, const real *theta,
-                        real *J, real *H, real *mu)
+                        real *J, real *H, real *mu, int first)
 {
     fitctx *x = (fitctx *)malloc(sizeof(fitctx));
     real th[9], err;
@@ -934,7 +934,7 @@ real SYM(orc_eval_pass)(int fittype, const float *roi, int sz, const float *coef
     x->off = (real)(x->nd[0] - sz) / R(2.0);
     for (int i = 0; i < x->NV; i++) th[i] = theta[i];
     set_coords(x, th);
-    err = eval_pass(x, th, 0, J, H);
+    err = eval_pass(x, th, first, J, H);
     if (mu)
         for (int ii = 0; ii < sz; ii++)
             for (int jj = 0; jj < sz; jj++) {

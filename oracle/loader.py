@@ -277,9 +277,10 @@ def ref_matinv(M):
     return D, Minv.reshape(n, n, order="F")
 
 
-def eval_pass(fittype, roi, theta, coeff, coeff3=None, f64=False):
-    """One deviance / gradient / Hessian pass at theta (no LM step).
-    Returns (err, J, H, mu) with mu indexed [row, column]."""
+def eval_pass(fittype, roi, theta, coeff, coeff3=None, f64=False, first=False):
+    """One deviance / gradient / Hessian pass at theta (no LM step); first
+    selects the first pass's quirks.  Returns (err, J, H, mu) with mu
+    indexed [row, column]."""
     lib = _L()
     roi = _f32(roi)
     S = roi.shape[0]
@@ -298,7 +299,7 @@ def eval_pass(fittype, roi, theta, coeff, coeff3=None, f64=False):
     f = lib.orc_eval_pass_f64 if f64 else lib.orc_eval_pass
     f.restype = C.c_double if f64 else C.c_float
     err = f(C.c_int(fittype), _ptr(roi), C.c_int(S), _ptr(coeff), cd.ctypes.data_as(ip),
-            _ptr(c3), _ptr(th, t), _ptr(J, t), _ptr(H, t), _ptr(mu, t))
+            _ptr(c3), _ptr(th, t), _ptr(J, t), _ptr(H, t), _ptr(mu, t), C.c_int(1 if first else 0))
     return err, J, H.reshape(NV, NV), mu.reshape(S, S, order="F")
 
 

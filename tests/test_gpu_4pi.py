@@ -46,23 +46,3 @@ def test_4pi_matches_oracle(orc, shared):
     ll_ok = np.mean(gm[2] >= of[2] - 1e-3 * np.abs(of[2]) - 1e-3)
     print("LL(gpu MIX) >= LL(oracle F32) on %.3f" % ll_ok, flush=True)
     assert ll_ok >= 0.75
-
-
-@pytest.mark.parametrize("iters", [1, 2, 4])
-def test_4pi_short_trajectories_match_f64_oracle(orc, iters):
-    """The first LM steps (init, evaluation over channels, slot assembly,
-    solve, limiter, bounds and wrap) before rounding differences can grow:
-    the FP64 kernel against the FP64 oracle to 1e-5."""
-    import bsfit
-    from bsfit import gen
-    for shared in ((1, 1, 1, 1, 1, 1), (1, 1, 1, 0, 1, 1), (0, 1, 1, 0, 1, 1)):
-        w = gen.make_rois_4pi(32, 13, 11, gen.model_coeffs("4pi"), shared=shared)
-        args = (w.imstack, shared, iters, w.coeff, w.phaseShift)
-        kw = dict(initZ=w.initZ, initPhi=w.initPhi)
-        P, CR, LL = bsfit.fit_4pi(*args, precision=bsfit.PREC_F64, **kw)
-        Po, CRo, LLo = orc.fit_4pi(*args, nthreads=8, f64=True, **kw)
-        NV = CR.shape[1]
-        rel = np.abs(P[:, :NV] - Po[:, :NV]) / np.maximum(np.abs(Po[:, :NV]), 1.0)
-        ok = (rel.max(1) < 1e-5) & (P[:, NV] == Po[:, NV])
-        print("shared", shared, "iterations", iters, "agree", ok.mean(), flush=True)
-        assert ok.mean() >= 0.95

@@ -102,13 +102,16 @@ def test_single_roi_and_iterations_zero(orc):
     assert np.allclose(LL0, LLo, rtol=1e-5, atol=1e-3)
 
 
-@pytest.mark.parametrize("ft,cfg,S", [(6, 0, None), (7, 1, None), (8, 3, 11), (8, 3, 17)])
+@pytest.mark.parametrize("ft,cfg,S", [(6, 0, None), (8, 3, 11), (8, 3, 17)])
 @pytest.mark.parametrize("iters", [1, 2, 4])
 def test_short_trajectories_match_f64_oracle(orc, ft, cfg, S, iters):
     """The first LM steps (initial estimates, accept / reject, damping,
-    solve, limiter, clamp fading, bounds, wraps, the fittype-7 w scaling and
-    the fittype-8 extra convergence test) before rounding differences grow:
-    the FP64 kernel against the FP64 oracle to 1e-5."""
+    solve, limiter, clamp fading, bounds, wraps and the fittype-8 extra
+    convergence test) before rounding differences grow: the FP64 kernel
+    against the FP64 oracle to 1e-5.  Fittype 7 is absent on purpose: its
+    first pass copies dudt[4] into dudt[5] (CPUmleFit_LM.cpp:2896), so the
+    first Hessian has two equal rows and whether its Cholesky passes is
+    decided by rounding (test_oracle_pins.py::test_ft7_first_hessian_singular)."""
     import bsfit
     from bsfit import gen
     w = gen.config(cfg, N=48, S=S)

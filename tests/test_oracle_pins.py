@@ -164,3 +164,19 @@ def test_pass_gradient_and_hessian(orc, ft):
         fd = (orc.eval_pass(ft, roi, tp, w.coeff, f64=True)[0] - orc.eval_pass(ft, roi, tm, w.coeff, f64=True)[0]) / (2 * h)
         assert abs(J[k] - fd / 2) < 1e-4 * (1 + abs(fd))
     assert np.allclose(H, H.T) and np.all(np.linalg.eigvalsh(H) > -1e-6 * np.abs(H).max())
+
+
+def test_ft7_first_hessian_singular(orc):
+    """The reference's fittype-7 first pass sets dudt[5] = dudt[4]
+    (CPUmleFit_LM.cpp:2896): the first Hessian has equal rows 4 and 5 and is
+    singular, so the first Cholesky's success (and with it the whole
+    trajectory) rests on rounding.  This is why fittype-7 and 4Pi fits agree
+    across arithmetics only as rates."""
+    from bsfit import gen
+    w = gen.config(1, N=2)
+    th = w.truth[0].astype(np.float64)
+    err, J, H, _ = orc.eval_pass(7, w.imstack[:, :, 0], th, w.coeff, f64=True, first=True)
+    assert np.array_equal(H[4], H[5]) and J[4] == J[5]
+    assert abs(np.linalg.det(H)) <= 1e-9 * abs(np.prod(np.diag(H)))
+    err2, J2, H2, _ = orc.eval_pass(7, w.imstack[:, :, 0], th, w.coeff, f64=True, first=False)
+    assert not np.array_equal(H2[4], H2[5])
