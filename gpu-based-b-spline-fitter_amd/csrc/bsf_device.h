@@ -47,34 +47,12 @@ __device__ __forceinline__ E bsp2(E x)
 // idx 1, 2) and the 4-tap stencils over taps k-2..k+1:
 //   W  = value weights, Dv = derivative weights (forward differences with
 //   quadratic weights rewritten as a stencil on the coefficients).
+// (Filled by make_geo.)
 template <typename E>
 struct Axis {
     E B1[3], B2[3], dB1[3], dB2[3];
     E W[4], Dv[4];
 };
-
-template <typename E>
-__device__ __forceinline__ void axis_factors(E d, Axis<E> &a)
-{
-    E dd = d - E(1);
-#pragma unroll
-    for (int idx = 1; idx <= 2; idx++) {
-        a.B1[idx] = bsp3<E>(d + (E)idx - E(1));
-        a.B2[idx] = bsp3<E>(d - (E)idx);
-        a.dB1[idx] = bsp2<E>(dd + (E)idx - E(0.5));
-        a.dB2[idx] = bsp2<E>(dd - (E)idx + E(0.5));
-    }
-    a.W[0] = a.B1[2];
-    a.W[1] = a.B1[1];
-    a.W[2] = a.B2[1];
-    a.W[3] = a.B2[2];
-    // sum_t (c[t+1]-c[t]) dW[t], dW = (dB1[2], dB1[1], dB2[1], dB2[2]);
-    // dB2[2] = beta2(d - 2.5) = 0 for d in [0,1), so tap k+2 drops out.
-    a.Dv[0] = -a.dB1[2];
-    a.Dv[1] = a.dB1[2] - a.dB1[1];
-    a.Dv[2] = a.dB1[1] - a.dB2[1];
-    a.Dv[3] = a.dB2[1] - a.dB2[2];
-}
 
 // Clamped base tap index for one coordinate (CPUsplineLib.cpp:1203-1225):
 // floor, clamp to [1, n-1], coordinate == n selects n; returns k = kc + 1.
@@ -112,35 +90,120 @@ struct Dev {
     int has_initP;
 };
 
+// LDS layout of a fit workgroup (one wave): ROI data (nch channels), pixel
+// (row, column) table, per-row / per-column tap offsets, axis factors, then
+// the planes (reused as reduction scratch) and kernel-specific extras.
+struct LdsLayout {
+    size_t pix, rtap, ctap, fac, pl, total;
+};
+
+__host__ __device__ inline size_t lds_align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline LdsLayout lds_layout(int S, int nch, size_t esz, size_t pl_elems)
+{
+    LdsLayout L;
+    size_t o = lds_align16((size_t)nch * S * S * 4);
+    L.pix = o;
+    o = lds_align16(o + (size_t)S * S * 4);
+    L.rtap = o;
+    o += 64 * 4;
+    L.ctap = o;
+    o += 64 * 4;
+    L.fac = o;
+    o = lds_align16(o + 40 * esz);
+    L.pl = o;
+    L.total = o + esz * pl_elems;
+    return L;
+}
+
+// Per-wave LDS workspace of one evaluation's geometry.
+template <typename E>
+struct GeoLds {
+    E *fac;      // [8*D] per-axis factors B1[1],B1[2],B2[1],B2[2],dB1[1],dB1[2],dB2[1],dB2[2]
+    int *rtap;   // [S] window row offset of tap k-2, per ROI row
+    int *ctap;   // [S] window column offset of tap k-2, per ROI column
+};
+
+// Geometry of one evaluation, computed across the wave: lane 8a+f evaluates
+// factor f of axis a (the reference's per-axis B-spline values, in its
+// operation order), lane i evaluates the clamped taps of ROI row / column i
+// (CPUsplineLib.cpp:1203-1225, per pixel exactly as the reference), and the
+// window spans the taps of all rows / columns.  Every lane must call it.
 template <typename E, int FT>
-__device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> &g)
+__device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> &g, int lane, const GeoLds<E> &L)
 {
     constexpr int D = Ft<FT>::D;
     const int S = A.S;
     const E off = (E)(A.nd[0] - S) / E(2);
     const E colc = (E)((S + 1) / 2) - cth[0];
     const E rowc = (E)((S + 1) / 2) - cth[1];
-    axis_factors<E>(rowc - floor(rowc), g.ax[0]);
-    axis_factors<E>(colc - floor(colc), g.ax[1]);
+    if (lane < 8 * D) {
+        const int a = lane >> 3, f = lane & 7, idx = (f & 1) + 1;
+        E c = rowc;
+        c = (a == 1) ? colc : c;
 #pragma unroll
-    for (int a = 2; a < D; a++) {
-        E c = cth[2 + a];
-        axis_factors<E>(c - floor(c), g.ax[a]);
-        g.knx[a - 2] = base_tap<E>(c, A.nd[a]);
+        for (int i = 2; i < D; i++) c = (a == i) ? cth[2 + i] : c;
+        const E d = c - floor(c), dd = d - E(1);
+        E v;
+        if (f < 2) v = bsp3<E>(d + (E)idx - E(1));
+        else if (f < 4) v = bsp3<E>(d - (E)idx);
+        else if (f < 6) v = bsp2<E>(dd + (E)idx - E(0.5));
+        else v = bsp2<E>(dd - (E)idx + E(0.5));
+        L.fac[lane] = v;
     }
-    int rmin = 1 << 30, rmax = -1, cmin = 1 << 30, cmax = -1;
-    for (int i = 0; i < S; i++) {
-        int kr = base_tap<E>(rowc + (E)i + off, A.nd[1]);
-        int kq = base_tap<E>(colc + (E)i + off, A.nd[0]);
-        rmin = min(rmin, kr);
-        rmax = max(rmax, kr);
-        cmin = min(cmin, kq);
-        cmax = max(cmax, kq);
+    // per row / column taps; the sequence is non-decreasing except where a
+    // coordinate equals n exactly (tap n+1, the maximum; all later ones are n)
+    const int li = lane < S ? lane : S - 1;
+    const E rc = rowc + (E)li + off, cc = colc + (E)li + off;
+    const int n1 = A.nd[1], n0 = A.nd[0];
+    const int kr = base_tap<E>(rc, n1), kq = base_tap<E>(cc, n0);
+    const unsigned long long sr = __ballot(lane < S && rc == (E)n1);
+    const unsigned long long sq = __ballot(lane < S && cc == (E)n0);
+    const int kr0 = __builtin_amdgcn_readlane(kr, 0), krl = __builtin_amdgcn_readlane(kr, S - 1);
+    const int kq0 = __builtin_amdgcn_readlane(kq, 0), kql = __builtin_amdgcn_readlane(kq, S - 1);
+    int rmin = kr0, rmax = krl, cmin = kq0, cmax = kql;
+    if (sr) {
+        rmax = n1 + 1;
+        rmin = (sr & 1ull) ? (S > 1 ? n1 : n1 + 1) : kr0;
+    }
+    if (sq) {
+        cmax = n0 + 1;
+        cmin = (sq & 1ull) ? (S > 1 ? n0 : n0 + 1) : kq0;
     }
     g.row0 = rmin - 2;
     g.col0 = cmin - 2;
     g.PR = rmax - rmin + 4;
     g.PC = cmax - cmin + 4;
+    if (lane < S) {
+        L.rtap[lane] = kr - rmin;
+        L.ctap[lane] = kq - cmin;
+    }
+#pragma unroll
+    for (int a = 2; a < D; a++) g.knx[a - 2] = base_tap<E>(cth[2 + a], A.nd[a]);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        Axis<E> &x = g.ax[a];
+        const E *f = L.fac + 8 * a;
+        x.B1[1] = f[0];
+        x.B1[2] = f[1];
+        x.B2[1] = f[2];
+        x.B2[2] = f[3];
+        x.dB1[1] = f[4];
+        x.dB1[2] = f[5];
+        x.dB2[1] = f[6];
+        x.dB2[2] = f[7];
+        x.W[0] = x.B1[2];
+        x.W[1] = x.B1[1];
+        x.W[2] = x.B2[1];
+        x.W[3] = x.B2[2];
+        // sum_t (c[t+1]-c[t]) dW[t], dW = (dB1[2], dB1[1], dB2[1], dB2[2]);
+        // dB2[2] = beta2(d - 2.5) = 0 for d in [0,1), so tap k+2 drops out.
+        x.Dv[0] = -x.dB1[2];
+        x.Dv[1] = x.dB1[2] - x.dB1[1];
+        x.Dv[2] = x.dB1[1] - x.dB2[1];
+        x.Dv[3] = x.dB2[1] - x.dB2[2];
+    }
 }
 
 // ------------------------------------------------------------ stage A
@@ -307,16 +370,10 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
 // --------------------------------------------------------- stage B + C
 // Model and dudt at one pixel from the planes.
 template <typename E, int FT>
-__device__ __forceinline__ void pixel_model(const Dev &A, const Geo<E, FT> &g, const E *planes, int plane_stride,
-                                            int ii, int jj, E cth0, E cth1, E Nph, E bg, E g2, E *dudt, E &model)
+__device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes, int plane_stride, int r, int c,
+                                            E Nph, E bg, E g2, E *dudt, E &model)
 {
     constexpr int D = Ft<FT>::D;
-    const int S = A.S;
-    const E off = (E)(A.nd[0] - S) / E(2);
-    const E colc = (E)((S + 1) / 2) - cth0;
-    const E rowc = (E)((S + 1) / 2) - cth1;
-    const int r = base_tap<E>(rowc + (E)ii + off, A.nd[1]) - 2 - g.row0;
-    const int c = base_tap<E>(colc + (E)jj + off, A.nd[0]) - 2 - g.col0;
     const int PR = g.PR;
     const Axis<E> &ay = g.ax[0], &ax = g.ax[1];
     const E *p0 = planes + c * PR + r;

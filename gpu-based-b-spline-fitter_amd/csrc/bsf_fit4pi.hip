@@ -55,9 +55,11 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     const int S = A.S, S2 = S * S, nch = A4.nch, nv = A4.nv;
     const int W = S + 4, plane_stride = W * W;
     extern __shared__ __align__(16) unsigned char smem[];
+    const LdsLayout LL = lds_layout(S, nch, sizeof(E), 0);
     float *s_data = (float *)smem;
-    size_t off = (((size_t)nch * S2 * 4) + 15) & ~(size_t)15;
-    E *s_pl = (E *)(smem + off);
+    int *s_pix = (int *)(smem + LL.pix);
+    const GeoLds<E> gl = {(E *)(smem + LL.fac), (int *)(smem + LL.rtap), (int *)(smem + LL.ctap)};
+    E *s_pl = (E *)(smem + LL.pl);
     const size_t plred = (size_t)3 * plane_stride > (size_t)29 * 65 ? (size_t)3 * plane_stride : (size_t)29 * 65;
     E *s_red = s_pl;
     E *s_H = s_pl + plred;          // nv * nv
@@ -67,6 +69,10 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     for (int p = lane; p < nch * S2; p += 64) {
         const int k = p / S2, q = p - k * S2;
         s_data[p] = A.data[((size_t)k * A.N + n) * S2 + q];
+    }
+    for (int p = lane; p < S2; p += 64) {
+        const int jj = p / S;
+        s_pix[p] = (p - jj * S) | (jj << 16);   // row | column << 16
     }
     __syncthreads();
 
@@ -139,18 +145,18 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
         for (int k = 0; k < nch; k++) {
             E cth[6];
             channel_params(k, cth);
-            make_geo<E, 7>(A, cth, g);
             __syncthreads();
+            make_geo<E, 7>(A, cth, g, lane, gl);
             stage_a<E, 7>(A, g, s_pl, plane_stride, lane);
             __syncthreads();
             E acc[28];
 #pragma unroll
             for (int q = 0; q < 28; q++) acc[q] = E(0);
             for (int p = lane; p < S2; p += 64) {
-                const int pjj = p / S, pii = p - pjj * S;
+                const int pix = s_pix[p];
                 E dudt[8], model;
-                pixel_model<E, 7>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], E(0), dudt,
-                                  model);
+                pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3],
+                                  E(0), dudt, model);
                 if (first) dudt[5] = dudt[4];
                 E data = (E)s_data[k * S2 + p];
                 if (data > E(0)) {
@@ -159,8 +165,10 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
                     acc[0] += E(2) * model;
                     data = E(0);
                 }
-                E t1 = E(1) - data / model;
-                E t2 = data / (model * model);
+                const E rm = E(1) / model;
+                const E dr = data * rm;
+                E t1 = E(1) - dr;
+                E t2 = dr * rm;
 #pragma unroll
                 for (int l = 0; l < 6; l++) acc[1 + l] += t1 * dudt[l];
                 int h = 7;
@@ -250,17 +258,18 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     for (int k = 0; k < nch; k++) {
         E cth[6];
         channel_params(k, cth);
-        make_geo<E, 7>(A, cth, g);
         __syncthreads();
+        make_geo<E, 7>(A, cth, g, lane, gl);
         stage_a<E, 7>(A, g, s_pl, plane_stride, lane);
         __syncthreads();
         E acc[22];
 #pragma unroll
         for (int q = 0; q < 22; q++) acc[q] = E(0);
         for (int p = lane; p < S2; p += 64) {
-            const int pjj = p / S, pii = p - pjj * S;
+            const int pix = s_pix[p];
             E dudt[8], model;
-            pixel_model<E, 7>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], E(0), dudt, model);
+            pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3], E(0),
+                              dudt, model);
             E data = (E)s_data[k * S2 + p];
             int h = 1;
 #pragma unroll
@@ -300,8 +309,8 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 size_t lds_bytes_4pi(int f64, int S, int nch, int nv)
 {
     const size_t W = S + 4;
-    size_t plred = 3 * W * W > 29 * 65 ? 3 * W * W : 29 * 65;
-    return ((((size_t)nch * S * S * 4) + 15) & ~(size_t)15) + (f64 ? 8 : 4) * (plred + nv * nv + nv + 1);
+    const size_t plred = 3 * W * W > 29 * 65 ? 3 * W * W : 29 * 65;
+    return lds_layout(S, nch, f64 ? 8 : 4, plred + (size_t)nv * nv + nv + 1).total;
 }
 
 template <typename E, int NVMAX>

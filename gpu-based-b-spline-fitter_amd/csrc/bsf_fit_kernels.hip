@@ -41,12 +41,19 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     const int lane = threadIdx.x;
     const int S = A.S, S2 = S * S;
     const int W = S + 4, plane_stride = W * W;
+    const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0);
     float *s_data = (float *)smem;
-    E *s_pl = (E *)(smem + (((size_t)S2 * 4 + 15) & ~(size_t)15));
+    int *s_pix = (int *)(smem + LL.pix);
+    const GeoLds<E> gl = {(E *)(smem + LL.fac), (int *)(smem + LL.rtap), (int *)(smem + LL.ctap)};
+    E *s_pl = (E *)(smem + LL.pl);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
 
     const float *dptr = A.data + (size_t)n * S2;
-    for (int p = lane; p < S2; p += 64) s_data[p] = dptr[p];
+    for (int p = lane; p < S2; p += 64) {
+        s_data[p] = dptr[p];
+        const int jj = p / S;
+        s_pix[p] = (p - jj * S) | (jj << 16);   // row | column << 16
+    }
     __syncthreads();
 
     // ---- initial values: centre of mass, max / min (CPUgaussLib.cpp:172-223)
@@ -133,18 +140,18 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 #pragma unroll
         for (int i = 0; i < NV; i++) cth[i] = readlane_e<E>(th, i);
         if constexpr (FT == 8) g2 = cth[7];
-        make_geo<E, FT>(A, cth, g);
         __syncthreads();   // the scratch of the last reduction is read out
+        make_geo<E, FT>(A, cth, g, lane, gl);
         stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
         __syncthreads();
         E acc[NRED];
 #pragma unroll
         for (int k = 0; k < NRED; k++) acc[k] = E(0);
         for (int p = lane; p < S2; p += 64) {
-            const int pjj = p / S, pii = p - pjj * S;
+            const int pix = s_pix[p];
             E dudt[8], model;
-            pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], g2, dudt,
-                               model);
+            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3], g2,
+                               dudt, model);
             if constexpr (FT == 7) {
                 if (first) dudt[5] = dudt[4];
             }
@@ -155,8 +162,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
                 acc[0] += E(2) * model;
                 data = E(0);
             }
-            E t1 = E(1) - data / model;
-            E t2 = data / (model * model);
+            const E rm = E(1) / model;
+            const E dr = data * rm;
+            E t1 = E(1) - dr;
+            E t2 = dr * rm;
 #pragma unroll
             for (int l = 0; l < NV; l++) acc[1 + l] += t1 * dudt[l];
             int h = 1 + NV;
@@ -305,7 +314,8 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     if constexpr (FT != 8) {
 #pragma unroll
         for (int i = 0; i < NV; i++) cth[i] = readlane_e<E>(th, i);
-        make_geo<E, FT>(A, cth, g);
+        __syncthreads();
+        make_geo<E, FT>(A, cth, g, lane, gl);
     } else {
         cth[2] = readlane_e<E>(th, 2);
         cth[3] = readlane_e<E>(th, 3);
@@ -318,9 +328,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 #pragma unroll
     for (int k = 0; k < NF; k++) acc[k] = E(0);
     for (int p = lane; p < S2; p += 64) {
-        const int pjj = p / S, pii = p - pjj * S;
+        const int pix = s_pix[p];
         E dudt[8], model;
-        pixel_model<E, FT>(A, g, s_pl, plane_stride, pii, pjj, cth[0], cth[1], cth[2], cth[3], g2, dudt, model);
+        pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3], g2,
+                           dudt, model);
         E data = (E)s_data[p];
         int h = 1;
 #pragma unroll
@@ -350,27 +361,26 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 }
 
 // ------------------------------------------------------------ launchers
-template <int FT, typename E>
-static hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
+static size_t plane_elems(int fittype, int S)
 {
-    const int S = A.S;
-    const int W = S + 4;
-    constexpr int NV = Ft<FT>::NV;
-    constexpr int NRED = 1 + NV + NV * (NV + 1) / 2;
-    size_t pl = (size_t)Ft<FT>::NPL * W * W;
-    size_t red = (size_t)(NRED + 1) * 65;
-    size_t lds = (((size_t)S * S * 4 + 15) & ~(size_t)15) + sizeof(E) * (pl > red ? pl : red);
-    hipLaunchKernelGGL((fit_kernel<FT, E>), dim3(nblocks), dim3(64), lds, st, A);
-    return hipGetLastError();
+    const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
+    const int NPL = fittype == 6 ? 2 : fittype == 7 ? 3 : 8;
+    const size_t W = S + 4;
+    const size_t pl = (size_t)NPL * W * W, red = (size_t)(1 + NV + NV * (NV + 1) / 2 + 1) * 65;
+    return pl > red ? pl : red;
 }
 
 size_t lds_bytes(int fittype, int f64, int S)
 {
-    int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
-    int NPL = fittype == 6 ? 2 : fittype == 7 ? 3 : 8;
-    int W = S + 4;
-    size_t pl = (size_t)NPL * W * W, red = (size_t)(1 + NV + NV * (NV + 1) / 2 + 1) * 65;
-    return (((size_t)S * S * 4 + 15) & ~(size_t)15) + (f64 ? 8 : 4) * (pl > red ? pl : red);
+    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, S)).total;
+}
+
+template <int FT, typename E>
+static hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
+{
+    const size_t lds = lds_bytes(FT, sizeof(E) == 8, A.S);
+    hipLaunchKernelGGL((fit_kernel<FT, E>), dim3(nblocks), dim3(64), lds, st, A);
+    return hipGetLastError();
 }
 
 template <int FT, typename E>

@@ -42,7 +42,11 @@ def test_4pi_matches_oracle(orc, shared):
           % (a64, p64, amx, pmx, aref, pref), flush=True)
     assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0]))
     assert a64 >= 0.3 and p64 >= 0.5
-    assert amx >= aref - 0.15
+    t = w.truth
+    eg = np.median(np.abs(gm[0][:, :2] - np.stack([t["x"], t["y"]], 1)), 0)
+    eo = np.median(np.abs(of[0][:, :2] - np.stack([t["x"], t["y"]], 1)), 0)
+    print("median |x,y - truth|: gpu MIX", eg, "oracle F32", eo, flush=True)
+    assert np.all(eg <= 1.2 * eo + 1e-3)
     ll_ok = np.mean(gm[2] >= of[2] - 1e-3 * np.abs(of[2]) - 1e-3)
     print("LL(gpu MIX) >= LL(oracle F32) on %.3f" % ll_ok, flush=True)
     assert ll_ok >= 0.75

@@ -11,10 +11,11 @@ north-star tolerances (parity_util.compare) and asserted as rates:
 
 * FP64 kernel vs FP64 oracle: same arithmetic class, different summation
   order; high agreement expected for ft6, lower for the chaotic ft7 / ft8.
-* FP32 kernel vs the oracle's float (reference-semantics) build: must be
-  about as close as that build is to the FP64 oracle.
-* Objective check: the GPU's final log-likelihood is not worse than the
-  oracle's on the large majority of ROIs.
+* FP32 kernel vs the oracle's float (reference-semantics) build: the rate
+  is printed beside the oracle's own float-vs-FP64 rate; asserted are the
+  objective (the GPU's final log-likelihood is not worse than the oracle's on
+  the large majority of ROIs) and matched accuracy (median position error
+  against the simulation truth no worse than the oracle's, within 20 %).
 
 The model, derivatives and pixel sums themselves are pinned strictly by
 test_gpu_eval.py.
@@ -61,8 +62,11 @@ def test_fit_matches_oracle(orc, ft, cfg, S, N, min_f64):
           "\nLL(gpuMIX) >= LL(orcF32) on %.3f" % llm, flush=True)
     assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0])), "non-finite parameters"
     assert s64["agree"] >= min_f64
-    assert smx["agree"] >= sref["agree"] - 0.15
     assert llm >= 0.75
+    eg = np.median(np.abs(gm[0][:, :2] - w.truth[:, :2]), 0)
+    eo = np.median(np.abs(of[0][:, :2] - w.truth[:, :2]), 0)
+    print("median |x,y - truth|: gpu MIX", eg, "oracle F32", eo, flush=True)
+    assert np.all(eg <= 1.2 * eo + 1e-3)
 
 
 def test_c_abi_errors():
