@@ -23,6 +23,7 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 FP32_PEAK = 157.3e12     # MI355X FP32 vector, FLOP/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fit_kernel_colour4d_traffic.md")
 
 
 def f_alg(ft, S, nch=1, NV=None):
@@ -60,6 +61,32 @@ def cpu_baseline(w, budget_s=15.0):
                       % (n, w.N, t)}
 
 
+def traffic_bytes():
+    """HBM bytes per launch of the fit kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy),
+    or None when no summary is present."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """A plain device copy of 256 MiB: the reference point for scaling the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,7 +95,12 @@ def main():
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--precision", choices=["mix", "f64"], default="mix")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 256 MiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(ROIs shard across GPUs with no exchange, see DESIGN.md)" % args.gpus, file=sys.stderr)
@@ -137,9 +169,10 @@ def main():
                    "fittype": 7, "rois": w.N, "roi_size": w.S, "iterations": w.iterations,
                    "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": None,
+                     "frac": achieved / FP32_PEAK, "traffic": traffic_bytes(),
                      "note": "FP32 vector-ALU bound (no MFMA on this path); peak = MI355X FP32 157.3 TFLOP/s; "
-                             "achieved = sum(iterations+2) x F_alg(S=15) = %d flop per ROI-evaluation / kernel time"
+                             "achieved = sum(iterations+2) x F_alg(S=15) = %d flop per ROI-evaluation / kernel time; "
+                             "traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)"
                              % f_alg(7, w.S)},
         "cpu_baseline": cpu,
     }
