@@ -87,6 +87,126 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
 
 
+SERVE_SIZES = (11, 13, 15, 17)
+SERVE_KINDS = ("ft6", "ft7", "4pi", "ft8")
+SERVE_MIX = (0.4, 0.2, 0.2, 0.2)
+
+
+def serve(args):
+    """Streaming workload: frames arrive as a Poisson process (args.rate per
+    second, seeded exponential gaps), each with Poisson(args.rois) ROIs drawn
+    40/20/20/20 from the 3D / colour-4D / 4Pi / vortex-5D fitters and
+    uniformly from ROI sizes 11..17.  Every model stays resident in one
+    serving session; a frame is one bsf_submit (one launch per model, mixed
+    sizes) and its latency runs from its arrival time to the moment its
+    results are in host memory (bsf_poll).  A second, closed-loop phase keeps
+    args.depth frames in flight to find the sustainable rate of the mix."""
+    import collections
+    import numpy as np
+    import bsfit
+    from bsfit import gen
+
+    seed = gen.SEED0 + 4000
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    c3 = gen.model_coeffs("astig3d")
+    c4 = gen.model_coeffs("color4d")
+    c4pi = gen.model_coeffs("4pi")
+    c5, c53 = gen.model_coeffs("dipole5d")
+    pool_n = 768
+    pools = {}
+    for S in SERVE_SIZES:
+        pools["ft6", S] = gen.make_rois(6, pool_n, S, seed + 10 + S, c3)
+        pools["ft7", S] = gen.make_rois(7, pool_n, S, seed + 20 + S, c4)
+        pools["4pi", S] = gen.make_rois_4pi(pool_n, S, seed + 30 + S, c4pi)
+        pools["ft8", S] = gen.make_rois(8, pool_n, S, seed + 40 + S, c5, c53)
+    prec = bsfit.PREC_F64 if args.precision == "f64" else bsfit.PREC_MIX
+    sess = bsfit.Session(arena_bytes=256 << 20, kernel_streams=args.kernel_streams, precision=prec)
+    w4 = pools["4pi", 11]
+    models = {"ft6": sess.register_model(6, c3), "ft7": sess.register_model(7, c4),
+              "4pi": sess.register_model(bsfit.FT_4PI, c4pi, nch=w4.nch, shared=w4.shared,
+                                         phaseShift=w4.phaseShift),
+              "ft8": sess.register_model(8, c5, c53)}
+    cells = [(k, S) for k in SERVE_KINDS for S in SERVE_SIZES]
+    probs = np.array([SERVE_MIX[SERVE_KINDS.index(k)] / len(SERVE_SIZES) for k, S in cells])
+
+    def make_frame():
+        n = int(rng.poisson(args.rois))
+        groups = []
+        for (k, S), c in zip(cells, rng.multinomial(n, probs)):
+            c = int(min(c, pool_n))
+            if c == 0:
+                continue
+            w = pools[k, S]
+            a = int(rng.integers(0, pool_n - c + 1))
+            if k == "4pi":
+                groups.append(bsfit.Group(models[k], w.imstack[:, :, a:a + c, :], w.iterations,
+                                          initZ=w.initZ[a:a + c], initPhi=w.initPhi[a:a + c]))
+            else:
+                groups.append(bsfit.Group(models[k], w.imstack[:, :, a:a + c], w.iterations, initP=w.initP))
+        return groups, sum(int(np.shape(g.imstack)[2]) for g in groups)
+
+    frames = [make_frame() for _ in range(args.frames)]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, args.frames))
+    print("serve: models, pools and %d frames ready in %.1f s (mean %.0f ROIs/frame)"
+          % (args.frames, time.perf_counter() - t0, np.mean([f[1] for f in frames])), file=sys.stderr, flush=True)
+    for g, _ in frames[:20]:   # warm-up: code objects loaded, streams and events created
+        sess.poll(sess.submit(g), wait=True)
+
+    # ---- open loop at the offered rate
+    lat = []
+    pending = collections.deque()
+    i = 0
+    start = time.perf_counter()
+    last = start
+    while i < args.frames or pending:
+        now = time.perf_counter() - start
+        if i < args.frames and now >= arrivals[i]:
+            pending.append((sess.submit(frames[i][0]), arrivals[i]))
+            i += 1
+            continue
+        for item in list(pending):
+            if sess.poll(item[0]) is not None:
+                last = time.perf_counter()
+                lat.append(last - start - item[1])
+                pending.remove(item)
+    span = last - start
+    nroi = sum(f[1] for f in frames)
+    lat_ms = np.array(lat) * 1e3
+
+    # ---- closed loop: args.depth frames in flight
+    pending = collections.deque()
+    cstart = time.perf_counter()
+    for g, _ in frames:
+        if len(pending) >= args.depth:
+            sess.poll(pending.popleft(), wait=True)
+        pending.append(sess.submit(g))
+    while pending:
+        sess.poll(pending.popleft(), wait=True)
+    cspan = time.perf_counter() - cstart
+    sess.close()
+    out = {
+        "metric": "serving: ROI fits/sec and frame latency, streamed 3D/4D/4Pi/5D mix",
+        "value": nroi / span,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "frames": args.frames,
+        "higher_is_better": True,
+        "dtype": "f32" if args.precision == "mix" else "f64",
+        "data": "synthetic (seeded spline-rendered ROI pools + Poisson noise)",
+        "config": {"workload": "frames at %g/s (Poisson arrivals), Poisson(%d) ROIs per frame, fitters "
+                               "40/20/20/20 3D/colour-4D/4Pi(4 ch)/vortex-5D, S uniform in {11,13,15,17}, "
+                               "50 LM iterations" % (args.rate, args.rois),
+                   "kernel_streams": args.kernel_streams, "precision": args.precision},
+        "offered_fits_per_s": nroi / float(arrivals[-1]),
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p90": float(np.percentile(lat_ms, 90)),
+                       "p99": float(np.percentile(lat_ms, 99)), "max": float(lat_ms.max()),
+                       "mean": float(lat_ms.mean())},
+        "saturated": {"fits_per_s": nroi / cspan, "frames_per_s": args.frames / cspan, "depth": args.depth},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +217,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--workload", choices=["fit", "serve"], default="fit",
+                    help="fit: the colour-4D batch fit (default); serve: the streamed mix through a session")
+    ap.add_argument("--frames", type=int, default=3000, help="serve: frames")
+    ap.add_argument("--rate", type=float, default=1000.0, help="serve: frame arrivals per second")
+    ap.add_argument("--rois", type=int, default=2000, help="serve: mean ROIs per frame")
+    ap.add_argument("--kernel-streams", type=int, default=2, help="serve: session kernel streams")
+    ap.add_argument("--depth", type=int, default=8, help="serve: frames in flight in the closed-loop phase")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -105,6 +232,9 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(ROIs shard across GPUs with no exchange, see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serve":
+        serve(args)
+        return
 
     import numpy as np
     import torch

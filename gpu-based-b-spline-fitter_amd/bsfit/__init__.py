@@ -14,3 +14,4 @@ from .api import (  # noqa: F401
     BsfError, GPUmleFit_LM_multiD, GPUmleFit_LM_4Pi_bspline, fit, fit_4pi, fit_device,
     lib_path, load, num_params, num_params_4pi, PREC_MIX, PREC_F64,
 )
+from .session import FT_4PI, Group, Session  # noqa: F401

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bsf_kernels.h"
+
 namespace bsf {
 
 template <int FT>
@@ -77,8 +79,19 @@ struct Geo {
     int PR, PC;        // window extent
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8) and each XCD has its own L2.  This gives each XCD a contiguous range
+// of ROIs, so the column-major output stores of neighbouring ROIs meet in one
+// L2 and are written back as full lines.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_block(int b, int G)
+{
+    const int x = b & 7, q = b >> 3, base = G >> 3, rem = G & 7;
+    return x * base + (x < rem ? x : rem) + q;
+}
+
 struct Dev {
     const float *data, *coeff, *coeff3;
+    const RoiDesc *rois;   // per-ROI placement, or NULL: uniform S, ROI n at data + n*S*S
     float *P, *CRLB, *LL;
     int N, S, iterations, roi_offset;
     int nd[5];

@@ -48,10 +48,17 @@ __device__ __forceinline__ int packed6(int p, int q)
 template <typename E, int NVMAX>
 __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 {
-    const Dev &A = A4.b;
-    const int n = blockIdx.x + A.roi_offset;
+    Dev A = A4.b;
+    const int n = xcd_block(blockIdx.x, gridDim.x) + A.roi_offset;
     if (n >= A.N) return;
     const int lane = threadIdx.x;
+    size_t dbase = (size_t)n * A.S * A.S, cstride = (size_t)A.N * A.S * A.S;
+    if (A.rois) {
+        const RoiDesc d = A.rois[n];
+        A.S = d.S;
+        dbase = (size_t)d.off;
+        cstride = (size_t)d.cstride;
+    }
     const int S = A.S, S2 = S * S, nch = A4.nch, nv = A4.nv;
     const int W = S + 4, plane_stride = W * W;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -68,7 +75,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 
     for (int p = lane; p < nch * S2; p += 64) {
         const int k = p / S2, q = p - k * S2;
-        s_data[p] = A.data[((size_t)k * A.N + n) * S2 + q];
+        s_data[p] = A.data[dbase + k * cstride + q];
     }
     for (int p = lane; p < S2; p += 64) {
         const int jj = p / S;
@@ -327,6 +334,7 @@ hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st)
     A.data = L.data;
     A.coeff = L.coeff;
     A.coeff3 = nullptr;
+    A.rois = L.rois;
     A.P = L.P;
     A.CRLB = L.CRLB;
     A.LL = L.LL;

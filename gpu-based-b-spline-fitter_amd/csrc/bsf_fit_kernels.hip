@@ -31,14 +31,19 @@ template <int FT, typename E>
 __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 {
     constexpr int NV = Ft<FT>::NV;
-    constexpr int NPL = Ft<FT>::NPL;
     constexpr int NH = NV * (NV + 1) / 2;
     constexpr int NRED = 1 + NV + NH;
     extern __shared__ __align__(16) unsigned char smem[];
 
-    const int n = blockIdx.x + A.roi_offset;
+    const int n = xcd_block(blockIdx.x, gridDim.x) + A.roi_offset;
     if (n >= A.N) return;
     const int lane = threadIdx.x;
+    const float *dptr = A.data + (size_t)n * A.S * A.S;
+    if (A.rois) {
+        const RoiDesc d = A.rois[n];
+        A.S = d.S;
+        dptr = A.data + d.off;
+    }
     const int S = A.S, S2 = S * S;
     const int W = S + 4, plane_stride = W * W;
     const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0);
@@ -48,7 +53,6 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     E *s_pl = (E *)(smem + LL.pl);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
 
-    const float *dptr = A.data + (size_t)n * S2;
     for (int p = lane; p < S2; p += 64) {
         s_data[p] = dptr[p];
         const int jj = p / S;
@@ -378,7 +382,7 @@ size_t lds_bytes(int fittype, int f64, int S)
 template <int FT, typename E>
 static hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
 {
-    const size_t lds = lds_bytes(FT, sizeof(E) == 8, A.S);
+    const size_t lds = lds_bytes(FT, sizeof(E) == 8, A.S);   // A.S = largest ROI size of the launch
     hipLaunchKernelGGL((fit_kernel<FT, E>), dim3(nblocks), dim3(64), lds, st, A);
     return hipGetLastError();
 }
@@ -395,6 +399,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     A.data = L.data;
     A.coeff = L.coeff;
     A.coeff3 = L.coeff3;
+    A.rois = L.rois;
     A.P = L.P;
     A.CRLB = L.CRLB;
     A.LL = L.LL;

@@ -6,6 +6,15 @@
 
 namespace bsf {
 
+// Placement of one ROI in a launch that mixes ROI sizes (the serving
+// session): size S, channel-0 image at data + off (column-major S x S),
+// channel k at data + off + k * cstride.
+struct RoiDesc {
+    long long off;
+    int cstride;
+    int S;
+};
+
 struct FitLaunch {
     int fittype;            // 6, 7, 8
     int f64;                // 0: FP32 evaluation + FP64 solve, 1: all FP64
@@ -13,8 +22,9 @@ struct FitLaunch {
     const float *coeff;     // device, bsarray layout, dims cdims[0..ndim)
     const float *coeff3;    // device, fittype 8 only
     float *P, *CRLB, *LL;   // device outputs, column-major with stride N
-    int N, S, iterations;
+    int N, S, iterations;   // S: the ROI size, or the largest one when rois is set
     int roi_offset, count;  // ROIs [roi_offset, roi_offset + count)
+    const RoiDesc *rois;    // device, per-ROI placement, or NULL
     int cdims[5], ndim;
     float initP[4];
     int has_initP;
@@ -30,8 +40,9 @@ struct Fit4PiLaunch {
     const float *data;        // device, [S, S, N, nch]
     const float *coeff;       // device, 4D (x, y, z, phase) coefficients
     float *P, *CRLB, *LL;     // device outputs
-    int N, S, iterations, nch;
+    int N, S, iterations, nch;   // S: the ROI size, or the largest one when rois is set
     int roi_offset, count;
+    const RoiDesc *rois;      // device, per-ROI placement, or NULL
     int cdims[4];
     int shared[6];
     float phaseShift[8];      // w units
@@ -41,5 +52,9 @@ struct Fit4PiLaunch {
 
 hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st);
 size_t lds_bytes_4pi(int f64, int S, int nch, int nv);
+
+// Error text of the calling thread (bsf_last_error): set it and return code.
+int fail(int code, const char *fmt, ...);
+void clear_error();
 
 }  // namespace bsf

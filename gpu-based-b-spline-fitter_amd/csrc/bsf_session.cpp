@@ -1,0 +1,685 @@
+// bsf_session.cpp -- serving sessions (include/bsfit.h): device-resident
+// models, a frame arena with HBM-budget admission and a pinned host mirror,
+// and per-frame launches spread over kernel streams.
+//
+// Per frame:
+//   host     inputs of every launch packed into the frame's pinned slot
+//            (ROI descriptors, images, 4Pi starts / offsets);
+//   s_in     one H2D copy of the slot's input section, event `in`;
+//   ks[j]    the frame's launches (one per model / iterations / start, mixed
+//            ROI sizes, largest first), each waiting on `in`;
+//   s_out    waits for the launches, one D2H copy of the output section,
+//            event `done`;
+//   poll     scatters the outputs into the groups' host arrays.
+// The arena is a ring: frames take slots in submission order and give them
+// back in submission order once their results have been delivered, so the
+// device and pinned slots of a frame are never reused while it runs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <vector>
+
+#include "../../include/bsfit.h"
+#include "bsf_kernels.h"
+
+struct bsf_session;
+
+namespace {
+
+using bsf::fail;
+
+constexpr size_t kAlign = 256;
+constexpr int kMaxLaunchRois = 1 << 24;   // one grid of one-wave workgroups
+
+size_t up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+int nv_of(int ft) { return ft == 6 ? 5 : ft == 7 ? 6 : ft == 8 ? 8 : -1; }
+int dim_of(int ft) { return ft == 6 ? 3 : ft == 7 ? 4 : ft == 8 ? 5 : ft == BSF_FT_4PI ? 4 : -1; }
+
+struct Model {
+    bool live = false;
+    int ft = 0, nv = 0, ndim = 0;
+    int cdims[5] = {1, 1, 1, 1, 1};
+    float *d_coeff = nullptr, *d_coeff3 = nullptr;
+    size_t bytes = 0;
+    int nch = 1;
+    int shared[6] = {0, 0, 0, 0, 0, 0};
+    float shift[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// One kernel launch of a frame: the groups sharing model, iterations and start.
+struct Launch {
+    int model = 0, iterations = 0, n = 0, Smax = 0, nch = 1, nv = 0;
+    float initP[4] = {0, 0, 0, 0};
+    int has_initP = 0, has_dT = 0, has_iz = 0, has_ip = 0;
+    std::vector<int> groups;   // launch order (largest ROI size first)
+    std::vector<int> start;    // first launch row of each group
+    size_t o_desc = 0, o_data = 0, o_dT = 0, o_iz = 0, o_ip = 0;   // input section
+    size_t o_P = 0, o_CR = 0, o_LL = 0;                           // output section
+    double cost = 0;
+    int stream = 0;
+};
+
+struct GroupOut {
+    float *P, *CRLB, *LL;
+    int count, nv, start, launch_n;
+    size_t o_P, o_CR, o_LL;   // slot-relative byte offsets of the launch outputs
+};
+
+struct Frame {
+    unsigned long long ticket = 0;
+    size_t off = 0, bytes = 0;
+    std::vector<GroupOut> outs;
+    std::vector<hipEvent_t> events;   // back() is `done`
+    bool delivered = false;
+};
+
+struct DeviceGuard {
+    int old = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&old) != hipSuccess) old = -1;
+        if (old != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (old >= 0) (void)hipSetDevice(old);
+    }
+};
+
+}  // namespace
+
+struct bsf_session {
+    int device = 0, precision = BSF_PREC_MIX, nks = 2;
+    size_t budget = 0, arena = 0, model_bytes = 0, lds_limit = 0;
+    char *d_arena = nullptr, *h_arena = nullptr;
+    hipStream_t s_in = nullptr, s_out = nullptr, ks[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<Model> models;
+    std::deque<Frame> inflight;
+    std::vector<hipEvent_t> event_pool;
+    unsigned long long next_ticket = 1;
+    std::mutex mu;
+    // per-submit scratch, kept to avoid reallocation on the steady-state path
+    std::vector<Launch> launches;
+    std::vector<int> order;
+};
+
+namespace {
+
+hipEvent_t take_event(bsf_session *s)
+{
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Copies a finished frame's outputs from the pinned slot into the groups'
+// host arrays (column-major, stride = group count).
+void deliver(bsf_session *s, Frame &f)
+{
+    const char *base = s->h_arena + f.off;
+    for (const GroupOut &g : f.outs) {
+        const size_t cnt = (size_t)g.count, ln = (size_t)g.launch_n, st = (size_t)g.start;
+        const float *P = (const float *)(base + g.o_P), *CR = (const float *)(base + g.o_CR);
+        const float *LL = (const float *)(base + g.o_LL);
+        for (int c = 0; c <= g.nv; c++) std::memcpy(g.P + c * cnt, P + c * ln + st, cnt * 4);
+        for (int c = 0; c < g.nv; c++) std::memcpy(g.CRLB + c * cnt, CR + c * ln + st, cnt * 4);
+        std::memcpy(g.LL, LL + st, cnt * 4);
+    }
+    for (hipEvent_t e : f.events) s->event_pool.push_back(e);
+    f.events.clear();
+    f.delivered = true;
+}
+
+void retire(bsf_session *s)
+{
+    while (!s->inflight.empty() && s->inflight.front().delivered) s->inflight.pop_front();
+}
+
+// Waits for a frame, delivers it and returns its slots if it is the oldest.
+int finish(bsf_session *s, Frame &f)
+{
+    if (f.delivered) return BSF_OK;
+    hipError_t e = hipEventSynchronize(f.events.back());
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: frame %llu failed: %s", f.ticket, hipGetErrorString(e));
+    deliver(s, f);
+    retire(s);
+    return BSF_OK;
+}
+
+// Ring allocation of `bytes` behind the newest in-flight frame; SIZE_MAX if
+// there is no room now.
+size_t ring_alloc(const bsf_session *s, size_t bytes)
+{
+    if (s->inflight.empty()) return bytes <= s->arena ? 0 : SIZE_MAX;
+    const Frame &a = s->inflight.front(), &b = s->inflight.back();
+    const size_t head = b.off + b.bytes, tail = a.off;
+    if (b.off >= a.off) {   // not wrapped: free [head, arena) and [0, tail)
+        if (s->arena - head >= bytes) return head;
+        if (tail >= bytes) return 0;
+        return SIZE_MAX;
+    }
+    return (tail - head >= bytes) ? head : SIZE_MAX;   // wrapped: free [head, tail)
+}
+
+// Waits for every frame in flight and delivers it (destroy, model release).
+int drain(bsf_session *s)
+{
+    int rc = BSF_OK;
+    while (!s->inflight.empty()) {
+        Frame &f = s->inflight.front();
+        if (f.delivered) {
+            s->inflight.pop_front();
+            continue;
+        }
+        int r = finish(s, f);   // delivers and pops on success
+        if (r) {
+            if (!rc) rc = r;
+            for (hipEvent_t e : f.events) s->event_pool.push_back(e);
+            s->inflight.pop_front();
+        }
+    }
+    return rc;
+}
+
+int validate_model(const bsf_model_desc *m)
+{
+    if (!m) return fail(BSF_ERR_ARG, "bsf: NULL model description");
+    const int D = dim_of(m->fittype);
+    if (D < 0) return fail(BSF_ERR_FITTYPE, "bsf: model fittype %d is not 6, 7, 8 or BSF_FT_4PI", m->fittype);
+    if (!m->coeff) return fail(BSF_ERR_ARG, "bsf: coeff must not be NULL");
+    if (m->coeff_ndim != D)
+        return fail(BSF_ERR_SHAPE, "bsf: fittype %d needs a %d-D coefficient array (got %d-D)", m->fittype, D,
+                    m->coeff_ndim);
+    for (int a = 0; a < D; a++)
+        if (m->coeff_dims[a] < 5)
+            return fail(BSF_ERR_SHAPE, "bsf: coefficient dim %d is %d; each axis needs >= 2 data samples (+3)", a,
+                        m->coeff_dims[a]);
+    if (m->fittype == 8 && !m->coeff3) return fail(BSF_ERR_ARG, "bsf: fittype 8 needs coeff3");
+    if (m->fittype == BSF_FT_4PI) {
+        if (m->nch < 1 || m->nch > 8) return fail(BSF_ERR_ARG, "bsf: 4Pi models need 1..8 channels (got %d)", m->nch);
+        int nv = 0;
+        for (int p = 0; p < 6; p++) nv += m->shared[p] ? 1 : m->nch;
+        if (nv > 24) return fail(BSF_ERR_ARG, "bsf: 4Pi parameter count %d exceeds 24", nv);
+    }
+    return BSF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsf_session_create(const bsf_session_config *cfg, bsf_session **out)
+{
+    bsf::clear_error();
+    if (!cfg || !out) return fail(BSF_ERR_ARG, "bsf: NULL session config or output");
+    *out = nullptr;
+    if (cfg->precision != BSF_PREC_MIX && cfg->precision != BSF_PREC_F64)
+        return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", cfg->precision);
+    const int nks = cfg->kernel_streams ? cfg->kernel_streams : 2;
+    if (nks < 1 || nks > 4) return fail(BSF_ERR_ARG, "bsf: kernel_streams must be 1..4 (got %d)", nks);
+    const size_t budget = cfg->hbm_budget ? cfg->hbm_budget : ((size_t)2 << 30);
+    const size_t arena = up(cfg->arena_bytes ? cfg->arena_bytes : ((size_t)256 << 20));
+    if (arena > budget)
+        return fail(BSF_ERR_NOMEM, "bsf: arena of %zu B exceeds the HBM budget of %zu B", arena, budget);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || cfg->device < 0 || cfg->device >= ndev)
+        return fail(BSF_ERR_DEVICE, "bsf: no HIP device %d (%d visible)", cfg->device, e == hipSuccess ? ndev : 0);
+    hipDeviceProp_t p;
+    if ((e = hipGetDeviceProperties(&p, cfg->device)) != hipSuccess)
+        return fail(BSF_ERR_DEVICE, "bsf: hipGetDeviceProperties: %s", hipGetErrorString(e));
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(BSF_ERR_DEVICE, "bsf: device %d is %s; this build targets gfx950 (MI355X)", cfg->device,
+                    p.gcnArchName);
+    DeviceGuard g(cfg->device);
+    bsf_session *s = new bsf_session;
+    s->device = cfg->device;
+    s->precision = cfg->precision;
+    s->nks = nks;
+    s->budget = budget;
+    s->arena = arena;
+    s->lds_limit = p.sharedMemPerBlock;
+    e = hipMalloc((void **)&s->d_arena, arena);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_arena, arena, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_in, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_out, hipStreamNonBlocking);
+    for (int j = 0; j < nks && e == hipSuccess; j++) e = hipStreamCreateWithFlags(&s->ks[j], hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        bsf_session_destroy(s);
+        return fail(e == hipErrorOutOfMemory ? BSF_ERR_NOMEM : BSF_ERR_DEVICE, "bsf: session set-up failed: %s",
+                    hipGetErrorString(e));
+    }
+    *out = s;
+    return BSF_OK;
+}
+
+int bsf_session_destroy(bsf_session *s)
+{
+    bsf::clear_error();
+    if (!s) return BSF_OK;
+    int rc = BSF_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        DeviceGuard g(s->device);
+        rc = drain(s);
+        for (Model &m : s->models) {
+            if (m.d_coeff) (void)hipFree(m.d_coeff);
+            if (m.d_coeff3) (void)hipFree(m.d_coeff3);
+        }
+        for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+        if (s->s_in) (void)hipStreamDestroy(s->s_in);
+        if (s->s_out) (void)hipStreamDestroy(s->s_out);
+        for (int j = 0; j < 4; j++)
+            if (s->ks[j]) (void)hipStreamDestroy(s->ks[j]);
+        if (s->d_arena) (void)hipFree(s->d_arena);
+        if (s->h_arena) (void)hipHostFree(s->h_arena);
+    }
+    delete s;
+    return rc;
+}
+
+int bsf_model_register(bsf_session *s, const bsf_model_desc *m, int *model_id)
+{
+    bsf::clear_error();
+    if (!s || !model_id) return fail(BSF_ERR_ARG, "bsf: NULL session or model id");
+    int rc = validate_model(m);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    DeviceGuard g(s->device);
+    Model M;
+    M.ft = m->fittype;
+    M.ndim = m->coeff_ndim;
+    size_t nc = 1;
+    for (int a = 0; a < M.ndim; a++) {
+        M.cdims[a] = m->coeff_dims[a];
+        nc *= (size_t)m->coeff_dims[a];
+    }
+    const size_t nc3 = M.ft == 8 ? (size_t)M.cdims[0] * M.cdims[1] * M.cdims[2] : 0;
+    if (nc * 4 >= ((size_t)1 << 31) || nc3 * 4 >= ((size_t)1 << 31))
+        return fail(BSF_ERR_SHAPE, "bsf: coefficient arrays must be below 2 GiB");
+    M.bytes = up(nc * 4) + up(nc3 * 4);
+    if (s->model_bytes + M.bytes + s->arena > s->budget)
+        return fail(BSF_ERR_NOMEM, "bsf: model of %zu B exceeds the HBM budget (%zu B used of %zu B)", M.bytes,
+                    s->model_bytes + s->arena, s->budget);
+    if (M.ft == BSF_FT_4PI) {
+        M.nch = m->nch;
+        M.nv = 0;
+        for (int p = 0; p < 6; p++) {
+            M.shared[p] = m->shared[p] ? 1 : 0;
+            M.nv += M.shared[p] ? 1 : M.nch;
+        }
+        for (int k = 0; k < M.nch; k++) M.shift[k] = m->phaseShift[k];
+    } else {
+        M.nv = nv_of(M.ft);
+    }
+    hipError_t e = hipMalloc((void **)&M.d_coeff, nc * 4);
+    if (e == hipSuccess && nc3) e = hipMalloc((void **)&M.d_coeff3, nc3 * 4);
+    if (e == hipSuccess) e = hipMemcpy(M.d_coeff, m->coeff, nc * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nc3) e = hipMemcpy(M.d_coeff3, m->coeff3, nc3 * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (M.d_coeff) (void)hipFree(M.d_coeff);
+        if (M.d_coeff3) (void)hipFree(M.d_coeff3);
+        return fail(e == hipErrorOutOfMemory ? BSF_ERR_NOMEM : BSF_ERR_DEVICE, "bsf: model upload failed: %s",
+                    hipGetErrorString(e));
+    }
+    M.live = true;
+    s->model_bytes += M.bytes;
+    s->models.push_back(M);
+    *model_id = (int)s->models.size() - 1;
+    return BSF_OK;
+}
+
+int bsf_model_release(bsf_session *s, int model_id)
+{
+    bsf::clear_error();
+    if (!s) return fail(BSF_ERR_ARG, "bsf: NULL session");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (model_id < 0 || model_id >= (int)s->models.size() || !s->models[model_id].live)
+        return fail(BSF_ERR_ARG, "bsf: unknown model id %d", model_id);
+    DeviceGuard g(s->device);
+    int rc = drain(s);
+    Model &M = s->models[model_id];
+    (void)hipFree(M.d_coeff);
+    if (M.d_coeff3) (void)hipFree(M.d_coeff3);
+    M.d_coeff = M.d_coeff3 = nullptr;
+    M.live = false;
+    s->model_bytes -= M.bytes;
+    return rc;
+}
+
+int bsf_session_set_precision(bsf_session *s, int precision)
+{
+    bsf::clear_error();
+    if (!s) return fail(BSF_ERR_ARG, "bsf: NULL session");
+    if (precision != BSF_PREC_MIX && precision != BSF_PREC_F64)
+        return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", precision);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->precision = precision;
+    return BSF_OK;
+}
+
+int bsf_session_inflight(bsf_session *s, int *frames, size_t *arena_used)
+{
+    bsf::clear_error();
+    if (!s) return fail(BSF_ERR_ARG, "bsf: NULL session");
+    std::lock_guard<std::mutex> lk(s->mu);
+    size_t used = 0;
+    for (const Frame &f : s->inflight) used += f.bytes;
+    if (frames) *frames = (int)s->inflight.size();
+    if (arena_used) *arena_used = used;
+    return BSF_OK;
+}
+
+int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long *ticket)
+{
+    bsf::clear_error();
+    if (!s || !f || !ticket) return fail(BSF_ERR_ARG, "bsf: NULL session, frame or ticket");
+    if (f->ngroups < 0 || (f->ngroups > 0 && !f->groups)) return fail(BSF_ERR_ARG, "bsf: bad group list");
+    if (flags & ~BSF_SUBMIT_NONBLOCK) return fail(BSF_ERR_ARG, "bsf: unknown submit flags 0x%x", flags);
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int f64 = s->precision == BSF_PREC_F64;
+
+    // ---- checks (the MEX's, per group) and grouping into launches
+    std::vector<Launch> &L = s->launches;
+    L.clear();
+    for (int gi = 0; gi < f->ngroups; gi++) {
+        const bsf_roi_group &g = f->groups[gi];
+        if (g.model < 0 || g.model >= (int)s->models.size() || !s->models[g.model].live)
+            return fail(BSF_ERR_ARG, "bsf: group %d: unknown model id %d", gi, g.model);
+        const Model &M = s->models[g.model];
+        if (g.sz < 1 || g.sz > BSF_MAX_ROI)
+            return fail(BSF_ERR_ROI_SIZE, "bsf: group %d: ROI size must be between 1 and %d (got %d)", gi,
+                        BSF_MAX_ROI, g.sz);
+        if (g.count < 0 || g.iterations < 0) return fail(BSF_ERR_ARG, "bsf: group %d: negative count or iterations", gi);
+        if (g.count == 0) continue;
+        if (!g.data || !g.P || !g.CRLB || !g.LL)
+            return fail(BSF_ERR_ARG, "bsf: group %d: NULL data or output pointer", gi);
+        if (g.count > kMaxLaunchRois) return fail(BSF_ERR_ARG, "bsf: group %d: more than %d ROIs", gi, kMaxLaunchRois);
+        if ((size_t)g.count * g.sz * g.sz >= ((size_t)1 << 31))
+            return fail(BSF_ERR_ARG, "bsf: group %d: %d ROIs of size %d exceed 2^31 pixels", gi, g.count, g.sz);
+        const int need = M.ft == 7 ? 2 : M.ft == 8 ? 4 : 0;
+        const bool use_initP = g.initP && need > 0;
+        if (use_initP && g.n_initP < need)
+            return fail(BSF_ERR_ARG, "bsf: group %d: initP for fittype %d needs %d values (got %d)", gi, M.ft, need,
+                        g.n_initP);
+        if (M.ft == BSF_FT_4PI && g.chan_stride && g.chan_stride < (size_t)g.count * g.sz * g.sz)
+            return fail(BSF_ERR_ARG, "bsf: group %d: channel stride %zu is below count*sz*sz", gi, g.chan_stride);
+        Launch key;
+        key.model = g.model;
+        key.iterations = g.iterations;
+        key.has_initP = use_initP ? 1 : 0;
+        for (int i = 0; i < need && use_initP; i++) key.initP[i] = g.initP[i];
+        key.has_dT = M.ft == BSF_FT_4PI && g.dT;
+        key.has_iz = M.ft == BSF_FT_4PI && g.initZ;
+        key.has_ip = M.ft == BSF_FT_4PI && g.initPhi;
+        Launch *hit = nullptr;
+        for (Launch &l : L)
+            if (l.model == key.model && l.iterations == key.iterations && l.has_initP == key.has_initP &&
+                std::memcmp(l.initP, key.initP, sizeof key.initP) == 0 && l.has_dT == key.has_dT &&
+                l.has_iz == key.has_iz && l.has_ip == key.has_ip) {
+                hit = &l;
+                break;
+            }
+        if (!hit) {
+            key.nch = M.ft == BSF_FT_4PI ? M.nch : 1;
+            key.nv = M.nv;
+            L.push_back(key);
+            hit = &L.back();
+        }
+        if ((long long)hit->n + g.count > kMaxLaunchRois)
+            return fail(BSF_ERR_ARG, "bsf: more than %d ROIs of one model in a frame", kMaxLaunchRois);
+        hit->groups.push_back(gi);
+        hit->n += g.count;
+        hit->Smax = std::max(hit->Smax, g.sz);
+    }
+
+    // ---- layout of the slot: inputs, then outputs
+    size_t o = 0;
+    for (Launch &l : L) {
+        const Model &M = s->models[l.model];
+        const size_t lds = M.ft == BSF_FT_4PI ? bsf::lds_bytes_4pi(f64, l.Smax, M.nch, M.nv)
+                                              : bsf::lds_bytes(M.ft, f64, l.Smax);
+        if (lds > s->lds_limit)
+            return fail(BSF_ERR_ROI_SIZE, "bsf: ROI size %d of model %d needs %zu B of LDS (limit %zu B)", l.Smax,
+                        l.model, lds, s->lds_limit);
+        std::stable_sort(l.groups.begin(), l.groups.end(),
+                         [&](int a, int b) { return f->groups[a].sz > f->groups[b].sz; });
+        size_t nd = 0;
+        double cost = 0;
+        l.start.clear();
+        int row = 0;
+        for (int gi : l.groups) {
+            const bsf_roi_group &g = f->groups[gi];
+            l.start.push_back(row);
+            row += g.count;
+            nd += (size_t)g.count * g.sz * g.sz * l.nch;
+            cost += (double)g.count * g.sz * g.sz * l.nch * (double)(1 << (2 * (dim_of(M.ft) - 2)));
+        }
+        l.cost = cost * (l.iterations + 2);
+        l.o_desc = o;
+        o += up((size_t)l.n * sizeof(bsf::RoiDesc));
+        l.o_data = o;
+        o += up(nd * 4);
+        l.o_dT = o;
+        o += l.has_dT ? up((size_t)l.n * 6 * l.nch * 4) : 0;
+        l.o_iz = o;
+        o += l.has_iz ? up((size_t)l.n * 4) : 0;
+        l.o_ip = o;
+        o += l.has_ip ? up((size_t)l.n * 4) : 0;
+    }
+    const size_t in_bytes = o;
+    for (Launch &l : L) {
+        l.o_P = o;
+        o += up((size_t)l.n * (l.nv + 1) * 4);
+        l.o_CR = o;
+        o += up((size_t)l.n * l.nv * 4);
+        l.o_LL = o;
+        o += up((size_t)l.n * 4);
+    }
+    const size_t total = o;
+    if (total > s->arena)
+        return fail(BSF_ERR_NOMEM, "bsf: frame needs %zu B, more than the whole arena (%zu B)", total, s->arena);
+    if (L.empty()) {   // nothing to fit: complete at once
+        *ticket = s->next_ticket++;
+        return BSF_OK;
+    }
+
+    // ---- admission: room in the arena behind the frames in flight
+    DeviceGuard dg(s->device);
+    size_t off;
+    while ((off = ring_alloc(s, total)) == SIZE_MAX) {
+        if (flags & BSF_SUBMIT_NONBLOCK)
+            return fail(BSF_BUSY, "bsf: arena full (%d frames in flight)", (int)s->inflight.size());
+        Frame &oldest = s->inflight.front();
+        if (oldest.delivered) {
+            retire(s);
+            continue;
+        }
+        int rc = finish(s, oldest);
+        if (rc) return rc;
+    }
+
+    // ---- pack the inputs into the pinned slot
+    char *h = s->h_arena + off, *d = s->d_arena + off;
+    for (Launch &l : L) {
+        bsf::RoiDesc *desc = (bsf::RoiDesc *)(h + l.o_desc);
+        float *dst = (float *)(h + l.o_data);
+        long long e0 = 0;
+        int row = 0;
+        for (size_t j = 0; j < l.groups.size(); j++) {
+            const bsf_roi_group &g = f->groups[l.groups[j]];
+            const size_t S2 = (size_t)g.sz * g.sz, cnt = (size_t)g.count;
+            const size_t cs = g.chan_stride ? g.chan_stride : cnt * S2;
+            for (int k = 0; k < l.nch; k++) std::memcpy(dst + e0 + k * cnt * S2, g.data + k * cs, cnt * S2 * 4);
+            for (size_t i = 0; i < cnt; i++) {
+                desc[row + i].off = e0 + (long long)(i * S2);
+                desc[row + i].cstride = (int)(cnt * S2);
+                desc[row + i].S = g.sz;
+            }
+            const int nch = l.nch;
+            if (l.has_dT) std::memcpy((float *)(h + l.o_dT) + (size_t)row * 6 * nch, g.dT, cnt * 6 * nch * 4);
+            if (l.has_iz) std::memcpy((float *)(h + l.o_iz) + row, g.initZ, cnt * 4);
+            if (l.has_ip) std::memcpy((float *)(h + l.o_ip) + row, g.initPhi, cnt * 4);
+            e0 += (long long)(cnt * S2 * l.nch);
+            row += g.count;
+        }
+    }
+
+    // ---- copies and launches
+    Frame F;
+    F.off = off;
+    F.bytes = total;
+    hipEvent_t ev_in = take_event(s);
+    if (!ev_in) return fail(BSF_ERR_DEVICE, "bsf: hipEventCreate failed");
+    F.events.push_back(ev_in);
+    hipError_t e = hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s->s_in);
+    if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
+    // largest launches first, each to the least-loaded kernel stream
+    std::vector<int> &ord = s->order;
+    ord.resize(L.size());
+    for (size_t i = 0; i < L.size(); i++) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return L[a].cost > L[b].cost; });
+    double load[4] = {0, 0, 0, 0};
+    for (int i : ord) {
+        if (e != hipSuccess) break;
+        Launch &l = L[i];
+        int j = 0;
+        for (int q = 1; q < s->nks; q++)
+            if (load[q] < load[j]) j = q;
+        load[j] += l.cost;
+        l.stream = j;
+        hipStream_t ks = s->ks[j];
+        const Model &M = s->models[l.model];
+        e = hipStreamWaitEvent(ks, ev_in, 0);
+        if (e != hipSuccess) break;
+        const bsf::RoiDesc *d_desc = (const bsf::RoiDesc *)(d + l.o_desc);
+        const float *d_data = (const float *)(d + l.o_data);
+        float *dP = (float *)(d + l.o_P), *dCR = (float *)(d + l.o_CR), *dLL = (float *)(d + l.o_LL);
+        if (M.ft == BSF_FT_4PI) {
+            bsf::Fit4PiLaunch K;
+            std::memset(&K, 0, sizeof K);
+            K.f64 = f64;
+            K.data = d_data;
+            K.coeff = M.d_coeff;
+            K.P = dP;
+            K.CRLB = dCR;
+            K.LL = dLL;
+            K.N = l.n;
+            K.S = l.Smax;
+            K.iterations = l.iterations;
+            K.nch = M.nch;
+            K.roi_offset = 0;
+            K.count = l.n;
+            K.rois = d_desc;
+            for (int a = 0; a < 4; a++) K.cdims[a] = M.cdims[a];
+            for (int p = 0; p < 6; p++) K.shared[p] = M.shared[p];
+            for (int k = 0; k < 8; k++) K.phaseShift[k] = M.shift[k];
+            K.dT = l.has_dT ? (const float *)(d + l.o_dT) : nullptr;
+            K.initZ = l.has_iz ? (const float *)(d + l.o_iz) : nullptr;
+            K.initPhi = l.has_ip ? (const float *)(d + l.o_ip) : nullptr;
+            e = bsf::launch_fit_4pi(K, ks);
+        } else {
+            bsf::FitLaunch K;
+            std::memset(&K, 0, sizeof K);
+            K.fittype = M.ft;
+            K.f64 = f64;
+            K.data = d_data;
+            K.coeff = M.d_coeff;
+            K.coeff3 = M.d_coeff3;
+            K.P = dP;
+            K.CRLB = dCR;
+            K.LL = dLL;
+            K.N = l.n;
+            K.S = l.Smax;
+            K.iterations = l.iterations;
+            K.roi_offset = 0;
+            K.count = l.n;
+            K.rois = d_desc;
+            K.ndim = M.ndim;
+            for (int a = 0; a < 5; a++) K.cdims[a] = M.cdims[a];
+            K.has_initP = l.has_initP;
+            for (int q = 0; q < 4; q++) K.initP[q] = l.initP[q];
+            e = bsf::launch_fit(K, ks);
+        }
+        if (e != hipSuccess) break;
+        hipEvent_t ev = take_event(s);
+        if (!ev) {
+            e = hipErrorOutOfMemory;
+            break;
+        }
+        F.events.push_back(ev);
+        e = hipEventRecord(ev, ks);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s->s_out, ev, 0);
+    }
+    hipEvent_t ev_done = nullptr;
+    if (e == hipSuccess) {
+        ev_done = take_event(s);
+        if (!ev_done) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess) {
+        F.events.push_back(ev_done);
+        e = hipMemcpyAsync(h + in_bytes, d + in_bytes, total - in_bytes, hipMemcpyDeviceToHost, s->s_out);
+        if (e == hipSuccess) e = hipEventRecord(ev_done, s->s_out);
+    }
+    if (e != hipSuccess) {
+        // work already queued may still read the slot: wait for it before
+        // the slot can be handed out again
+        (void)hipStreamSynchronize(s->s_in);
+        for (int j = 0; j < s->nks; j++) (void)hipStreamSynchronize(s->ks[j]);
+        (void)hipStreamSynchronize(s->s_out);
+        for (hipEvent_t ev : F.events) s->event_pool.push_back(ev);
+        return fail(BSF_ERR_DEVICE, "bsf: frame submission failed: %s", hipGetErrorString(e));
+    }
+    for (Launch &l : L)
+        for (size_t j = 0; j < l.groups.size(); j++) {
+            const bsf_roi_group &g = f->groups[l.groups[j]];
+            GroupOut go;
+            go.P = g.P;
+            go.CRLB = g.CRLB;
+            go.LL = g.LL;
+            go.count = g.count;
+            go.nv = l.nv;
+            go.start = l.start[j];
+            go.launch_n = l.n;
+            go.o_P = l.o_P;
+            go.o_CR = l.o_CR;
+            go.o_LL = l.o_LL;
+            F.outs.push_back(go);
+        }
+    F.ticket = s->next_ticket++;
+    *ticket = F.ticket;
+    s->inflight.push_back(std::move(F));
+    return BSF_OK;
+}
+
+int bsf_poll(bsf_session *s, unsigned long long ticket, int wait)
+{
+    bsf::clear_error();
+    if (!s) return fail(BSF_ERR_ARG, "bsf: NULL session");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (ticket == 0 || ticket >= s->next_ticket) return fail(BSF_ERR_ARG, "bsf: unknown ticket %llu", ticket);
+    for (Frame &f : s->inflight) {
+        if (f.ticket != ticket) continue;
+        if (f.delivered) return BSF_OK;
+        DeviceGuard g(s->device);
+        if (!wait) {
+            hipError_t e = hipEventQuery(f.events.back());
+            if (e == hipErrorNotReady) return BSF_PENDING;
+            if (e != hipSuccess)
+                return fail(BSF_ERR_DEVICE, "bsf: frame %llu failed: %s", ticket, hipGetErrorString(e));
+        }
+        return finish(s, f);
+    }
+    return BSF_OK;   // delivered and retired, or a frame with no ROIs
+}
+
+}  // extern "C"

@@ -12,7 +12,7 @@
 #include "../../include/bsfit.h"
 #include "bsf_kernels.h"
 
-namespace {
+namespace bsf {
 
 thread_local std::string g_err;
 
@@ -26,6 +26,15 @@ int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
+
+void clear_error() { g_err.clear(); }
+
+}  // namespace bsf
+
+namespace {
+
+using bsf::fail;
+using bsf::g_err;
 
 int nv_of(int ft) { return ft == 6 ? 5 : ft == 7 ? 6 : ft == 8 ? 8 : -1; }
 int dim_of(int ft) { return ft == 6 ? 3 : ft == 7 ? 4 : ft == 8 ? 5 : -1; }

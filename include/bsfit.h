@@ -148,6 +148,108 @@ int bsf_fit_4pi_device(const bsf_request_4pi *req, bsf_result *res, void *stream
  * Replaces mexErrMsgIdAndTxt (GPUmleFit_LM/mexFunction.cpp:140-176). */
 const char *bsf_last_error(void);
 
+/* -------------------------------------------------------------- sessions
+ * A session keeps models (coefficient arrays) resident on one device and
+ * fits frames asynchronously.  A frame is a list of ROI groups; a group is
+ * one MEX call's worth of ROIs (one model, one ROI size, one initP), as the
+ * reference's callers batch them (GPUmleFit_LM/mexFunction.cpp:259-263: one
+ * size per call).  The session packs all groups of a frame that share a
+ * model, iteration count and start into one kernel launch (mixed ROI sizes),
+ * runs the launches of a frame concurrently, and copies the results back.
+ *
+ * Device memory: the models plus a frame arena of fixed size, allocated at
+ * creation; both count against hbm_budget.  A frame is admitted when its
+ * inputs and outputs fit in the arena next to the frames still in flight
+ * (HBM-budget admission); a blocking submit waits for the oldest frames,
+ * a non-blocking one returns BSF_BUSY.  Submits and polls allocate nothing.
+ * All calls on one session are serialised by the session (thread-safe).
+ *
+ * No counterpart in the reference, which allocates, copies and frees per
+ * call (mexFunction.cpp:466-586, 741-748); this is the serving path for
+ * callers that fit a stream of frames. */
+#define BSF_FT_4PI 104      /* model fittype of the 4Pi multi-channel fitter */
+
+enum {
+    BSF_BUSY = 7,           /* non-blocking submit: the arena has no room now */
+    BSF_PENDING = 8         /* poll: the frame has not finished */
+};
+
+typedef struct bsf_session bsf_session;
+
+typedef struct bsf_session_config {
+    int device;             /* HIP device ordinal */
+    size_t hbm_budget;      /* device bytes for models + arena (0: 2 GiB) */
+    size_t arena_bytes;     /* frame arena, part of the budget (0: 256 MiB) */
+    int kernel_streams;     /* streams that run fit launches, 1..4 (0: 2) */
+    int precision;          /* enum bsf_precision for submitted frames */
+} bsf_session_config;
+
+typedef struct bsf_model_desc {
+    int fittype;                /* 6, 7, 8 or BSF_FT_4PI */
+    const float *coeff;         /* host bsarray coefficients (as bsf_request) */
+    int coeff_ndim;             /* 3, 4, 5 (4 for BSF_FT_4PI) */
+    int coeff_dims[5];
+    const float *coeff3;        /* fittype 8: 3D coefficients */
+    int nch;                    /* BSF_FT_4PI: channels (1..8) */
+    unsigned int shared[6];     /* BSF_FT_4PI: shared-parameter mask */
+    float phaseShift[8];        /* BSF_FT_4PI: channel phase offsets (w units) */
+} bsf_model_desc;
+
+typedef struct bsf_roi_group {
+    int model;                  /* id from bsf_model_register */
+    int sz;                     /* ROI size, 1..51 */
+    int count;                  /* ROIs in the group (0 allowed) */
+    int iterations;
+    const float *data;          /* host [sz, sz, count] (4Pi: per channel) */
+    size_t chan_stride;         /* 4Pi: floats between channels, 0: count*sz*sz */
+    const float *initP;         /* fittype 7: 2, 8: 4 values, or NULL */
+    int n_initP;
+    const float *dT;            /* 4Pi: host [6, nch, count] or NULL */
+    const float *initZ;         /* 4Pi: host [count] or NULL */
+    const float *initPhi;       /* 4Pi: host [count] or NULL */
+    float *P, *CRLB, *LL;       /* host outputs as bsf_result (stride count);
+                                   written by bsf_poll / a blocking submit,
+                                   must stay valid until the frame completes */
+} bsf_roi_group;
+
+typedef struct bsf_frame {
+    const bsf_roi_group *groups;
+    int ngroups;
+} bsf_frame;
+
+#define BSF_SUBMIT_NONBLOCK 1
+
+/* Creates a session on cfg->device: streams, events, the frame arena and
+ * its pinned host mirror. */
+int bsf_session_create(const bsf_session_config *cfg, bsf_session **out);
+
+/* Waits for the frames in flight, delivers their results, frees everything. */
+int bsf_session_destroy(bsf_session *s);
+
+/* Copies a model's coefficients to the device; *model_id names it in
+ * groups.  Fails with BSF_ERR_NOMEM when models + arena exceed hbm_budget. */
+int bsf_model_register(bsf_session *s, const bsf_model_desc *m, int *model_id);
+
+/* Frees a model's device memory (waits for frames in flight first). */
+int bsf_model_release(bsf_session *s, int model_id);
+
+/* Precision of frames submitted from now on (enum bsf_precision). */
+int bsf_session_set_precision(bsf_session *s, int precision);
+
+/* Queues a frame.  Host inputs are copied before return (the caller may
+ * reuse them); outputs are delivered by bsf_poll.  flags: 0 = wait for
+ * arena room, BSF_SUBMIT_NONBLOCK = return BSF_BUSY instead.  A frame larger
+ * than the whole arena fails with BSF_ERR_NOMEM. */
+int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long *ticket);
+
+/* BSF_OK once the frame's results are in its groups' host outputs,
+ * BSF_PENDING while it runs (wait = 0), BSF_ERR_ARG for an unknown ticket.
+ * wait != 0 blocks until the frame completes. */
+int bsf_poll(bsf_session *s, unsigned long long ticket, int wait);
+
+/* Occupancy of the session: frames in flight and arena bytes they hold. */
+int bsf_session_inflight(bsf_session *s, int *frames, size_t *arena_used);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,3 +47,19 @@ def test_errors_without_device_or_bad_args():
         with pytest.raises(bsfit.BsfError) as e:
             bsfit.fit(6, w.imstack, 50, w.coeff)
         assert e.value.code == 6
+
+
+def test_session_errors_without_device_or_bad_args():
+    import torch
+    import bsfit
+    # configuration errors are reported before any device work
+    with pytest.raises(bsfit.BsfError) as e:
+        bsfit.Session(kernel_streams=9)
+    assert e.value.code == 1
+    with pytest.raises(bsfit.BsfError) as e:
+        bsfit.Session(hbm_budget=1 << 20, arena_bytes=2 << 20)
+    assert e.value.code == 5
+    if not torch.cuda.is_available():
+        with pytest.raises(bsfit.BsfError) as e:
+            bsfit.Session()
+        assert e.value.code == 6
