@@ -107,7 +107,7 @@ struct Dev {
 // (row, column) table, per-row / per-column tap offsets, axis factors, then
 // the planes (reused as reduction scratch) and kernel-specific extras.
 struct LdsLayout {
-    size_t pix, rtap, ctap, fac, pl, total;
+    size_t rdat, rtap, ctap, fac, pl, total;
 };
 
 __host__ __device__ inline size_t lds_align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -116,12 +116,12 @@ __host__ __device__ inline LdsLayout lds_layout(int S, int nch, size_t esz, size
 {
     LdsLayout L;
     size_t o = lds_align16((size_t)nch * S * S * 4);
-    L.pix = o;
-    o = lds_align16(o + (size_t)S * S * 4);
+    L.rdat = o;                                  // 1/data per pixel (FP32 evaluation only)
+    o = lds_align16(o + (esz == 4 ? (size_t)nch * S * S * 4 : 0));
     L.rtap = o;
-    o += 64 * 4;
+    o = lds_align16(o + (size_t)S * 4);
     L.ctap = o;
-    o += 64 * 4;
+    o = lds_align16(o + (size_t)S * 4);
     L.fac = o;
     o = lds_align16(o + 40 * esz);
     L.pl = o;
@@ -241,8 +241,12 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
     if constexpr (K >= 2) tapb += (g.knx[1] - 2) * s3;
     if constexpr (K >= 3) tapb += (g.knx[2] - 2) * s4;
     tapb = __builtin_amdgcn_readfirstlane(tapb);
+    // window point q = b*PR + a without an integer divide: for q < 2^12 the
+    // FP32 quotient (q + 0.5) / PR stays within 1e-4 of its exact value,
+    // which is at least 0.5/PR away from an integer, so the floor is exact
+    const float invPR = 1.0f / (float)PR;
     for (int q = lane; q < npts; q += 64) {
-        const int a = q % PR, b = q / PR;
+        const int b = (int)(((float)q + 0.5f) * invPR), a = q - b * PR;
         const int vo = ((g.row0 + a) + (g.col0 + b) * (int)A.s1) * 4;
         if constexpr (K == 1) {
             const Axis<E> &z = g.ax[2];
@@ -313,13 +317,16 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
                 }
             E v = E(0), dz = E(0), dw = E(0), du = E(0), zb = E(0), ub = E(0);
             E prev[2][2];
+            // the u tap goes into the per-lane offset (4 VGPRs), the (z, w)
+            // taps into the wave-uniform one (16 SGPRs instead of 64)
+            const int vu[4] = {vo, vo + s4, vo + 2 * s4, vo + 3 * s4};
 #pragma unroll
             for (int tz = 0; tz < 4; tz++) {
                 E cs[4][4];
 #pragma unroll
                 for (int tw = 0; tw < 4; tw++)
 #pragma unroll
-                    for (int tu = 0; tu < 4; tu++) cs[tw][tu] = (E)bload(rc, vo, tapb + tz * s2 + tw * s3 + tu * s4);
+                    for (int tu = 0; tu < 4; tu++) cs[tw][tu] = (E)bload(rc, vu[tu], tapb + tz * s2 + tw * s3);
                 E a00 = E(0), a01 = E(0), a10 = E(0);
 #pragma unroll
                 for (int tw = 0; tw < 4; tw++) {
@@ -472,6 +479,21 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
     }
 }
 
+// Poisson deviance term of one pixel, 2[(mu - d) - d ln(mu/d)] for d > 0 and
+// 2 mu otherwise (d <= 0 counts as 0, as the reference), without a branch.
+// FP32: mu * (1/d) with 1/d precomputed per pixel and the hardware log2 (the
+// ratio is never denormal for a fit's models); FP64: the exact expression.
+template <typename E>
+__device__ __forceinline__ E deviance_term(E model, E data, float rdat, E &d0)
+{
+    const bool pos = data > E(0);
+    d0 = pos ? data : E(0);
+    E lq;
+    if constexpr (sizeof(E) == 8) lq = log(pos ? model / data : E(1));
+    else lq = __builtin_amdgcn_logf(pos ? model * rdat : 1.0f) * 0.69314718055994531f;
+    return E(2) * ((model - d0) - d0 * lq);
+}
+
 // ------------------------------------------------------------- LM solve
 // The NV x NV systems are spread over the wave: lane i holds row i, pivots
 // and columns travel by v_readlane (uniform lane index), so the solve costs a
@@ -574,27 +596,74 @@ __device__ __forceinline__ int packed_index(int i, int j)
     return a * NV - (a * (a - 1)) / 2 + (b - a);
 }
 
-// Sum V per-lane values across the wave; the totals stay in LDS at
-// scratch[V*65 + v] for every lane to read.
+// ROI pixel p (column-major, p = jj*S + ii) to row ii and column jj without
+// an integer divide: exact for p < 2^12, as the window decode of stage_a.
+__device__ __forceinline__ void pixel_rc(int p, float invS, int S, int &ii, int &jj)
+{
+    jj = (int)(((float)p + 0.5f) * invS);
+    ii = p - jj * S;
+}
+
+// Lane-permuting moves within quads (DPP quad_perm), FP32 and FP64.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int kQuadXor1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;   // quad_perm [2,3,0,1]
+
+// Reduction scratch: 16 values x 65 (transposed, padded) plus the totals.
+constexpr int kRedChunk = 16;
+__host__ __device__ constexpr int red_elems(int V) { return kRedChunk * 65 + V; }
+
+// Sum V per-lane values across the wave, 16 values at a time: the chunk is
+// written transposed to LDS, lane 4v+q adds entries 16q..16q+15 of value v,
+// and the four quarters combine by quad permutes (the same total in all four
+// lanes).  The totals land in scratch[16*65 + k] for every lane to read.
 template <typename E, int V>
 __device__ __forceinline__ void wave_sum_lds(const E (&v)[V], E *scratch, int lane)
 {
+    E *tot = scratch + kRedChunk * 65;
+    const int vi = lane >> 2, q4 = lane & 3;
 #pragma unroll
-    for (int k = 0; k < V; k++) scratch[k * 65 + lane] = v[k];
-    __syncthreads();
-    if (lane < V) {
-        E s0 = E(0), s1 = E(0), s2 = E(0), s3 = E(0);
-        const E *row = scratch + lane * 65;
-#pragma unroll 4
-        for (int j = 0; j < 64; j += 4) {
-            s0 += row[j];
-            s1 += row[j + 1];
-            s2 += row[j + 2];
-            s3 += row[j + 3];
+    for (int k0 = 0; k0 < V; k0 += kRedChunk) {
+#pragma unroll
+        for (int k = k0; k < V && k < k0 + kRedChunk; k++) scratch[(k - k0) * 65 + lane] = v[k];
+        __syncthreads();
+        const int C = (V - k0) < kRedChunk ? (V - k0) : kRedChunk;
+        E s0 = E(0), s1 = E(0);
+        if (vi < C) {
+            const E *row = scratch + vi * 65 + q4 * 16;
+#pragma unroll
+            for (int j = 0; j < 16; j += 2) {
+                s0 += row[j];
+                s1 += row[j + 1];
+            }
         }
-        scratch[V * 65 + lane] = (s0 + s1) + (s2 + s3);
+        E t = s0 + s1;
+        t += dpp_move<kQuadXor1>(t);
+        t += dpp_move<kQuadXor2>(t);
+        if (vi < C && q4 == 0) tot[k0 + vi] = t;
+        __syncthreads();
     }
-    __syncthreads();
+}
+
+// 1/x for the per-pixel weights: the hardware reciprocal (1 ulp) in FP32, a
+// correctly rounded divide in FP64.
+template <typename E>
+__device__ __forceinline__ E recip(E x)
+{
+    if constexpr (sizeof(E) == 8) return 1.0 / x;
+    else return __builtin_amdgcn_rcpf(x);
 }
 
 template <typename E>

@@ -64,10 +64,12 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     extern __shared__ __align__(16) unsigned char smem[];
     const LdsLayout LL = lds_layout(S, nch, sizeof(E), 0);
     float *s_data = (float *)smem;
-    int *s_pix = (int *)(smem + LL.pix);
+    float *s_rdat = (float *)(smem + LL.rdat);
     const GeoLds<E> gl = {(E *)(smem + LL.fac), (int *)(smem + LL.rtap), (int *)(smem + LL.ctap)};
     E *s_pl = (E *)(smem + LL.pl);
-    const size_t plred = (size_t)3 * plane_stride > (size_t)29 * 65 ? (size_t)3 * plane_stride : (size_t)29 * 65;
+    const size_t pl = (size_t)3 * plane_stride;
+    const size_t plred = pl > (size_t)red_elems(28) ? pl : (size_t)red_elems(28);
+    const float invS = 1.0f / (float)S;
     E *s_red = s_pl;
     E *s_H = s_pl + plred;          // nv * nv
     E *s_J = s_H + nv * nv;         // nv
@@ -75,11 +77,9 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 
     for (int p = lane; p < nch * S2; p += 64) {
         const int k = p / S2, q = p - k * S2;
-        s_data[p] = A.data[dbase + k * cstride + q];
-    }
-    for (int p = lane; p < S2; p += 64) {
-        const int jj = p / S;
-        s_pix[p] = (p - jj * S) | (jj << 16);   // row | column << 16
+        const float d = A.data[dbase + k * cstride + q];
+        s_data[p] = d;
+        if constexpr (sizeof(E) == 4) s_rdat[p] = d > 0.f ? 1.0f / d : 0.f;
     }
     __syncthreads();
 
@@ -160,19 +160,15 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 #pragma unroll
             for (int q = 0; q < 28; q++) acc[q] = E(0);
             for (int p = lane; p < S2; p += 64) {
-                const int pix = s_pix[p];
+                int ii, jj;
+                pixel_rc(p, invS, S, ii, jj);
                 E dudt[8], model;
-                pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3],
-                                  E(0), dudt, model);
+                pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], E(0), dudt,
+                                  model);
                 if (first) dudt[5] = dudt[4];
-                E data = (E)s_data[k * S2 + p];
-                if (data > E(0)) {
-                    acc[0] += E(2) * ((model - data) - data * log(model / data));
-                } else {
-                    acc[0] += E(2) * model;
-                    data = E(0);
-                }
-                const E rm = E(1) / model;
+                E data;
+                acc[0] += deviance_term<E>(model, (E)s_data[k * S2 + p], s_rdat[k * S2 + p], data);
+                const E rm = recip<E>(model);
                 const E dr = data * rm;
                 E t1 = E(1) - dr;
                 E t2 = dr * rm;
@@ -186,7 +182,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
             }
             __syncthreads();
             wave_sum_lds<E, 28>(acc, s_red, lane);
-            const E *tot = s_red + 28 * 65;
+            const E *tot = s_red + kRedChunk * 65;
             if (lane < 36) {
                 const int p = lane / 6, q = lane - p * 6;
                 s_H[A4.slot[p][k] * nv + A4.slot[q][k]] += tot[7 + packed6(p, q)];
@@ -273,10 +269,10 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 #pragma unroll
         for (int q = 0; q < 22; q++) acc[q] = E(0);
         for (int p = lane; p < S2; p += 64) {
-            const int pix = s_pix[p];
+            int ii, jj;
+            pixel_rc(p, invS, S, ii, jj);
             E dudt[8], model;
-            pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3], E(0),
-                              dudt, model);
+            pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], E(0), dudt, model);
             E data = (E)s_data[k * S2 + p];
             int h = 1;
 #pragma unroll
@@ -290,7 +286,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
         }
         __syncthreads();
         wave_sum_lds<E, 22>(acc, s_red, lane);
-        const E *tot = s_red + 22 * 65;
+        const E *tot = s_red + kRedChunk * 65;
         if (lane < 36) {
             const int p = lane / 6, q = lane - p * 6;
             s_H[A4.slot[p][k] * nv + A4.slot[q][k]] += tot[1 + packed6(p, q)];
@@ -316,7 +312,8 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 size_t lds_bytes_4pi(int f64, int S, int nch, int nv)
 {
     const size_t W = S + 4;
-    const size_t plred = 3 * W * W > 29 * 65 ? 3 * W * W : 29 * 65;
+    const size_t pl = 3 * W * W;
+    const size_t plred = pl > (size_t)red_elems(28) ? pl : (size_t)red_elems(28);
     return lds_layout(S, nch, f64 ? 8 : 4, plred + (size_t)nv * nv + nv + 1).total;
 }
 
@@ -370,8 +367,8 @@ hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st)
     A4.dT = L.dT;
     A4.initZ = L.initZ;
     A4.initPhi = L.initPhi;
-    const size_t lds = lds_bytes_4pi(L.f64, L.S, L.nch, n);
     const int nb = L.count;
+    const size_t lds = lds_bytes_4pi(L.f64, L.S, L.nch, n);
     if (L.f64) {
         if (n <= 8) return launch4<double, 8>(A4, nb, st, lds);
         if (n <= 12) return launch4<double, 12>(A4, nb, st, lds);

@@ -48,15 +48,16 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     const int W = S + 4, plane_stride = W * W;
     const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0);
     float *s_data = (float *)smem;
-    int *s_pix = (int *)(smem + LL.pix);
+    float *s_rdat = (float *)(smem + LL.rdat);
     const GeoLds<E> gl = {(E *)(smem + LL.fac), (int *)(smem + LL.rtap), (int *)(smem + LL.ctap)};
     E *s_pl = (E *)(smem + LL.pl);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
+    const float invS = 1.0f / (float)S;
 
     for (int p = lane; p < S2; p += 64) {
-        s_data[p] = dptr[p];
-        const int jj = p / S;
-        s_pix[p] = (p - jj * S) | (jj << 16);   // row | column << 16
+        const float d = dptr[p];
+        s_data[p] = d;
+        if constexpr (sizeof(E) == 4) s_rdat[p] = d > 0.f ? 1.0f / d : 0.f;
     }
     __syncthreads();
 
@@ -152,21 +153,16 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 #pragma unroll
         for (int k = 0; k < NRED; k++) acc[k] = E(0);
         for (int p = lane; p < S2; p += 64) {
-            const int pix = s_pix[p];
+            int ii, jj;
+            pixel_rc(p, invS, S, ii, jj);
             E dudt[8], model;
-            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3], g2,
-                               dudt, model);
+            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
             if constexpr (FT == 7) {
                 if (first) dudt[5] = dudt[4];
             }
-            E data = (E)s_data[p];
-            if (data > E(0)) {
-                acc[0] += E(2) * ((model - data) - data * log(model / data));
-            } else {
-                acc[0] += E(2) * model;
-                data = E(0);
-            }
-            const E rm = E(1) / model;
+            E data;
+            acc[0] += deviance_term<E>(model, (E)s_data[p], s_rdat[p], data);
+            const E rm = recip<E>(model);
             const E dr = data * rm;
             E t1 = E(1) - dr;
             E t2 = dr * rm;
@@ -180,7 +176,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         }
         __syncthreads();   // planes are dead; the scratch reuses them
         wave_sum_lds<E, NRED>(acc, s_red, lane);
-        const E *tot = s_red + NRED * 65;
+        const E *tot = s_red + kRedChunk * 65;
         jv = tot[1 + myrow];
 #pragma unroll
         for (int j = 0; j < NV; j++) hrow[j] = tot[1 + NV + packed_index<NV>(myrow, j)];
@@ -332,10 +328,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 #pragma unroll
     for (int k = 0; k < NF; k++) acc[k] = E(0);
     for (int p = lane; p < S2; p += 64) {
-        const int pix = s_pix[p];
+        int ii, jj;
+        pixel_rc(p, invS, S, ii, jj);
         E dudt[8], model;
-        pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[pix & 0xffff], gl.ctap[pix >> 16], cth[2], cth[3], g2,
-                           dudt, model);
+        pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
         E data = (E)s_data[p];
         int h = 1;
 #pragma unroll
@@ -349,7 +345,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     }
     __syncthreads();
     wave_sum_lds<E, NF>(acc, s_red, lane);
-    const E *tot = s_red + NF * 65;
+    const E *tot = s_red + kRedChunk * 65;
     double mrow[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) mrow[j] = (double)tot[1 + packed_index<NV>(myrow, j)];
@@ -365,12 +361,14 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 }
 
 // ------------------------------------------------------------ launchers
+// planes, shared with the reduction scratch
 static size_t plane_elems(int fittype, int S)
 {
     const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
     const int NPL = fittype == 6 ? 2 : fittype == 7 ? 3 : 8;
     const size_t W = S + 4;
-    const size_t pl = (size_t)NPL * W * W, red = (size_t)(1 + NV + NV * (NV + 1) / 2 + 1) * 65;
+    const size_t pl = (size_t)NPL * W * W;
+    const size_t red = (size_t)red_elems(1 + NV + NV * (NV + 1) / 2);
     return pl > red ? pl : red;
 }
 
@@ -385,12 +383,6 @@ static hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
     const size_t lds = lds_bytes(FT, sizeof(E) == 8, A.S);   // A.S = largest ROI size of the launch
     hipLaunchKernelGGL((fit_kernel<FT, E>), dim3(nblocks), dim3(64), lds, st, A);
     return hipGetLastError();
-}
-
-template <int FT, typename E>
-static hipError_t launch_ft(const Dev &A, int nblocks, hipStream_t st)
-{
-    return launch_t<FT, E>(A, nblocks, st);
 }
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
@@ -424,12 +416,12 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     A.dbg = L.dbg;
     const int nb = L.count;
     switch (L.fittype * 2 + (L.f64 ? 1 : 0)) {
-    case 12: return launch_ft<6, float>(A, nb, st);
-    case 13: return launch_ft<6, double>(A, nb, st);
-    case 14: return launch_ft<7, float>(A, nb, st);
-    case 15: return launch_ft<7, double>(A, nb, st);
-    case 16: return launch_ft<8, float>(A, nb, st);
-    case 17: return launch_ft<8, double>(A, nb, st);
+    case 12: return launch_t<6, float>(A, nb, st);
+    case 13: return launch_t<6, double>(A, nb, st);
+    case 14: return launch_t<7, float>(A, nb, st);
+    case 15: return launch_t<7, double>(A, nb, st);
+    case 16: return launch_t<8, float>(A, nb, st);
+    case 17: return launch_t<8, double>(A, nb, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -33,6 +33,7 @@ struct FitLaunch {
 };
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st);
+// LDS bytes of one fit workgroup (one wave) for ROI size S
 size_t lds_bytes(int fittype, int f64, int S);
 
 struct Fit4PiLaunch {

@@ -73,6 +73,7 @@ struct Frame {
     unsigned long long ticket = 0;
     size_t off = 0, bytes = 0;
     std::vector<GroupOut> outs;
+    std::vector<std::pair<int, double>> work;   // (kernel stream, cost) of each launch
     std::vector<hipEvent_t> events;   // back() is `done`
     bool delivered = false;
 };
@@ -94,9 +95,11 @@ struct DeviceGuard {
 
 struct bsf_session {
     int device = 0, precision = BSF_PREC_MIX, nks = 2;
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // queued, undelivered launch cost per kernel stream
     size_t budget = 0, arena = 0, model_bytes = 0, lds_limit = 0;
     char *d_arena = nullptr, *h_arena = nullptr;
-    hipStream_t s_in = nullptr, s_out = nullptr, ks[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t s_in = nullptr, s_out = nullptr;
+    hipStream_t ks[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::vector<Model> models;
     std::deque<Frame> inflight;
     std::vector<hipEvent_t> event_pool;
@@ -136,6 +139,8 @@ void deliver(bsf_session *s, Frame &f)
     }
     for (hipEvent_t e : f.events) s->event_pool.push_back(e);
     f.events.clear();
+    for (const auto &w : f.work) s->load[w.first] -= w.second;
+    f.work.clear();
     f.delivered = true;
 }
 
@@ -184,6 +189,7 @@ int drain(bsf_session *s)
         if (r) {
             if (!rc) rc = r;
             for (hipEvent_t e : f.events) s->event_pool.push_back(e);
+            for (const auto &w : f.work) s->load[w.first] -= w.second;
             s->inflight.pop_front();
         }
     }
@@ -225,7 +231,7 @@ int bsf_session_create(const bsf_session_config *cfg, bsf_session **out)
     if (cfg->precision != BSF_PREC_MIX && cfg->precision != BSF_PREC_F64)
         return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", cfg->precision);
     const int nks = cfg->kernel_streams ? cfg->kernel_streams : 2;
-    if (nks < 1 || nks > 4) return fail(BSF_ERR_ARG, "bsf: kernel_streams must be 1..4 (got %d)", nks);
+    if (nks < 1 || nks > 8) return fail(BSF_ERR_ARG, "bsf: kernel_streams must be 1..8 (got %d)", nks);
     const size_t budget = cfg->hbm_budget ? cfg->hbm_budget : ((size_t)2 << 30);
     const size_t arena = up(cfg->arena_bytes ? cfg->arena_bytes : ((size_t)256 << 20));
     if (arena > budget)
@@ -278,7 +284,7 @@ int bsf_session_destroy(bsf_session *s)
         for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
         if (s->s_in) (void)hipStreamDestroy(s->s_in);
         if (s->s_out) (void)hipStreamDestroy(s->s_out);
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < 8; j++)
             if (s->ks[j]) (void)hipStreamDestroy(s->ks[j]);
         if (s->d_arena) (void)hipFree(s->d_arena);
         if (s->h_arena) (void)hipHostFree(s->h_arena);
@@ -543,19 +549,21 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
     F.events.push_back(ev_in);
     hipError_t e = hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s->s_in);
     if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
-    // largest launches first, each to the least-loaded kernel stream
+    // largest launches first, each to the kernel stream with the least queued
+    // work (launches of earlier frames not yet delivered included), so the
+    // big launches of consecutive frames run side by side
     std::vector<int> &ord = s->order;
     ord.resize(L.size());
     for (size_t i = 0; i < L.size(); i++) ord[i] = (int)i;
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return L[a].cost > L[b].cost; });
-    double load[4] = {0, 0, 0, 0};
     for (int i : ord) {
         if (e != hipSuccess) break;
         Launch &l = L[i];
         int j = 0;
         for (int q = 1; q < s->nks; q++)
-            if (load[q] < load[j]) j = q;
-        load[j] += l.cost;
+            if (s->load[q] < s->load[j]) j = q;
+        s->load[j] += l.cost;
+        F.work.emplace_back(j, l.cost);
         l.stream = j;
         hipStream_t ks = s->ks[j];
         const Model &M = s->models[l.model];
@@ -637,6 +645,7 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
         for (int j = 0; j < s->nks; j++) (void)hipStreamSynchronize(s->ks[j]);
         (void)hipStreamSynchronize(s->s_out);
         for (hipEvent_t ev : F.events) s->event_pool.push_back(ev);
+        for (const auto &w : F.work) s->load[w.first] -= w.second;
         return fail(BSF_ERR_DEVICE, "bsf: frame submission failed: %s", hipGetErrorString(e));
     }
     for (Launch &l : L)

@@ -81,6 +81,7 @@ int validate(const bsf_request *r, const bsf_result *res)
     return BSF_OK;
 }
 
+// The device is a gfx950 and one workgroup's LDS fits.
 int check_device(int S, int ft, int f64)
 {
     int dev = 0;
@@ -195,7 +196,7 @@ int bsf_fit_host(const bsf_request *r, bsf_result *res)
 #endif
     if (e == hipSuccess) {
         bsf::FitLaunch L = make_launch(r, d_data, d_c, nc3 ? d_c3 : nullptr, d_P, d_CR, d_LL);
-#ifdef BSF_TRACE
+    #ifdef BSF_TRACE
         L.dbg = d_tr;
 #endif
         e = launch_all(L, st);
@@ -296,7 +297,8 @@ static int check_device_4pi(const bsf_request_4pi *r)
     hipDeviceProp_t p;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
         return fail(BSF_ERR_DEVICE, "bsf: device query failed");
-    size_t lds = bsf::lds_bytes_4pi(r->precision == BSF_PREC_F64, r->sz, r->nch, nv_4pi(r->shared, r->nch));
+    const int f64 = r->precision == BSF_PREC_F64, nv = nv_4pi(r->shared, r->nch);
+    size_t lds = bsf::lds_bytes_4pi(f64, r->sz, r->nch, nv);
     if (lds > p.sharedMemPerBlock)
         return fail(BSF_ERR_ROI_SIZE, "bsf: 4Pi ROI size %d x %d channels needs %zu B of LDS (limit %zu B)", r->sz,
                     r->nch, lds, (size_t)p.sharedMemPerBlock);
@@ -384,7 +386,10 @@ int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
     if (e == hipSuccess && bt) e = hipMemcpyAsync(d_dT, r->dT, bt, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && bz) e = hipMemcpyAsync(d_iz, r->initZ, bz, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && bp) e = hipMemcpyAsync(d_ip, r->initPhi, bp, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = launch_all_4pi(make_launch_4pi(r, d_data, d_c, d_dT, d_iz, d_ip, d_P, d_CR, d_LL), st);
+    if (e == hipSuccess) {
+        bsf::Fit4PiLaunch L = make_launch_4pi(r, d_data, d_c, d_dT, d_iz, d_ip, d_P, d_CR, d_LL);
+            e = launch_all_4pi(L, st);
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(res->P, d_P, N * (nv + 1) * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->CRLB, d_CR, N * nv * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->LL, d_LL, N * 4, hipMemcpyDeviceToHost, st);
@@ -428,7 +433,7 @@ int bsf_test_eval_pass(const bsf_request *r, const float *theta, float *out)
         bsf::FitLaunch L = make_launch(r, d_data, d_c, d_c3, d_out, d_out + N * (NV + 1), d_out + N * (2 * NV + 1));
         L.theta_in = d_th;
         L.dbg = d_o;
-        e = launch_all(L, nullptr);
+            e = launch_all(L, nullptr);
     }
     if (e == hipSuccess) e = hipMemcpy(out, d_o, no * 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
