@@ -107,12 +107,12 @@ struct Dev {
 // (row, column) table, per-row / per-column tap offsets, axis factors, then
 // the planes (reused as reduction scratch) and kernel-specific extras.
 struct LdsLayout {
-    size_t rdat, rtap, ctap, fac, pl, total;
+    size_t rdat, rtap, ctap, fac, hj, pl, total;
 };
 
 __host__ __device__ inline size_t lds_align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline LdsLayout lds_layout(int S, int nch, size_t esz, size_t pl_elems)
+__host__ __device__ inline LdsLayout lds_layout(int S, int nch, size_t esz, size_t pl_elems, size_t hj_elems = 0)
 {
     LdsLayout L;
     size_t o = lds_align16((size_t)nch * S * S * 4);
@@ -124,6 +124,8 @@ __host__ __device__ inline LdsLayout lds_layout(int S, int nch, size_t esz, size
     o = lds_align16(o + (size_t)S * 4);
     L.fac = o;
     o = lds_align16(o + 40 * esz);
+    L.hj = o;                                    // J and packed H of the last evaluation
+    o = lds_align16(o + hj_elems * esz);
     L.pl = o;
     L.total = o + esz * pl_elems;
     return L;
@@ -594,6 +596,53 @@ __device__ __forceinline__ int packed_index(int i, int j)
 {
     const int a = i < j ? i : j, b = i < j ? j : i;
     return a * NV - (a * (a - 1)) / 2 + (b - a);
+}
+
+// Damped normal equations H x = J in FP64, the same in every lane: J and the
+// packed upper triangle of H come from LDS (jh[0..NV) = J, jh[NV..) = H).
+// Square-root-free LDL^T, right-looking.  Its pivots are the reference
+// Cholesky's radicands (CPUsplineLib.cpp:1363-1388): the solve fails, as
+// there, when one is negative.  Returns 0 and x on success.
+template <int NV, typename E>
+__device__ __forceinline__ int ldlt_solve(const E *jh, double (&x)[NV])
+{
+    double a[NV][NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) a[i][j] = (double)jh[NV + packed_index<NV>(j, i)];
+    double r[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double d = a[k][k];
+        if (!(d >= 0.0)) return 1;
+        r[k] = 1.0 / d;
+        double t[NV];
+#pragma unroll
+        for (int i = k + 1; i < NV; i++) {
+            t[i] = a[i][k];              // l_ik d_k
+            a[i][k] = t[i] * r[k];       // l_ik
+        }
+#pragma unroll
+        for (int i = k + 1; i < NV; i++)
+#pragma unroll
+            for (int j = k + 1; j <= i; j++) a[i][j] -= a[i][k] * t[j];
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) {       // L y = J, then z = D^-1 y
+        double y = (double)jh[i];
+#pragma unroll
+        for (int k = 0; k < i; k++) y -= a[i][k] * x[k];
+        x[i] = y;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) x[i] *= r[i];
+#pragma unroll
+    for (int i = NV - 1; i >= 0; i--) {  // L^T x = z
+#pragma unroll
+        for (int k = i + 1; k < NV; k++) x[i] -= a[k][i] * x[k];
+    }
+    return 0;
 }
 
 // ROI pixel p (column-major, p = jj*S + ii) to row ii and column jj without

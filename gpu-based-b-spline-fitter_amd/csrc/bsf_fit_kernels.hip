@@ -46,11 +46,15 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     }
     const int S = A.S, S2 = S * S;
     const int W = S + 4, plane_stride = W * W;
-    const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0);
+    const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0, NV + NH);
     float *s_data = (float *)smem;
     float *s_rdat = (float *)(smem + LL.rdat);
     const GeoLds<E> gl = {(E *)(smem + LL.fac), (int *)(smem + LL.rtap), (int *)(smem + LL.ctap)};
     E *s_pl = (E *)(smem + LL.pl);
+    // J and packed H of the last evaluation: the reference keeps the Hessian
+    // between iterations (a rejected step reuses it, a failed Cholesky leaves
+    // its damped diagonal), so they live in LDS across iterations
+    E *s_jh = (E *)(smem + LL.hj);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
     const float invS = 1.0f / (float)S;
 
@@ -133,14 +137,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     E cth[NV];
     Geo<E, FT> g;
     E g2 = E(0);
-    // lane i (< NV) holds row i of the Hessian and J_i; the reference keeps
-    // the Hessian between iterations (a rejected step reuses it, a failed
-    // Cholesky leaves its damped diagonal), so does this.
-    E hrow[NV], jv = E(0);
     const int myrow = myl;
 
-    // one ROI evaluation at the current parameters: fills hrow / jv and
-    // returns the deviance
+    // one ROI evaluation at the current parameters: fills s_jh and returns
+    // the deviance
     auto evaluate = [&](bool first) __attribute__((always_inline)) -> E {
 #pragma unroll
         for (int i = 0; i < NV; i++) cth[i] = readlane_e<E>(th, i);
@@ -177,9 +177,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         __syncthreads();   // planes are dead; the scratch reuses them
         wave_sum_lds<E, NRED>(acc, s_red, lane);
         const E *tot = s_red + kRedChunk * 65;
-        jv = tot[1 + myrow];
-#pragma unroll
-        for (int j = 0; j < NV; j++) hrow[j] = tot[1 + NV + packed_index<NV>(myrow, j)];
+        if (lane < NV + NH) s_jh[lane] = tot[1 + lane];
         return tot[0];
     };
 
@@ -188,11 +186,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         th = (E)A.theta_in[(size_t)n * NV + myl];
         const E e = evaluate(false);
         float *o = A.dbg + (size_t)n * (1 + NV + NV * NV);
+        __syncthreads();
         if (lane == 0) o[0] = (float)e;
         if (lane < NV) {
-            o[1 + lane] = (float)jv;
+            o[1 + lane] = (float)s_jh[lane];
 #pragma unroll
-            for (int j = 0; j < NV; j++) o[1 + NV + lane * NV + j] = (float)hrow[j];
+            for (int j = 0; j < NV; j++) o[1 + NV + lane * NV + j] = (float)s_jh[NV + packed_index<NV>(lane, j)];
         }
         return;
     }
@@ -213,17 +212,24 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             if (newLambda > E(1)) newLambda = (E)((double)newLambda * 0.8);
             else if (newLambda < E(1)) newLambda = E(1);
         }
-        double arow[NV], xi;
-#pragma unroll
-        for (int j = 0; j < NV; j++) {
-            hrow[j] = (j == lane) ? hrow[j] * newLambda : hrow[j];
-            arow[j] = (double)hrow[j];
+        // damp the diagonal in place (it persists, as the reference's)
+        __syncthreads();
+        if (lane < NV) {
+            E *dg = s_jh + NV + packed_index<NV>(lane, lane);
+            *dg = *dg * newLambda;
         }
-        int info = chol_solve_rows<NV>(arow, (double)jv, lane, xi);
+        __syncthreads();
+        double xs[NV];
+        int info = ldlt_solve<NV, E>(s_jh, xs);
         if (info == 0) {
-            upd = (E)xi;
+            E u = (E)xs[0];
+#pragma unroll
+            for (int i = 1; i < NV; i++) u = (myl == i) ? (E)xs[i] : u;
+            upd = u;
             if constexpr (FT != 6) {
-                if (__any(lane < NV && isnan(upd))) info = 1;
+#pragma unroll
+                for (int i = 0; i < NV; i++)
+                    if (isnan((E)xs[i])) info = 1;
             }
         }
         if (info != 0) {
@@ -317,8 +323,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         __syncthreads();
         make_geo<E, FT>(A, cth, g, lane, gl);
     } else {
+        // the last evaluation's geometry again (x, y, z, w, u unchanged in
+        // cth; recomputing it is cheaper than keeping it live in registers)
         cth[2] = readlane_e<E>(th, 2);
         cth[3] = readlane_e<E>(th, 3);
+        __syncthreads();
+        make_geo<E, FT>(A, cth, g, lane, gl);
     }
     __syncthreads();
     stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
@@ -374,7 +384,8 @@ static size_t plane_elems(int fittype, int S)
 
 size_t lds_bytes(int fittype, int f64, int S)
 {
-    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, S)).total;
+    const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
+    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, S), NV + NV * (NV + 1) / 2).total;
 }
 
 template <int FT, typename E>
