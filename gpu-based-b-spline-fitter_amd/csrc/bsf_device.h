@@ -21,28 +21,30 @@ template <>
 struct Ft<8> { static constexpr int D = 5, K = 3, NV = 8, NPL = 8; };
 
 // ---------------------------------------------------------------- basis
-// Centred cubic / quadratic B-spline, the reference's piecewise formulas in
-// the same operation order (CPUsplineLib.cpp:1289-1326).
+// Centred cubic / quadratic B-splines (the reference's kernel_evalBSpline,
+// CPUsplineLib.cpp:1289-1326, writes them as four / three polynomial pieces
+// in x) in closed symmetric form without branches:
+//   beta3(x) = 2/3 - x^2 + |x|^3/2 (|x| < 1), (2 - |x|)^3 / 6 (|x| < 2), else 0
+//   beta2(x) = 3/4 - x^2 (|x| < 1/2), (3/2 - |x|)^2 / 2 (|x| < 3/2), else 0
+// Equal to the pieces up to rounding, and more accurate near the ends of the
+// support; every lane of make_geo evaluates a different piece, so branches
+// would run all of them.
 template <typename E>
-__device__ __forceinline__ E bsp3(E x)
+__device__ __forceinline__ E bsp3s(E x)
 {
-    const E c43 = E(4) / E(3), c23 = E(2) / E(3), c16 = E(1) / E(6);
-    E x2 = x * x, x3 = x2 * x, b = E(0);
-    if (x >= E(1) && x < E(2)) b = c43 - E(2) * x + x2 - c16 * x3;
-    else if (x >= E(0) && x < E(1)) b = c23 - x2 + E(0.5) * x3;
-    else if (x >= E(-1) && x < E(0)) b = c23 - x2 - E(0.5) * x3;
-    else if (x >= E(-2) && x < E(-1)) b = c43 + E(2) * x + x2 + c16 * x3;
-    return b;
+    const E a = fabs(x), a2 = a * a, t = E(2) - a;
+    const E inner = (E(2) / E(3) - a2) + E(0.5) * a2 * a;
+    const E outer = t * t * t * (E(1) / E(6));
+    return a < E(1) ? inner : (a < E(2) ? outer : E(0));
 }
 
 template <typename E>
-__device__ __forceinline__ E bsp2(E x)
+__device__ __forceinline__ E bsp2s(E x)
 {
-    E x2 = x * x, b = E(0);
-    if (x >= E(0.5) && x < E(1.5)) b = E(1.125) - E(1.5) * x + E(0.5) * x2;
-    else if (x >= E(-0.5) && x < E(0.5)) b = E(0.75) - x2;
-    else if (x >= E(-1.5) && x < E(-0.5)) b = (E)((double)E(1.125) + 1.5 * (double)x + (double)(E(0.5) * x2));
-    return b;
+    const E a = fabs(x), t = E(1.5) - a;
+    const E inner = E(0.75) - a * a;
+    const E outer = E(0.5) * t * t;
+    return a < E(0.5) ? inner : (a < E(1.5) ? outer : E(0));
 }
 
 // Per-axis factors for fractional part d (the reference's B1/B2/dB1/dB2 at
@@ -159,12 +161,11 @@ __device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> 
 #pragma unroll
         for (int i = 2; i < D; i++) c = (a == i) ? cth[2 + i] : c;
         const E d = c - floor(c), dd = d - E(1);
-        E v;
-        if (f < 2) v = bsp3<E>(d + (E)idx - E(1));
-        else if (f < 4) v = bsp3<E>(d - (E)idx);
-        else if (f < 6) v = bsp2<E>(dd + (E)idx - E(0.5));
-        else v = bsp2<E>(dd - (E)idx + E(0.5));
-        L.fac[lane] = v;
+        // f 0,1: beta3(d + idx - 1); 2,3: beta3(d - idx);
+        //   4,5: beta2(dd + idx - 1/2); 6,7: beta2(dd - idx + 1/2)
+        const E x3 = (f < 2) ? d + (E)(idx - 1) : d - (E)idx;
+        const E x2 = (f < 6) ? dd + ((E)idx - E(0.5)) : dd - ((E)idx - E(0.5));
+        L.fac[lane] = (f < 4) ? bsp3s<E>(x3) : bsp2s<E>(x2);
     }
     // per row / column taps; the sequence is non-decreasing except where a
     // coordinate equals n exactly (tap n+1, the maximum; all later ones are n)
@@ -599,24 +600,37 @@ __device__ __forceinline__ int packed_index(int i, int j)
 }
 
 // Damped normal equations H x = J in FP64, the same in every lane: J and the
-// packed upper triangle of H come from LDS (jh[0..NV) = J, jh[NV..) = H).
+// packed upper triangle of H come from LDS (jh[0..NV) = J, jh[NV..) = H, the
+// evaluation's sums held exactly in FP64).
 // Square-root-free LDL^T, right-looking.  Its pivots are the reference
 // Cholesky's radicands (CPUsplineLib.cpp:1363-1388): the solve fails, as
 // there, when one is negative.  Returns 0 and x on success.
-template <int NV, typename E>
-__device__ __forceinline__ int ldlt_solve(const E *jh, double (&x)[NV])
+// 1/d in FP64: v_rcp_f64 and two Newton steps (within an ulp for the
+// normal pivots here); 1/0 = +inf as a divide gives.
+__device__ __forceinline__ double rcp_f64(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return d == 0.0 ? __builtin_inf() : r;
+}
+
+template <int NV>
+__device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV])
 {
     double a[NV][NV];
 #pragma unroll
     for (int i = 0; i < NV; i++)
 #pragma unroll
-        for (int j = 0; j <= i; j++) a[i][j] = (double)jh[NV + packed_index<NV>(j, i)];
+        for (int j = 0; j <= i; j++) a[i][j] = jh[NV + packed_index<NV>(j, i)];
     double r[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         const double d = a[k][k];
         if (!(d >= 0.0)) return 1;
-        r[k] = 1.0 / d;
+        r[k] = rcp_f64(d);
         double t[NV];
 #pragma unroll
         for (int i = k + 1; i < NV; i++) {
@@ -630,7 +644,7 @@ __device__ __forceinline__ int ldlt_solve(const E *jh, double (&x)[NV])
     }
 #pragma unroll
     for (int i = 0; i < NV; i++) {       // L y = J, then z = D^-1 y
-        double y = (double)jh[i];
+        double y = jh[i];
 #pragma unroll
         for (int k = 0; k < i; k++) y -= a[i][k] * x[k];
         x[i] = y;

@@ -274,11 +274,12 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
             E dudt[8], model;
             pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], E(0), dudt, model);
             E data = (E)s_data[k * S2 + p];
+            const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
             int h = 1;
 #pragma unroll
             for (int a = 0; a < 6; a++)
 #pragma unroll
-                for (int b = a; b < 6; b++) acc[h++] += dudt[b] * dudt[a] / model;
+                for (int b = a; b < 6; b++) acc[h++] += dudt[b] * dudt[a] * rmf;
             if (model > E(0)) {
                 if (data > E(0)) acc[0] += data * log(model) - model - data * log(data) + data;
                 else acc[0] += -model;

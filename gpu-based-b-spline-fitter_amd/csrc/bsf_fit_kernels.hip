@@ -46,7 +46,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     }
     const int S = A.S, S2 = S * S;
     const int W = S + 4, plane_stride = W * W;
-    const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0, NV + NH);
+    const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0, (NV + NH) * 8 / sizeof(E));
     float *s_data = (float *)smem;
     float *s_rdat = (float *)(smem + LL.rdat);
     const GeoLds<E> gl = {(E *)(smem + LL.fac), (int *)(smem + LL.rtap), (int *)(smem + LL.ctap)};
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     // J and packed H of the last evaluation: the reference keeps the Hessian
     // between iterations (a rejected step reuses it, a failed Cholesky leaves
     // its damped diagonal), so they live in LDS across iterations
-    E *s_jh = (E *)(smem + LL.hj);
+    double *s_jh = (double *)(smem + LL.hj);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
     const float invS = 1.0f / (float)S;
 
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         __syncthreads();   // planes are dead; the scratch reuses them
         wave_sum_lds<E, NRED>(acc, s_red, lane);
         const E *tot = s_red + kRedChunk * 65;
-        if (lane < NV + NH) s_jh[lane] = tot[1 + lane];
+        if (lane < NV + NH) s_jh[lane] = (double)tot[1 + lane];
         return tot[0];
     };
 
@@ -215,12 +215,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         // damp the diagonal in place (it persists, as the reference's)
         __syncthreads();
         if (lane < NV) {
-            E *dg = s_jh + NV + packed_index<NV>(lane, lane);
-            *dg = *dg * newLambda;
+            double *dg = s_jh + NV + packed_index<NV>(lane, lane);
+            *dg = (double)((E)*dg * newLambda);
         }
         __syncthreads();
         double xs[NV];
-        int info = ldlt_solve<NV, E>(s_jh, xs);
+        int info = ldlt_solve<NV>(s_jh, xs);
         if (info == 0) {
             E u = (E)xs[0];
 #pragma unroll
@@ -343,11 +343,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         E dudt[8], model;
         pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
         E data = (E)s_data[p];
+        const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
         int h = 1;
 #pragma unroll
         for (int a = 0; a < NV; a++)
 #pragma unroll
-            for (int b = a; b < NV; b++) acc[h++] += dudt[b] * dudt[a] / model;
+            for (int b = a; b < NV; b++) acc[h++] += dudt[b] * dudt[a] * rmf;
         if (model > E(0)) {
             if (data > E(0)) acc[0] += data * log(model) - model - data * log(data) + data;
             else acc[0] += -model;
@@ -385,7 +386,7 @@ static size_t plane_elems(int fittype, int S)
 size_t lds_bytes(int fittype, int f64, int S)
 {
     const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
-    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, S), NV + NV * (NV + 1) / 2).total;
+    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, S), (NV + NV * (NV + 1) / 2) * (f64 ? 1 : 2)).total;
 }
 
 template <int FT, typename E>
