@@ -150,10 +150,20 @@ def serve(args):
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, args.frames))
     print("serve: models, pools and %d frames ready in %.1f s (mean %.0f ROIs/frame)"
           % (args.frames, time.perf_counter() - t0, np.mean([f[1] for f in frames])), file=sys.stderr, flush=True)
-    for g, _ in frames[:20]:   # warm-up: code objects loaded, streams and events created
-        sess.poll(sess.submit(g), wait=True)
+    # warm-up with frames in flight, so that every stream (and its hardware
+    # queue) and the event pool are set up before the timed phases
+    warm = [sess.submit(g) for g, _ in frames[:4 * args.kernel_streams]]
+    for t in warm:
+        sess.poll(t, wait=True)
 
-    # ---- open loop at the offered rate
+    # ---- open loop at the offered rate.  The driver's own garbage collector
+    # is frozen for the timed phases: a full pass over the pre-built frames
+    # stalls this loop for tens of milliseconds (a pause of the harness, not
+    # of the session)
+    import gc
+    gc.collect()
+    gc.freeze()
+    gc.disable()
     lat = []
     pending = collections.deque()
     i = 0
@@ -173,6 +183,10 @@ def serve(args):
     span = last - start
     nroi = sum(f[1] for f in frames)
     lat_ms = np.array(lat) * 1e3
+    slow = np.nonzero(lat_ms > 20.0)[0]
+    if len(slow):
+        print("serve: %d frames above 20 ms (completion order %d..%d)" % (len(slow), slow[0], slow[-1]),
+              file=sys.stderr, flush=True)
 
     # ---- closed loop: args.depth frames in flight
     pending = collections.deque()
@@ -184,6 +198,8 @@ def serve(args):
     while pending:
         sess.poll(pending.popleft(), wait=True)
     cspan = time.perf_counter() - cstart
+    gc.enable()
+    gc.unfreeze()
     sess.close()
     out = {
         "metric": "serving: ROI fits/sec and frame latency, streamed 3D/4D/4Pi/5D mix",
@@ -197,7 +213,8 @@ def serve(args):
         "config": {"workload": "frames at %g/s (Poisson arrivals), Poisson(%d) ROIs per frame, fitters "
                                "40/20/20/20 3D/colour-4D/4Pi(4 ch)/vortex-5D, S uniform in {11,13,15,17}, "
                                "50 LM iterations" % (args.rate, args.rois),
-                   "kernel_streams": args.kernel_streams, "precision": args.precision},
+                   "kernel_streams": args.kernel_streams, "precision": args.precision,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "offered_fits_per_s": nroi / float(arrivals[-1]),
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p90": float(np.percentile(lat_ms, 90)),
                        "p99": float(np.percentile(lat_ms, 99)), "max": float(lat_ms.max()),
@@ -222,7 +239,7 @@ def main():
     ap.add_argument("--frames", type=int, default=3000, help="serve: frames")
     ap.add_argument("--rate", type=float, default=1000.0, help="serve: frame arrivals per second")
     ap.add_argument("--rois", type=int, default=2000, help="serve: mean ROIs per frame")
-    ap.add_argument("--kernel-streams", type=int, default=2, help="serve: session kernel streams")
+    ap.add_argument("--kernel-streams", type=int, default=6, help="serve: session kernel streams")
     ap.add_argument("--depth", type=int, default=8, help="serve: frames in flight in the closed-loop phase")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -233,6 +250,12 @@ def main():
               "(ROIs shard across GPUs with no exchange, see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "serve":
+        # the session's kernel streams plus its two copy streams each need a
+        # hardware queue of their own to run concurrently; HIP reads this
+        # when it initialises (its default is 4)
+        need = min(32, args.kernel_streams + 2)
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < need:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(need)
         serve(args)
         return
 

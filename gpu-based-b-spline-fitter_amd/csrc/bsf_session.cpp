@@ -264,6 +264,11 @@ int bsf_session_create(const bsf_session_config *cfg, bsf_session **out)
         return fail(e == hipErrorOutOfMemory ? BSF_ERR_NOMEM : BSF_ERR_DEVICE, "bsf: session set-up failed: %s",
                     hipGetErrorString(e));
     }
+    // Touch every page of the pinned mirror now: a page's first CPU write
+    // faults, and packing a frame into fresh pages runs at a quarter of the
+    // speed, which a stream of frames cannot absorb once the ring advances
+    // into them.
+    std::memset(s->h_arena, 0, arena);
     *out = s;
     return BSF_OK;
 }
