@@ -224,6 +224,70 @@ def serve(args):
     print(json.dumps(out))
 
 
+def coeff_bench(args):
+    """Cubic B-spline coefficient build of the vortex-5D model (21x21x21x17x9
+    samples, lambda 0 on every axis), samples and coefficients resident in
+    HBM, one bsf_coeff_build_device per step.  Roofline: HBM, algorithmic
+    bytes = samples read + coefficients written (FP32).  CPU baseline: the
+    oracle's numpy FP64 restatement of the reference's bsarray."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import bsfit
+    from bsfit import gen
+    lib = bsfit.load()
+    lib.bsf_coeff_work_bytes.argtypes = [C.c_void_p, C.c_int]
+    lib.bsf_coeff_work_bytes.restype = C.c_size_t
+    lib.bsf_coeff_build_device.argtypes = [C.c_void_p] * 2 + [C.c_int] + [C.c_void_p] * 4
+    lib.bsf_coeff_build_device.restype = C.c_int
+    x = np.asfortranarray(gen.psf_dipole5d().astype(np.float32))
+    dims = np.array(x.shape, np.int32)
+    nout = int(np.prod([d + 3 for d in x.shape]))
+    dev = torch.device("cuda:0")
+    d_x = torch.from_numpy(np.ascontiguousarray(x.ravel(order="F"))).to(dev)
+    d_c = torch.empty(nout, dtype=torch.float32, device=dev)
+    d_w = torch.empty(lib.bsf_coeff_work_bytes(dims.ctypes.data, x.ndim), dtype=torch.uint8, device=dev)
+    st = torch.cuda.current_stream()
+
+    def step():
+        rc = lib.bsf_coeff_build_device(d_x.data_ptr(), dims.ctypes.data, x.ndim, None, d_c.data_ptr(),
+                                        d_w.data_ptr(), st.cuda_stream)
+        if rc:
+            raise bsfit.BsfError(rc, lib.bsf_last_error().decode())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record(st)
+    for _ in range(args.steps):
+        step()
+    b.record(st)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    ms = a.elapsed_time(b) / args.steps
+    alg = x.size * 4 + nout * 4
+    cpu = None
+    if not args.no_cpu:
+        from oracle import bsarray as B
+        t1 = time.perf_counter()
+        B.bsarray(x.astype(np.float64))
+        tc = time.perf_counter() - t1
+        cpu = {"value": tc * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+               "sample": "the whole build (oracle/bsarray.py, numpy FP64 restatement of bsarray)"}
+    print(json.dumps({
+        "metric": "cubic B-spline coefficient build time (bsarray, degree 3)", "value": ms, "unit": "ms",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
+        "higher_is_better": False, "vs_baseline": None, "dtype": "f64", "data": "synthetic (vortex-5D PSF samples)",
+        "config": {"workload": "vortex-5D model, 21x21x21x17x9 samples -> 24x24x24x20x12 coefficients, lambda 0"},
+        "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": alg / (ms * 1e-3) / 8e12, "traffic": None,
+                     "note": "algorithmic bytes = %d samples + %d coefficients, FP32; the build keeps an FP64 "
+                             "complex work array and makes one pass per axis and pole" % (x.size, nout)},
+        "cpu_baseline": cpu}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,8 +298,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy (for scaling rocprofv3 byte counters)")
-    ap.add_argument("--workload", choices=["fit", "serve"], default="fit",
-                    help="fit: the colour-4D batch fit (default); serve: the streamed mix through a session")
+    ap.add_argument("--workload", choices=["fit", "serve", "coeff"], default="fit",
+                    help="fit: the colour-4D batch fit (default); serve: the streamed mix through a session; "
+                         "coeff: the B-spline coefficient build of the 5D model")
     ap.add_argument("--frames", type=int, default=3000, help="serve: frames")
     ap.add_argument("--rate", type=float, default=1000.0, help="serve: frame arrivals per second")
     ap.add_argument("--rois", type=int, default=2000, help="serve: mean ROIs per frame")
@@ -249,6 +314,9 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(ROIs shard across GPUs with no exchange, see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "coeff":
+        coeff_bench(args)
+        return
     if args.workload == "serve":
         # the session's kernel streams plus its two copy streams each need a
         # hardware queue of their own to run concurrently; HIP reads this

@@ -245,3 +245,21 @@ def GPUmleFit_LM_4Pi_bspline(imstack, shared, iterations, coeffs, dT=None, initZ
                              phaseShift=None):
     """The reference 4Pi demo's call surface (demo_4pi/bspline_based_4piloc_v4.m:196-206)."""
     return fit_4pi(imstack, shared, iterations, coeffs, phaseShift, dT=dT, initZ=initZ, initPhi=initPhi)
+
+
+def coeff_build(samples, lam=None):
+    """Cubic B-spline coefficients of a sample array on the GPU, the
+    reference's bsarray(samples, 'lambda', lam).coeffs (degree 3): samples
+    N-D (Fortran order), lam a scalar or one value per axis (smoothing), or
+    None.  Returns float32 coefficients with every axis + 3, Fortran order."""
+    lib = load()
+    lib.bsf_coeff_build_host.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    lib.bsf_coeff_build_host.restype = C.c_int
+    x = _f32(samples)
+    nd = x.ndim
+    dims = np.ascontiguousarray(x.shape, np.int32)
+    lm = None if lam is None else np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (nd,)))
+    out = np.zeros(tuple(d + 3 for d in x.shape), np.float32, order="F")
+    _check(lib.bsf_coeff_build_host(x.ctypes.data, dims.ctypes.data, nd,
+                                    lm.ctypes.data if lm is not None else None, out.ctypes.data))
+    return out

@@ -34,6 +34,12 @@ class ModelDesc(C.Structure):
                 ("shared", C.c_uint * 6), ("phaseShift", C.c_float * 8)]
 
 
+class ModelSamples(C.Structure):
+    _fields_ = [("fittype", C.c_int), ("samples", C.c_void_p), ("ndim", C.c_int), ("dims", C.c_int * 5),
+                ("lambda_", C.c_void_p), ("samples3", C.c_void_p), ("lambda3", C.c_void_p), ("nch", C.c_int),
+                ("shared", C.c_uint * 6), ("phaseShift", C.c_float * 8)]
+
+
 class RoiGroup(C.Structure):
     _fields_ = [("model", C.c_int), ("sz", C.c_int), ("count", C.c_int), ("iterations", C.c_int),
                 ("data", C.c_void_p), ("chan_stride", C.c_size_t), ("initP", C.c_void_p), ("n_initP", C.c_int),
@@ -52,11 +58,13 @@ def _bind(lib):
     lib.bsf_session_destroy.argtypes = [C.c_void_p]
     lib.bsf_model_register.argtypes = [C.c_void_p, C.POINTER(ModelDesc), C.POINTER(C.c_int)]
     lib.bsf_model_release.argtypes = [C.c_void_p, C.c_int]
+    lib.bsf_model_register_samples.argtypes = [C.c_void_p, C.POINTER(ModelSamples), C.POINTER(C.c_int)]
     lib.bsf_session_set_precision.argtypes = [C.c_void_p, C.c_int]
     lib.bsf_submit.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_int, C.POINTER(C.c_ulonglong)]
     lib.bsf_poll.argtypes = [C.c_void_p, C.c_ulonglong, C.c_int]
     lib.bsf_session_inflight.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
-    for f in ("bsf_session_create", "bsf_session_destroy", "bsf_model_register", "bsf_model_release",
+    for f in ("bsf_session_create", "bsf_session_destroy", "bsf_model_register", "bsf_model_register_samples",
+              "bsf_model_release",
               "bsf_session_set_precision", "bsf_submit", "bsf_poll", "bsf_session_inflight"):
         getattr(lib, f).restype = C.c_int
     lib._bsf_session_bound = True
@@ -148,6 +156,39 @@ class Session:
             nv = int(sum(1 if sh[p] else nch for p in range(6)))
         mid = C.c_int(-1)
         _check(self._lib.bsf_model_register(self._h, C.byref(d), C.byref(mid)))
+        self._models[mid.value] = (int(fittype), nv, int(nch) if int(fittype) == FT_4PI else 1)
+        return mid.value
+
+    def register_model_samples(self, fittype, samples, lam=None, samples3=None, lam3=None, nch=1, shared=None,
+                               phaseShift=None):
+        """Register a model from its PSF samples: the coefficients are built
+        on the device (bsf_model_register_samples); lam / lam3 per-axis
+        smoothing as bsarray's 'lambda'."""
+        x = _f32(samples)
+        x3 = _f32(samples3) if samples3 is not None else None
+        d = ModelSamples()
+        d.fittype = int(fittype)
+        d.samples = x.ctypes.data
+        d.ndim = x.ndim
+        for a, n in enumerate(x.shape[:5]):
+            d.dims[a] = int(n)
+        lm = None if lam is None else np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (x.ndim,)))
+        lm3 = None if lam3 is None else np.ascontiguousarray(np.broadcast_to(np.asarray(lam3, np.float64), (3,)))
+        d.lambda_ = lm.ctypes.data if lm is not None else None
+        d.samples3 = x3.ctypes.data if x3 is not None else None
+        d.lambda3 = lm3.ctypes.data if lm3 is not None else None
+        nv = _NV.get(int(fittype), 0)
+        if int(fittype) == FT_4PI:
+            d.nch = int(nch)
+            sh = np.ravel(shared if shared is not None else np.ones(6))
+            for p in range(6):
+                d.shared[p] = int(sh[p] != 0)
+            ps = np.ravel(phaseShift if phaseShift is not None else np.zeros(nch))
+            for k in range(min(8, len(ps))):
+                d.phaseShift[k] = float(ps[k])
+            nv = int(sum(1 if sh[p] else nch for p in range(6)))
+        mid = C.c_int(-1)
+        _check(self._lib.bsf_model_register_samples(self._h, C.byref(d), C.byref(mid)))
         self._models[mid.value] = (int(fittype), nv, int(nch) if int(fittype) == FT_4PI else 1)
         return mid.value
 

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <complex>
+
 namespace bsf {
 
 // Placement of one ROI in a launch that mixes ROI sizes (the serving
@@ -53,6 +55,14 @@ struct Fit4PiLaunch {
 
 hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st);
 size_t lds_bytes_4pi(int f64, int S, int nch, int nv);
+
+// Cubic B-spline coefficients (bsarray, degree 3): samples [dims] column-major
+// (device, FP32), lambda one per axis or NULL, coeff [dims + 3] (device,
+// FP32); work is device scratch of coeff_work_bytes(dims, ndim) bytes.
+constexpr int kMaxCoeffDims = 8;
+hipError_t coeff_build(const float *samples, const int *dims, int ndim, const double *lambda, float *coeff,
+                       void *work, hipStream_t st);
+size_t coeff_work_bytes(const int *dims, int ndim);
 
 // Error text of the calling thread (bsf_last_error): set it and return code.
 int fail(int code, const char *fmt, ...);

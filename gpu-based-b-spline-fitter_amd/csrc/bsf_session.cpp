@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -219,6 +220,60 @@ int validate_model(const bsf_model_desc *m)
     return BSF_OK;
 }
 
+// Fills M from a validated description, checks the HBM budget and allocates
+// the model's device buffers (nc / nc3 coefficient counts).
+int alloc_model(bsf_session *s, const bsf_model_desc *m, Model &M, size_t &nc, size_t &nc3)
+{
+    M.ft = m->fittype;
+    M.ndim = m->coeff_ndim;
+    nc = 1;
+    for (int a = 0; a < M.ndim; a++) {
+        M.cdims[a] = m->coeff_dims[a];
+        nc *= (size_t)m->coeff_dims[a];
+    }
+    nc3 = M.ft == 8 ? (size_t)M.cdims[0] * M.cdims[1] * M.cdims[2] : 0;
+    if (nc * 4 >= ((size_t)1 << 31) || nc3 * 4 >= ((size_t)1 << 31))
+        return fail(BSF_ERR_SHAPE, "bsf: coefficient arrays must be below 2 GiB");
+    M.bytes = (nc * 4 + 255) / 256 * 256 + (nc3 * 4 + 255) / 256 * 256;
+    if (s->model_bytes + M.bytes + s->arena > s->budget)
+        return fail(BSF_ERR_NOMEM, "bsf: model of %zu B exceeds the HBM budget (%zu B used of %zu B)", M.bytes,
+                    s->model_bytes + s->arena, s->budget);
+    if (M.ft == BSF_FT_4PI) {
+        M.nch = m->nch;
+        M.nv = 0;
+        for (int p = 0; p < 6; p++) {
+            M.shared[p] = m->shared[p] ? 1 : 0;
+            M.nv += M.shared[p] ? 1 : M.nch;
+        }
+        for (int k = 0; k < M.nch; k++) M.shift[k] = m->phaseShift[k];
+    } else {
+        M.nv = nv_of(M.ft);
+    }
+    hipError_t e = hipMalloc((void **)&M.d_coeff, nc * 4);
+    if (e == hipSuccess && nc3) e = hipMalloc((void **)&M.d_coeff3, nc3 * 4);
+    if (e != hipSuccess) {
+        if (M.d_coeff) (void)hipFree(M.d_coeff);
+        M.d_coeff = nullptr;
+        return fail(BSF_ERR_NOMEM, "bsf: model allocation failed: %s", hipGetErrorString(e));
+    }
+    return BSF_OK;
+}
+
+// Publishes a model whose buffers were filled (e: the fill's status).
+int add_model(bsf_session *s, Model &M, hipError_t e, int *model_id)
+{
+    if (e != hipSuccess) {
+        (void)hipFree(M.d_coeff);
+        if (M.d_coeff3) (void)hipFree(M.d_coeff3);
+        return fail(BSF_ERR_DEVICE, "bsf: model upload failed: %s", hipGetErrorString(e));
+    }
+    M.live = true;
+    s->model_bytes += M.bytes;
+    s->models.push_back(M);
+    *model_id = (int)s->models.size() - 1;
+    return BSF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -307,46 +362,59 @@ int bsf_model_register(bsf_session *s, const bsf_model_desc *m, int *model_id)
     std::lock_guard<std::mutex> lk(s->mu);
     DeviceGuard g(s->device);
     Model M;
-    M.ft = m->fittype;
-    M.ndim = m->coeff_ndim;
-    size_t nc = 1;
-    for (int a = 0; a < M.ndim; a++) {
-        M.cdims[a] = m->coeff_dims[a];
-        nc *= (size_t)m->coeff_dims[a];
-    }
-    const size_t nc3 = M.ft == 8 ? (size_t)M.cdims[0] * M.cdims[1] * M.cdims[2] : 0;
-    if (nc * 4 >= ((size_t)1 << 31) || nc3 * 4 >= ((size_t)1 << 31))
-        return fail(BSF_ERR_SHAPE, "bsf: coefficient arrays must be below 2 GiB");
-    M.bytes = up(nc * 4) + up(nc3 * 4);
-    if (s->model_bytes + M.bytes + s->arena > s->budget)
-        return fail(BSF_ERR_NOMEM, "bsf: model of %zu B exceeds the HBM budget (%zu B used of %zu B)", M.bytes,
-                    s->model_bytes + s->arena, s->budget);
-    if (M.ft == BSF_FT_4PI) {
-        M.nch = m->nch;
-        M.nv = 0;
-        for (int p = 0; p < 6; p++) {
-            M.shared[p] = m->shared[p] ? 1 : 0;
-            M.nv += M.shared[p] ? 1 : M.nch;
-        }
-        for (int k = 0; k < M.nch; k++) M.shift[k] = m->phaseShift[k];
-    } else {
-        M.nv = nv_of(M.ft);
-    }
-    hipError_t e = hipMalloc((void **)&M.d_coeff, nc * 4);
-    if (e == hipSuccess && nc3) e = hipMalloc((void **)&M.d_coeff3, nc3 * 4);
-    if (e == hipSuccess) e = hipMemcpy(M.d_coeff, m->coeff, nc * 4, hipMemcpyHostToDevice);
+    size_t nc = 0, nc3 = 0;
+    if ((rc = alloc_model(s, m, M, nc, nc3))) return rc;
+    hipError_t e = hipMemcpy(M.d_coeff, m->coeff, nc * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && nc3) e = hipMemcpy(M.d_coeff3, m->coeff3, nc3 * 4, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        if (M.d_coeff) (void)hipFree(M.d_coeff);
-        if (M.d_coeff3) (void)hipFree(M.d_coeff3);
-        return fail(e == hipErrorOutOfMemory ? BSF_ERR_NOMEM : BSF_ERR_DEVICE, "bsf: model upload failed: %s",
-                    hipGetErrorString(e));
+    return add_model(s, M, e, model_id);
+}
+
+int bsf_model_register_samples(bsf_session *s, const bsf_model_samples *m, int *model_id)
+{
+    bsf::clear_error();
+    if (!s || !m || !model_id) return fail(BSF_ERR_ARG, "bsf: NULL session, samples or model id");
+    if (m->ndim < 1 || m->ndim > 5) return fail(BSF_ERR_SHAPE, "bsf: model samples need 3..5 dims (got %d)", m->ndim);
+    bsf_model_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.fittype = m->fittype;
+    d.coeff = m->samples;
+    d.coeff_ndim = m->ndim;
+    for (int a = 0; a < m->ndim; a++) d.coeff_dims[a] = m->dims[a] + 3;
+    d.coeff3 = m->samples3;
+    d.nch = m->nch;
+    for (int p = 0; p < 6; p++) d.shared[p] = m->shared[p];
+    for (int k = 0; k < 8; k++) d.phaseShift[k] = m->phaseShift[k];
+    int rc = validate_model(&d);
+    if (rc) return rc;
+    for (int a = 0; a < m->ndim; a++)
+        if ((m->lambda && (!(m->lambda[a] >= 0.0) || std::isinf(m->lambda[a]))) ||
+            (a < 3 && m->fittype == 8 && m->lambda3 && (!(m->lambda3[a] >= 0.0) || std::isinf(m->lambda3[a]))))
+            return fail(BSF_ERR_ARG, "bsf: lambda must be finite and >= 0 (axis %d)", a);
+    std::lock_guard<std::mutex> lk(s->mu);
+    DeviceGuard g(s->device);
+    Model M;
+    size_t nc = 0, nc3 = 0;
+    if ((rc = alloc_model(s, &d, M, nc, nc3))) return rc;
+    // samples and the builder's scratch live only for the registration
+    size_t n = 1;
+    for (int a = 0; a < m->ndim; a++) n *= (size_t)m->dims[a];
+    char *tmp = nullptr;
+    const size_t xb = (n * 4 + 255) / 256 * 256;
+    hipError_t e = hipMalloc((void **)&tmp, xb + bsf::coeff_work_bytes(m->dims, m->ndim));
+    if (e == hipSuccess) {
+        hipStream_t st = s->s_in;
+        e = hipMemcpyAsync(tmp, m->samples, n * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = bsf::coeff_build((float *)tmp, m->dims, m->ndim, m->lambda, M.d_coeff, tmp + xb, st);
+        if (e == hipSuccess && nc3) {
+            const size_t n3 = (size_t)m->dims[0] * m->dims[1] * m->dims[2];
+            if (e == hipSuccess) e = hipStreamSynchronize(st);   // tmp is reused
+            if (e == hipSuccess) e = hipMemcpyAsync(tmp, m->samples3, n3 * 4, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = bsf::coeff_build((float *)tmp, m->dims, 3, m->lambda3, M.d_coeff3, tmp + xb, st);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        (void)hipFree(tmp);
     }
-    M.live = true;
-    s->model_bytes += M.bytes;
-    s->models.push_back(M);
-    *model_id = (int)s->models.size() - 1;
-    return BSF_OK;
+    return add_model(s, M, e, model_id);
 }
 
 int bsf_model_release(bsf_session *s, int model_id)

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <cmath>
 #include <cstdlib>
 
 #include "../../include/bsfit.h"
@@ -396,6 +397,76 @@ int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     rc = BSF_OK;
     if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
+    (void)hipFree(buf);
+    return rc;
+}
+
+// ------------------------------------------------------ coefficient builder
+// Argument checks of bsarray.m:58-173 for degree 3 (and the builder's limits).
+static int validate_coeff(const int *dims, int ndim, const double *lambda)
+{
+    if (!dims || ndim < 1 || ndim > bsf::kMaxCoeffDims)
+        return fail(BSF_ERR_SHAPE, "bsf: coefficient builder takes 1..%d dimensions (got %d)", bsf::kMaxCoeffDims,
+                    ndim);
+    size_t n = 1;
+    for (int a = 0; a < ndim; a++) {
+        if (dims[a] < 2) return fail(BSF_ERR_SHAPE, "bsf: dimension %d has %d samples; each needs >= 2", a, dims[a]);
+        n *= (size_t)dims[a];
+    }
+    if (n >= ((size_t)1 << 31)) return fail(BSF_ERR_SHAPE, "bsf: sample arrays must have fewer than 2^31 elements");
+    for (int a = 0; lambda && a < ndim; a++)
+        if (!(lambda[a] >= 0.0) || std::isinf(lambda[a]))
+            return fail(BSF_ERR_ARG, "bsf: lambda must be finite and >= 0 (axis %d: %g)", a, lambda[a]);
+    return BSF_OK;
+}
+
+static size_t numel_pad(const int *dims, int ndim, int pad)
+{
+    size_t n = 1;
+    for (int a = 0; a < ndim; a++) n *= (size_t)(dims[a] + pad);
+    return n;
+}
+
+size_t bsf_coeff_work_bytes(const int *dims, int ndim)
+{
+    if (validate_coeff(dims, ndim, nullptr)) return 0;
+    return bsf::coeff_work_bytes(dims, ndim);
+}
+
+int bsf_coeff_build_device(const float *samples, const int *dims, int ndim, const double *lambda, float *coeff,
+                           void *work, void *stream)
+{
+    g_err.clear();
+    int rc = validate_coeff(dims, ndim, lambda);
+    if (rc) return rc;
+    if (!samples || !coeff || !work) return fail(BSF_ERR_ARG, "bsf: NULL samples, coeff or work pointer");
+    if ((rc = check_device(1, 6, 0))) return rc;
+    hipError_t e = bsf::coeff_build(samples, dims, ndim, lambda, coeff, work, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: coefficient build failed: %s", hipGetErrorString(e));
+    return BSF_OK;
+}
+
+int bsf_coeff_build_host(const float *samples, const int *dims, int ndim, const double *lambda, float *coeff)
+{
+    g_err.clear();
+    int rc = validate_coeff(dims, ndim, lambda);
+    if (rc) return rc;
+    if (!samples || !coeff) return fail(BSF_ERR_ARG, "bsf: NULL samples or coeff pointer");
+    if ((rc = check_device(1, 6, 0))) return rc;
+    const size_t n = numel_pad(dims, ndim, 0), nout = numel_pad(dims, ndim, 3);
+    const size_t bw = bsf::coeff_work_bytes(dims, ndim);
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    char *buf = nullptr;
+    hipError_t e = hipMalloc((void **)&buf, up(n * 4) + up(nout * 4) + bw);
+    if (e != hipSuccess) return fail(BSF_ERR_NOMEM, "bsf: device allocation failed");
+    float *d_x = (float *)buf, *d_c = (float *)(buf + up(n * 4));
+    void *d_w = buf + up(n * 4) + up(nout * 4);
+    hipStream_t st = hipStreamPerThread;
+    e = hipMemcpyAsync(d_x, samples, n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = bsf::coeff_build(d_x, dims, ndim, lambda, d_c, d_w, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(coeff, d_c, nout * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    rc = e == hipSuccess ? BSF_OK : fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
     (void)hipFree(buf);
     return rc;
 }

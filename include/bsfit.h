@@ -148,6 +148,28 @@ int bsf_fit_4pi_device(const bsf_request_4pi *req, bsf_result *res, void *stream
  * Replaces mexErrMsgIdAndTxt (GPUmleFit_LM/mexFunction.cpp:140-176). */
 const char *bsf_last_error(void);
 
+/* ------------------------------------------------ coefficient builder
+ * Cubic B-spline coefficients of a sampled PSF model, the array the fit entry
+ * points take as coeff / coeff3.  Replaces the reference's MATLAB
+ *   b = bsarray(a, 'lambda', lambda); coeff = single(b.coeffs)
+ * with degree 3 on every axis (MultiD_bspline/bsarray/bsarray.m:58-173,
+ * directFilter.m:1-297, getBSplineFiltCoeffs.m, symExpFilt.m:26-45), computed
+ * in FP64 on the GPU:
+ *   samples  float [dims[0], ..., dims[ndim-1]] column-major, ndim 1..8,
+ *            every dim >= 2
+ *   lambda   one smoothing weight >= 0 per axis (0 = interpolation), or NULL
+ *   coeff    float [dims[0]+3, ..., dims[ndim-1]+3] column-major */
+int bsf_coeff_build_host(const float *samples, const int *dims, int ndim, const double *lambda, float *coeff);
+
+/* Device scratch bytes bsf_coeff_build_device needs for these dims (0 if the
+ * dims are invalid). */
+size_t bsf_coeff_work_bytes(const int *dims, int ndim);
+
+/* The same on device buffers (samples, coeff, work), queued on `stream`;
+ * lambda stays a host array; nothing is allocated. */
+int bsf_coeff_build_device(const float *samples, const int *dims, int ndim, const double *lambda, float *coeff,
+                           void *work, void *stream);
+
 /* -------------------------------------------------------------- sessions
  * A session keeps models (coefficient arrays) resident on one device and
  * fits frames asynchronously.  A frame is a list of ROI groups; a group is
@@ -231,6 +253,26 @@ int bsf_session_destroy(bsf_session *s);
 /* Copies a model's coefficients to the device; *model_id names it in
  * groups.  Fails with BSF_ERR_NOMEM when models + arena exceed hbm_budget. */
 int bsf_model_register(bsf_session *s, const bsf_model_desc *m, int *model_id);
+
+/* Registers a model from its samples: the coefficients (and, fittype 8, the
+ * 3D coefficients) are built on the device (bsf_coeff_build) straight into
+ * the model's buffers.  Dims are the sample dims (coefficients get +3);
+ * fittype 8's 3D samples share dims[0..2].  Same result as building with
+ * bsf_coeff_build_host and registering the coefficients. */
+typedef struct bsf_model_samples {
+    int fittype;                /* 6, 7, 8 or BSF_FT_4PI */
+    const float *samples;       /* host, [dims] column-major */
+    int ndim;                   /* 3, 4, 5 (4 for BSF_FT_4PI) */
+    int dims[5];
+    const double *lambda;       /* per-axis smoothing, or NULL */
+    const float *samples3;      /* fittype 8: host [dims[0], dims[1], dims[2]] */
+    const double *lambda3;      /* fittype 8: 3 values, or NULL */
+    int nch;                    /* BSF_FT_4PI, as bsf_model_desc */
+    unsigned int shared[6];
+    float phaseShift[8];
+} bsf_model_samples;
+
+int bsf_model_register_samples(bsf_session *s, const bsf_model_samples *m, int *model_id);
 
 /* Frees a model's device memory (waits for frames in flight first). */
 int bsf_model_release(bsf_session *s, int model_id);

@@ -63,3 +63,18 @@ def test_session_errors_without_device_or_bad_args():
         with pytest.raises(bsfit.BsfError) as e:
             bsfit.Session()
         assert e.value.code == 6
+
+
+def test_coefficient_builder_argument_errors():
+    import torch
+    import bsfit
+    with pytest.raises(bsfit.BsfError) as e:
+        bsfit.coeff_build(np.zeros((5, 1, 4), np.float32))       # an axis with one sample
+    assert e.value.code == 4
+    with pytest.raises(bsfit.BsfError) as e:
+        bsfit.coeff_build(np.zeros((5, 4), np.float32), lam=[0.1, -1.0])
+    assert e.value.code == 1
+    if not torch.cuda.is_available():
+        with pytest.raises(bsfit.BsfError) as e:
+            bsfit.coeff_build(np.zeros((5, 4), np.float32))
+        assert e.value.code == 6
