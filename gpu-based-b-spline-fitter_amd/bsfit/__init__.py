@@ -11,7 +11,7 @@ runs in ``libbsfit.so`` (gfx950 HIP kernels); there is no CPU path: if the
 library or a gfx950 device is missing, calls raise ``BsfError``.
 """
 from .api import (  # noqa: F401
-    BsfError, GPUmleFit_LM_multiD, GPUmleFit_LM_4Pi_bspline, coeff_build, fit, fit_4pi, fit_device,
+    BsfError, GPUmleFit_LM_multiD, GPUmleFit_LM_4Pi_bspline, coeff_build, fit, fit_4pi, fit_device, fit_multistart,
     lib_path, load, num_params, num_params_4pi, PREC_MIX, PREC_F64,
 )
 from .session import FT_4PI, Group, Session  # noqa: F401

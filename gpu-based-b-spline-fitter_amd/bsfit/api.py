@@ -126,6 +126,34 @@ def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=
     return P, CR, LL
 
 
+def fit_multistart(fittype, imstack, iterations, coeff, starts, coeff3=None, strict=False, precision=PREC_MIX):
+    """Multi-start fit (bsf_fit_multistart_host): starts (K, 2) for fittype 7
+    or (K, 4) for fittype 8; a later start's result replaces the kept one
+    where its log-likelihood is higher by >= 1e-4 (strict: > 1e-4), as the
+    reference's callers combine their per-start MEX calls.  Returns
+    (P, CRLB, LL) as fit()."""
+    lib = load()
+    lib.bsf_fit_multistart_host.argtypes = [C.POINTER(Request), C.c_void_p, C.c_int, C.c_int, C.POINTER(Result)]
+    lib.bsf_fit_multistart_host.restype = C.c_int
+    imstack = _f32(imstack)
+    if imstack.ndim == 2:
+        imstack = imstack[:, :, None]
+    S, _, N = imstack.shape
+    coeff = _f32(coeff)
+    c3 = _f32(coeff3) if coeff3 is not None else None
+    st = np.ascontiguousarray(np.atleast_2d(starts), np.float32)
+    NV = _NV.get(int(fittype), 1)
+    P = np.zeros((N, NV + 1), np.float32, order="F")
+    CR = np.zeros((N, NV), np.float32, order="F")
+    LL = np.zeros(N, np.float32)
+    req = _request(fittype, S, N, iterations, coeff.ctypes.data, coeff.shape,
+                   c3.ctypes.data if c3 is not None else None, st[0], precision, imstack.ctypes.data)
+    req.n_initP = st.shape[1]
+    res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
+    _check(lib.bsf_fit_multistart_host(C.byref(req), st.ctypes.data, st.shape[0], int(bool(strict)), C.byref(res)))
+    return P, CR, LL
+
+
 def fit_device(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_ptr, CRLB_ptr, LL_ptr,
                initP=None, coeff3_ptr=None, precision=PREC_MIX, stream=None):
     """Queue a fit on device-resident buffers (raw device addresses, e.g.

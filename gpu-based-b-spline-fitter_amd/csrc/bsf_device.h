@@ -103,6 +103,8 @@ struct Dev {
     float *dbg;            // test hook output: err, J[NV], H[NV*NV] per ROI
     float initP[4];
     int has_initP;
+    int keep;              // 0: write the results; 1 / 2: write them only where the log-likelihood
+                           // beats the one in LL by >= / > 1e-4 (a later start of a multi-start fit)
 };
 
 // LDS layout of a fit workgroup (one wave): ROI data (nch channels), pixel

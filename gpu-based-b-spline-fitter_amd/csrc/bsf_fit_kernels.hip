@@ -361,13 +361,22 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 #pragma unroll
     for (int j = 0; j < NV; j++) mrow[j] = (double)tot[1 + packed_index<NV>(myrow, j)];
     const double crlb = inverse_diag_rows<NV>(mrow, lane);
+    const float llf = (float)tot[0];
+    // A later start of a multi-start fit replaces the kept result only where
+    // its log-likelihood beats it by 1e-4, compared in FP32 as the
+    // reference's callers compare the MEX outputs
+    // (demo_orientation/store/fitb5n3.m:38-45: >=; simplefitter_spline.m:488: >).
+    if (A.keep) {
+        const float d = llf - A.LL[n];
+        if (!(A.keep == 2 ? d > 1e-4f : d >= 1e-4f)) return;
+    }
     if (lane < NV) {
         A.P[(size_t)A.N * lane + n] = (float)th;
         A.CRLB[(size_t)A.N * lane + n] = (float)crlb;
     }
     if (lane == 0) {
         A.P[(size_t)A.N * NV + n] = (float)kk;
-        A.LL[n] = (float)tot[0];
+        A.LL[n] = llf;
     }
 }
 
@@ -424,6 +433,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     }
     for (int i = 0; i < 4; i++) A.initP[i] = L.initP[i];
     A.has_initP = L.has_initP;
+    A.keep = L.keep;
     A.theta_in = L.theta_in;
     A.dbg = L.dbg;
     const int nb = L.count;

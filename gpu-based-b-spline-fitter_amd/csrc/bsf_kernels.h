@@ -30,6 +30,7 @@ struct FitLaunch {
     int cdims[5], ndim;
     float initP[4];
     int has_initP;
+    int keep;               // multi-start: 0 write, 1 keep if LL gains >= 1e-4, 2 if > 1e-4
     const float *theta_in;  // test hook (device): NV parameters per ROI, or NULL
     float *dbg;             // test hook (device): err, J, H per ROI
 };

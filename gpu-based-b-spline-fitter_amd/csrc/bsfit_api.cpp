@@ -160,9 +160,12 @@ int bsf_fit_device(const bsf_request *r, bsf_result *res, void *stream)
     return BSF_OK;
 }
 
-int bsf_fit_host(const bsf_request *r, bsf_result *res)
+// Host-array fit from nstarts starts (rows of `starts`, n_initP values each;
+// NULL starts: the request's initP, one start): one upload, one launch per
+// start (later starts keep a result only where it beats the kept
+// log-likelihood), one download.
+static int fit_host_starts(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res)
 {
-    g_err.clear();
     int rc = validate(r, res);
     if (rc) return rc;
     if ((rc = check_device(r->sz, r->fittype, r->precision == BSF_PREC_F64))) return rc;
@@ -197,10 +200,17 @@ int bsf_fit_host(const bsf_request *r, bsf_result *res)
 #endif
     if (e == hipSuccess) {
         bsf::FitLaunch L = make_launch(r, d_data, d_c, nc3 ? d_c3 : nullptr, d_P, d_CR, d_LL);
-    #ifdef BSF_TRACE
+#ifdef BSF_TRACE
         L.dbg = d_tr;
 #endif
-        e = launch_all(L, st);
+        for (int k = 0; k < nstarts && e == hipSuccess; k++) {
+            if (starts) {
+                for (int q = 0; q < r->n_initP && q < 4; q++) L.initP[q] = starts[(size_t)k * r->n_initP + q];
+                L.has_initP = 1;
+            }
+            L.keep = k == 0 ? 0 : (strict ? 2 : 1);
+            e = launch_all(L, st);
+        }
     }
 #ifdef BSF_TRACE
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -220,6 +230,61 @@ int bsf_fit_host(const bsf_request *r, bsf_result *res)
     if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
     (void)hipFree(buf);
     return rc;
+}
+
+int bsf_fit_host(const bsf_request *r, bsf_result *res)
+{
+    g_err.clear();
+    return fit_host_starts(r, nullptr, 1, 0, res);
+}
+
+// The starts of a multi-start request: fittype 7 or 8, 1..64 starts of
+// n_initP (>= 2 / 4) values.
+static int validate_starts(const bsf_request *r, const float *starts, int nstarts)
+{
+    if (!r) return fail(BSF_ERR_ARG, "bsf: NULL request");
+    if (r->fittype != 7 && r->fittype != 8)
+        return fail(BSF_ERR_FITTYPE, "bsf: multi-start fits take fittype 7 or 8 (got %d)", r->fittype);
+    if (!starts || nstarts < 1 || nstarts > 64)
+        return fail(BSF_ERR_ARG, "bsf: multi-start fits need 1..64 starts (got %d)", nstarts);
+    const int need = r->fittype == 7 ? 2 : 4;
+    if (r->n_initP < need)
+        return fail(BSF_ERR_ARG, "bsf: each start of fittype %d needs %d values (n_initP = %d)", r->fittype, need,
+                    r->n_initP);
+    return BSF_OK;
+}
+
+int bsf_fit_multistart_host(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res)
+{
+    g_err.clear();
+    int rc = validate_starts(r, starts, nstarts);
+    if (rc) return rc;
+    bsf_request q = *r;
+    q.initP = starts;   // row 0 passes the single-start checks
+    return fit_host_starts(&q, starts, nstarts, strict, res);
+}
+
+int bsf_fit_multistart_device(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res,
+                              void *stream)
+{
+    g_err.clear();
+    int rc = validate_starts(r, starts, nstarts);
+    if (rc) return rc;
+    bsf_request q = *r;
+    q.initP = starts;
+    if ((rc = validate(&q, res))) return rc;
+    if ((rc = check_device(q.sz, q.fittype, q.precision == BSF_PREC_F64))) return rc;
+    if (q.nfits == 0) return BSF_OK;
+    bsf::FitLaunch L = make_launch(&q, q.imstack, q.coeff, q.coeff3, res->P, res->CRLB, res->LL);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nstarts && e == hipSuccess; k++) {
+        for (int i = 0; i < q.n_initP && i < 4; i++) L.initP[i] = starts[(size_t)k * q.n_initP + i];
+        L.has_initP = 1;
+        L.keep = k == 0 ? 0 : (strict ? 2 : 1);
+        e = launch_all(L, (hipStream_t)stream);
+    }
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
+    return BSF_OK;
 }
 
 int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_ndim, int fittype, int iterations,

@@ -92,6 +92,22 @@ int bsf_fit_host(const bsf_request *req, bsf_result *res);
  * (GPUmleFit_LM/wrapper.cu:256-279). */
 int bsf_fit_device(const bsf_request *req, bsf_result *res, void *stream);
 
+/* Multi-start fit: every ROI is fitted from nstarts starts (rows of
+ * `starts`, req->n_initP values each: fittype 7 {z, w}, fittype 8
+ * {z, w, u, g2}) and the outputs keep, start by start, a result whose
+ * log-likelihood beats the kept one by at least 1e-4 (strict != 0: by more
+ * than 1e-4), compared in FP32.  Replaces the reference callers' loop over
+ * GPUmleFit_LM_multiD calls with the same rule
+ * (demo_orientation/store/fitb5n3.m:32-46 (>=), MultiD_fitter/
+ * simplefitter_spline.m:482-494 (>) and 512-520 (>=)): one upload, one
+ * launch per start on the device buffers, one download.  req->initP is
+ * ignored; fittypes 7 and 8; 1..64 starts. */
+int bsf_fit_multistart_host(const bsf_request *req, const float *starts, int nstarts, int strict, bsf_result *res);
+
+/* The same on device buffers (as bsf_fit_device; starts is a host array). */
+int bsf_fit_multistart_device(const bsf_request *req, const float *starts, int nstarts, int strict, bsf_result *res,
+                              void *stream);
+
 /* The MEX argument list as plain C (GPUmleFit_LM/mexFunction.cpp:14-55):
  * imdims = size(imstack) (2 dims: one ROI), coeff_dims = size(coeff).
  * varmap is accepted and ignored as the MEX ignores it for fittypes 6-8

@@ -78,3 +78,23 @@ def test_coefficient_builder_argument_errors():
         with pytest.raises(bsfit.BsfError) as e:
             bsfit.coeff_build(np.zeros((5, 4), np.float32))
         assert e.value.code == 6
+
+
+def test_multistart_argument_errors():
+    import torch
+    import bsfit
+    from bsfit import gen
+    w = gen.config(1, N=2)
+    with pytest.raises(bsfit.BsfError) as e:                      # fittype 6 has no start
+        bsfit.fit_multistart(6, w.imstack, 50, w.coeff, np.zeros((2, 2), np.float32))
+    assert e.value.code == 2
+    with pytest.raises(bsfit.BsfError) as e:                      # fittype 7 starts are {z, w}
+        bsfit.fit_multistart(7, w.imstack, 50, w.coeff, np.zeros((2, 1), np.float32))
+    assert e.value.code == 1
+    with pytest.raises(bsfit.BsfError) as e:                      # at most 64 starts
+        bsfit.fit_multistart(7, w.imstack, 50, w.coeff, np.zeros((65, 2), np.float32))
+    assert e.value.code == 1
+    if not torch.cuda.is_available():
+        with pytest.raises(bsfit.BsfError) as e:
+            bsfit.fit_multistart(7, w.imstack, 50, w.coeff, np.ones((2, 2), np.float32))
+        assert e.value.code == 6
