@@ -14,4 +14,5 @@ from .api import (  # noqa: F401
     BsfError, GPUmleFit_LM_multiD, GPUmleFit_LM_4Pi_bspline, coeff_build, fit, fit_4pi, fit_device, fit_multistart,
     lib_path, load, num_params, num_params_4pi, PREC_MIX, PREC_F64,
 )
+from .frames import find_rois, peak_filter, peak_filter_psf  # noqa: F401
 from .session import FT_4PI, Group, Session  # noqa: F401

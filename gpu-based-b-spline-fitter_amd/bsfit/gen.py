@@ -396,3 +396,43 @@ def config(idx, N=None, S=None, noise=True, seed=None):
                 out.append(make_rois(8, n, s, seed + 17 * s, c5, c3, noise=noise))
         return out
     raise ValueError("configuration %r has no fittype 6/7/8 workload" % idx)
+
+
+# ------------------------------------------------------------ camera frames
+def make_frames(nframes, nrows, ncols, per_frame, seed, coeff=None, offset=100.0, conversion=0.5, patch=21,
+                margin=None, dtype=np.uint16):
+    """Seeded camera frames (nrows, ncols, nframes), Fortran order, for the
+    frame -> ROI front-end: per frame `per_frame` emitters rendered from the
+    3D astigmatic spline model (27x27x61 grid, as configuration 0) on a
+    patch x patch footprint at random sub-pixel positions and z, photons
+    ~ logU(1e3, 1e4), a background ~ U(5, 30) per frame, Poisson noise, then
+    ADU = photons / conversion + offset, rounded (uint16) or not (float32).
+
+    Returns (frames, truth) with truth rows (frame index from 0, x, y, z
+    spline coordinate, photons, bg), x = column and y = row in pixels from 0."""
+    if coeff is None:
+        coeff = model_coeffs("astig3d")
+    rng = np.random.default_rng(seed)
+    h = (patch - 1) // 2
+    margin = h + 1 if margin is None else margin
+    n = nframes * per_frame
+    fi = np.repeat(np.arange(nframes), per_frame)
+    x = rng.uniform(margin, ncols - 1 - margin, n)
+    y = rng.uniform(margin, nrows - 1 - margin, n)
+    zc = 1 + (rng.uniform(-500, 500, n) + 600) / 20.0
+    ph = np.exp(rng.uniform(math.log(1e3), math.log(1e4), n))
+    bgf = rng.uniform(5, 30, nframes)
+    cj, ci = np.rint(x).astype(int), np.rint(y).astype(int)
+    c0 = float(h)
+    f = eval_spline_rois(coeff, patch, np.stack([c0 + x - cj, c0 + y - ci], 1), zc[:, None])
+    mu = np.repeat(bgf[None, None, :], nrows, 0).repeat(ncols, 1).astype(np.float64)
+    for k in range(n):
+        mu[ci[k] - h:ci[k] + h + 1, cj[k] - h:cj[k] + h + 1, fi[k]] += ph[k] * np.maximum(f[k], 0.0)
+    counts = rng.poisson(mu).astype(np.float64)
+    adu = counts / conversion + offset
+    if dtype == np.uint16:
+        frames = np.clip(np.rint(adu), 0, 65535).astype(np.uint16)
+    else:
+        frames = adu.astype(np.float32)
+    truth = np.stack([fi, x, y, zc, ph, bgf[fi]], 1)
+    return np.asfortranarray(frames), truth

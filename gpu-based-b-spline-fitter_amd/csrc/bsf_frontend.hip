@@ -1,0 +1,450 @@
+// bsf_frontend.hip -- camera frames to the ROI stack the fitter takes, on
+// gfx950: the GUI fitter's per-frame front-end
+// (MultiD_fitter/simplefitter_spline.m:259-307, MultiD_fitter/shared/
+// maximumfind2.c:18-65).
+//
+// Per frame: photons = (single(raw) - offset) * conversion; optionally minus
+// the minimum photon count of the frame's first column; filter2 'same'
+// correlation with an mr x mc kernel (zero padding, tap (u, v) reads row
+// i + u - floor((mr-1)/2), column j + v - floor((mc-1)/2), summed as
+// acc = fmaf(h[v*mr+u], x, acc) from 0, v outer, u inner); strict
+// 8-neighbour maxima of the filtered frame on interior pixels with value >
+// cutoff and dx < x <= ncols - dx, dx < y <= nrows - dx (x = column + 1,
+// y = row + 1, dx = S / 2); optionally the PSF-convolution mode's border
+// test; then for each maximum, in column-major scan order over the frames,
+// the S x S window of photons around it (columns reversed when mirrored).
+//
+// Kernels, all on the caller's stream:
+//   column_min_kernel  one workgroup per frame: the first column's minimum.
+//   detect_kernel      one workgroup per 64-row x 40-column tile: raw pixels
+//                      of the tile plus its filter and NMS halo -> photons in
+//                      LDS -> filtered values of the tile plus a one-pixel
+//                      ring in LDS -> maxima test; each wave writes one
+//                      64-bit ballot per tile column, so bit b of word
+//                      (frame, column, q) marks row 64q + b and the words in
+//                      index order list the maxima in scan order.  Nothing
+//                      else of the frame is written.
+//   count/scan/emit    order-preserving compaction of the mask words into
+//                      (x, y, frame) rows.
+//   cut_kernel         one wave per ROI (grid-stride over the device count):
+//                      the window re-read from the raw frame and converted,
+//                      the filtered value recomputed in the same order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "bsf_kernels.h"
+
+// photons and differences are rounded separately, as MATLAB rounds each
+// single-precision operation: no contraction into fused multiply-adds
+#pragma clang fp contract(off)
+
+namespace bsf {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTR = 64;           // tile rows: one mask word per column
+constexpr int kTC = 40;           // tile columns
+constexpr int kFR = kTR + 2;      // filtered rows incl. the NMS ring
+constexpr int kFC = kTC + 2;      // filtered columns incl. the ring
+constexpr int kFLD = kFR + 1;     // LDS leading dimension of the filtered tile
+constexpr int kR = 11;            // filtered rows per thread
+constexpr int kRG = kFR / kR;     // row groups (6)
+static_assert(kRG * kR == kFR, "row groups must tile the filtered rows");
+static_assert(kRG * kFC <= kThreads, "one filter thread per (row group, column)");
+constexpr int kMaxTemplR = 17;    // kernels with mr <= this get an unrolled filter
+
+constexpr int kMaxGridZ = 65535;          // frames per detect launch
+constexpr int kCW = 8;                    // mask words per thread in the compaction
+constexpr int kCB = kThreads * kCW;       // mask words per compaction block
+
+template <typename T>
+__device__ __forceinline__ float photon(T v, float offset, float conv)
+{
+    const float d = (float)v - offset;
+    return d * conv;
+}
+
+struct DetectArgs {
+    const void *raw;
+    int nr, nc;
+    float offset, conv;
+    const float *h;       // device [mr, mc]
+    int mr, mc;
+    const float *sub;     // per-frame value subtracted from the photons, or NULL
+    double cutoff;
+    int dx;
+    double psf_half;      // > 0: border test of the PSF-convolution mode
+    uint64_t *mask;
+    int wpc;              // mask words per column
+    int f0;               // first frame of this launch
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void column_min_kernel(const T *__restrict__ raw, int nr, int nc, float offset,
+                                                              float conv, float *__restrict__ sub)
+{
+    __shared__ float red[kThreads / 64];
+    const T *col = raw + (size_t)blockIdx.x * nr * nc;
+    float m = __builtin_nanf("");
+    for (int i = threadIdx.x; i < nr; i += kThreads) m = fminf(m, photon(col[i], offset, conv));
+    for (int o = 32; o; o >>= 1) m = fminf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = red[0];
+        for (int w = 1; w < kThreads / 64; w++) r = fminf(r, red[w]);
+        sub[blockIdx.x] = r;
+    }
+}
+
+template <typename T, int MR>
+__global__ __launch_bounds__(kThreads) void detect_kernel(DetectArgs A)
+{
+    extern __shared__ float lds[];
+    const int mr = MR > 0 ? MR : A.mr;
+    const int mc = A.mc;
+    const int ar = (mr - 1) / 2, ac = (mc - 1) / 2;
+    const int XR = kFR + mr - 1, XC = kFC + mc - 1;
+    const int LDX = XR | 1;                       // odd: lanes on neighbouring columns hit different banks
+    float *X = lds;                               // photons [XC][LDX]
+    float *Fl = lds + XC * LDX;                   // filtered [kFC][kFLD]
+
+    const int f = A.f0 + (int)blockIdx.z;
+    const int i0 = blockIdx.y * kTR, j0 = blockIdx.x * kTC;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const T *fr = (const T *)A.raw + (size_t)f * A.nr * A.nc;
+    const float sub = A.sub ? A.sub[f] : 0.0f;
+
+    // photons of the tile, the filter halo and the NMS ring; zero outside the frame
+    const int gr0 = i0 - 1 - ar, gc0 = j0 - 1 - ac;
+    for (int c = wave; c < XC; c += kThreads / 64) {
+        const int gc = gc0 + c;
+        const bool cin = gc >= 0 && gc < A.nc;
+        for (int r = lane; r < XR; r += 64) {
+            const int gr = gr0 + r;
+            float x = 0.0f;
+            if (cin && gr >= 0 && gr < A.nr) {
+                x = photon(fr[(size_t)gc * A.nr + gr], A.offset, A.conv);
+                if (A.sub) x = x - sub;
+            }
+            X[c * LDX + r] = x;
+        }
+    }
+    __syncthreads();
+
+    // filtered values of the tile and its ring: thread (row group g, column c)
+    // accumulates kR rows of one column
+    if (threadIdx.x < kRG * kFC) {
+        const int c = threadIdx.x % kFC, g = threadIdx.x / kFC;
+        float acc[kR];
+#pragma unroll
+        for (int r = 0; r < kR; r++) acc[r] = 0.0f;
+        for (int v = 0; v < mc; v++) {
+            const float *xc = X + (c + v) * LDX + g * kR;
+            const float *hv = A.h + v * mr;
+            if constexpr (MR > 0) {
+                // input row t feeds output row r through tap u = t - r
+#pragma unroll
+                for (int t = 0; t < kR + MR - 1; t++) {
+                    const float x = xc[t];
+#pragma unroll
+                    for (int r = 0; r < kR; r++) {
+                        const int u = t - r;
+                        if (u >= 0 && u < MR) acc[r] = fmaf(hv[u], x, acc[r]);
+                    }
+                }
+            } else {
+                for (int u = 0; u < mr; u++) {
+                    const float hu = hv[u];
+#pragma unroll
+                    for (int r = 0; r < kR; r++) acc[r] = fmaf(hu, xc[r + u], acc[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kR; r++) Fl[c * kFLD + g * kR + r] = acc[r];
+    }
+    __syncthreads();
+
+    // maxima: wave w tests tile columns w, w+4, ...; lane = row in the tile
+    const int gi = i0 + lane;
+    for (int cc = wave; cc < kTC; cc += kThreads / 64) {
+        const int gj = j0 + cc;
+        if (gj >= A.nc) break;                    // wave-uniform
+        const float *p = Fl + (cc + 1) * kFLD + lane + 1;
+        const float m = p[0];
+        bool keep = gi >= 1 && gi <= A.nr - 2 && gj >= 1 && gj <= A.nc - 2;
+        keep = keep && m > p[-kFLD - 1] && m > p[-kFLD] && m > p[-kFLD + 1] && m > p[-1] && m > p[1] &&
+               m > p[kFLD - 1] && m > p[kFLD] && m > p[kFLD + 1];
+        keep = keep && (double)m > A.cutoff;
+        const int x = gj + 1, y = gi + 1;
+        keep = keep && x > A.dx && x <= A.nc - A.dx && y > A.dx && y <= A.nr - A.dx;
+        if (A.psf_half > 0.0) {
+            const double hp = A.psf_half;
+            keep = keep && !((double)x < hp || (double)x > A.nr - hp || (double)y < hp || (double)y > A.nr - hp);
+        }
+        const uint64_t bits = __ballot(keep);
+        if (lane == 0) A.mask[((size_t)f * A.nc + gj) * A.wpc + blockIdx.y] = bits;
+    }
+}
+
+// exclusive prefix sum over the workgroup of one value per thread; *total
+// receives the sum
+__device__ __forceinline__ int block_exclusive_scan(int v, int *total)
+{
+    __shared__ int wsum[kThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int base = 0, all = 0;
+    for (int w = 0; w < kThreads / 64; w++) {
+        if (w < wave) base += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return base + inc - v;
+}
+
+__global__ __launch_bounds__(kThreads) void count_kernel(const uint64_t *__restrict__ mask, long W, int *__restrict__ bsum)
+{
+    const long w0 = (long)blockIdx.x * kCB + (long)threadIdx.x * kCW;
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < kCW; k++)
+        if (w0 + k < W) n += __popcll(mask[w0 + k]);
+    int total;
+    block_exclusive_scan(n, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// one workgroup: exclusive scan of the block sums in place; the grand total
+// goes to *count
+__global__ __launch_bounds__(kThreads) void scan_kernel(int *__restrict__ bsum, int nb, int *__restrict__ count)
+{
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kThreads) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < nb ? bsum[b] : 0;
+        int total;
+        const int ex = block_exclusive_scan(v, &total);
+        if (b < nb) bsum[b] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *count = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void emit_kernel(const uint64_t *__restrict__ mask, long W,
+                                                        const int *__restrict__ boff, int wpc, int nc, int first_frame,
+                                                        int cap, int *__restrict__ peaks)
+{
+    const long w0 = (long)blockIdx.x * kCB + (long)threadIdx.x * kCW;
+    uint64_t m[kCW];
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < kCW; k++) {
+        m[k] = w0 + k < W ? mask[w0 + k] : 0;
+        n += __popcll(m[k]);
+    }
+    int total;
+    int idx = boff[blockIdx.x] + block_exclusive_scan(n, &total);
+#pragma unroll
+    for (int k = 0; k < kCW; k++) {
+        uint64_t b = m[k];
+        if (!b) continue;
+        const long w = w0 + k;
+        const long fc = w / wpc;
+        const int q = (int)(w - fc * wpc);
+        const int f = (int)(fc / nc), j = (int)(fc - (long)f * nc);
+        while (b) {
+            const int bit = __builtin_ctzll(b);
+            b &= b - 1;
+            if (idx < cap) {
+                peaks[idx] = j + 1;
+                peaks[cap + idx] = q * 64 + bit + 1;
+                peaks[2 * cap + idx] = first_frame + f;
+            }
+            idx++;
+        }
+    }
+}
+
+struct CutArgs {
+    const void *raw;
+    int nr, nc;
+    float offset, conv;
+    const int *peaks;
+    const int *count;
+    int cap, first_frame, S, dx, mirror;
+    float *stack;
+    float *value;         // filtered value at each maximum, or NULL
+    const float *h;
+    int mr, mc;
+    const float *sub;
+};
+
+template <typename T>
+__global__ __launch_bounds__(64) void cut_kernel(CutArgs A)
+{
+    const int n = min(*A.count, A.cap);
+    const int SS = A.S * A.S;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int x = A.peaks[k] - 1, y = A.peaks[A.cap + k] - 1, f = A.peaks[2 * A.cap + k] - A.first_frame;
+        const T *fr = (const T *)A.raw + (size_t)f * A.nr * A.nc;
+        float *dst = A.stack + (size_t)k * SS;
+        for (int e = threadIdx.x; e < SS; e += 64) {
+            const int c = e / A.S, r = e - c * A.S;
+            const int gc = A.mirror ? x + A.dx - c : x - A.dx + c;
+            dst[e] = photon(fr[(size_t)gc * A.nr + (y - A.dx + r)], A.offset, A.conv);
+        }
+        if (A.value && threadIdx.x == 0) {
+            // the detect kernel's sum for this pixel, term for term
+            const int ar = (A.mr - 1) / 2, ac = (A.mc - 1) / 2;
+            const float sub = A.sub ? A.sub[f] : 0.0f;
+            float acc = 0.0f;
+            for (int v = 0; v < A.mc; v++) {
+                const int gc = x - ac + v;
+                for (int u = 0; u < A.mr; u++) {
+                    const int gr = y - ar + u;
+                    float xv = 0.0f;
+                    if (gc >= 0 && gc < A.nc && gr >= 0 && gr < A.nr) {
+                        xv = photon(fr[(size_t)gc * A.nr + gr], A.offset, A.conv);
+                        if (A.sub) xv = xv - sub;
+                    }
+                    acc = fmaf(A.h[v * A.mr + u], xv, acc);
+                }
+            }
+            A.value[k] = acc;
+        }
+    }
+}
+
+__global__ void zero_count_kernel(int *count) { *count = 0; }
+
+struct WorkLayout {
+    size_t sub, mask, bsum, total;
+    long W;
+    int nb, wpc;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+WorkLayout work_layout(int nr, int nc, int nframes)
+{
+    WorkLayout w;
+    w.wpc = (nr + kTR - 1) / kTR;
+    w.W = (long)nframes * nc * w.wpc;
+    w.nb = (int)((w.W + kCB - 1) / kCB);
+    w.sub = 0;
+    w.mask = align256(sizeof(float) * (size_t)nframes);
+    w.bsum = w.mask + align256(sizeof(uint64_t) * (size_t)w.W);
+    w.total = w.bsum + align256(sizeof(int) * (size_t)(w.nb > 0 ? w.nb : 1));
+    return w;
+}
+
+using DetectFn = void (*)(DetectArgs);
+
+template <typename T, size_t... I>
+DetectFn detect_fn(int mr, std::index_sequence<I...>)
+{
+    // mr == 0 selects the generic kernel; mr in 1..kMaxTemplR the unrolled ones
+    static const DetectFn table[] = {detect_kernel<T, (int)I>...};
+    return mr <= kMaxTemplR ? table[mr] : table[0];
+}
+
+template <typename T>
+hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
+{
+    const WorkLayout w = work_layout(L.nr, L.nc, L.nframes);
+    char *work = (char *)L.work;
+    float *sub = L.subtract_min ? (float *)(work + w.sub) : nullptr;
+    uint64_t *mask = (uint64_t *)(work + w.mask);
+    int *bsum = (int *)(work + w.bsum);
+    const T *raw = (const T *)L.frames;
+
+    if (sub) hipLaunchKernelGGL(column_min_kernel<T>, dim3(L.nframes), dim3(kThreads), 0, st, raw, L.nr, L.nc,
+                                L.offset, L.conv, sub);
+
+    DetectArgs A;
+    A.raw = L.frames;
+    A.nr = L.nr;
+    A.nc = L.nc;
+    A.offset = L.offset;
+    A.conv = L.conv;
+    A.h = L.h;
+    A.mr = L.mr;
+    A.mc = L.mc;
+    A.sub = sub;
+    A.cutoff = L.cutoff;
+    A.dx = L.S / 2;
+    A.psf_half = L.psf_size > 0.0 ? L.psf_size / 2.0 : 0.0;
+    A.mask = mask;
+    A.wpc = w.wpc;
+    const size_t lds = front_detect_lds_bytes(L.mr, L.mc);
+    const DetectFn fn = detect_fn<T>(L.mr, std::make_index_sequence<kMaxTemplR + 1>{});
+    for (int f0 = 0; f0 < L.nframes; f0 += kMaxGridZ) {
+        A.f0 = f0;
+        const int nf = L.nframes - f0 < kMaxGridZ ? L.nframes - f0 : kMaxGridZ;
+        const dim3 grid((unsigned)((L.nc + kTC - 1) / kTC), (unsigned)w.wpc, (unsigned)nf);
+        void *args[] = {&A};
+        const hipError_t e = hipLaunchKernel((const void *)fn, grid, dim3(kThreads), args, lds, st);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(count_kernel, dim3((unsigned)w.nb), dim3(kThreads), 0, st, mask, w.W, bsum);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kThreads), 0, st, bsum, w.nb, L.count);
+    hipLaunchKernelGGL(emit_kernel, dim3((unsigned)w.nb), dim3(kThreads), 0, st, mask, w.W, bsum, w.wpc, L.nc,
+                       L.first_frame, L.cap, L.peaks);
+    if (L.cap > 0) {
+        CutArgs C;
+        C.raw = L.frames;
+        C.nr = L.nr;
+        C.nc = L.nc;
+        C.offset = L.offset;
+        C.conv = L.conv;
+        C.peaks = L.peaks;
+        C.count = L.count;
+        C.cap = L.cap;
+        C.first_frame = L.first_frame;
+        C.S = L.S;
+        C.dx = L.S / 2;
+        C.mirror = L.mirror;
+        C.stack = L.stack;
+        C.value = L.value;
+        C.h = L.h;
+        C.mr = L.mr;
+        C.mc = L.mc;
+        C.sub = sub;
+        const int blocks = L.cap < 8192 ? L.cap : 8192;
+        hipLaunchKernelGGL(cut_kernel<T>, dim3((unsigned)blocks), dim3(64), 0, st, C);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t front_detect_lds_bytes(int mr, int mc)
+{
+    const int XR = kFR + mr - 1, XC = kFC + mc - 1;
+    return sizeof(float) * ((size_t)XC * (XR | 1) + (size_t)kFC * kFLD);
+}
+
+size_t front_work_bytes(int nr, int nc, int nframes) { return work_layout(nr, nc, nframes).total; }
+
+hipError_t launch_find_rois(const FrontLaunch &L, hipStream_t st)
+{
+    if (L.nframes <= 0) {
+        hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, L.count);
+        return hipGetLastError();
+    }
+    return L.pixel_type == 0 ? launch_typed<uint16_t>(L, st) : launch_typed<float>(L, st);
+}
+
+}  // namespace bsf

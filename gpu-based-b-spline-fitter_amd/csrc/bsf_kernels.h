@@ -66,6 +66,33 @@ hipError_t coeff_build(const float *samples, const int *dims, int ndim, const do
 size_t coeff_work_bytes(const int *dims, int ndim);
 
 // Error text of the calling thread (bsf_last_error): set it and return code.
+// Frame -> ROI front-end.  frames [nr, nc, nframes] column-major (device;
+// pixel_type 0 = uint16, 1 = float32); h [mr, mc] the correlation kernel
+// (device); outputs on the device: stack [S, S, cap], peaks [cap, 3] (x, y,
+// frame), value [cap] or NULL, count (one int: the ROIs found, possibly more
+// than cap); work is device scratch of front_work_bytes(nr, nc, nframes).
+struct FrontLaunch {
+    const void *frames;
+    int pixel_type;
+    int nr, nc, nframes, first_frame;
+    float offset, conv;
+    const float *h;
+    int mr, mc;
+    int subtract_min;
+    double cutoff;
+    int S, mirror;
+    double psf_size;
+    int cap;
+    float *stack;
+    int *peaks;
+    float *value;
+    int *count;
+    void *work;
+};
+hipError_t launch_find_rois(const FrontLaunch &L, hipStream_t st);
+size_t front_work_bytes(int nr, int nc, int nframes);
+size_t front_detect_lds_bytes(int mr, int mc);
+
 int fail(int code, const char *fmt, ...);
 void clear_error();
 

@@ -310,6 +310,84 @@ int bsf_poll(bsf_session *s, unsigned long long ticket, int wait);
 /* Occupancy of the session: frames in flight and arena bytes they hold. */
 int bsf_session_inflight(bsf_session *s, int *frames, size_t *arena_used);
 
+/* ------------------------------------------------ frames -> ROI stack
+ * The GUI fitter's per-frame front-end (MultiD_fitter/simplefitter_spline.m:
+ * 197-307 with MultiD_fitter/shared/maximumfindcall.m:4-21 and
+ * maximumfind2.c:18-65), which cuts the ROIs that GPUmleFit_LM_multiD fits:
+ *   imphot = (single(frame) - offset) * conversion          (:259)
+ *   impf   = filter2(h, imphot)                  Gaussian mode (:275-276)
+ *          = filter2(h, imphot - min(imphot(:,1)))  DoG / PSF modes (:266-273)
+ *   maxima = pixels of impf above all 8 neighbours, interior pixels, in
+ *            column-major scan order, x = column + 1, y = row + 1
+ *   kept   where impf > cutoff, dx < x <= ncols - dx, dx < y <= nrows - dx
+ *            (:277-280; dx = floor(roi/2)), and with psf_size > 0 where
+ *            psf_size/2 <= x, y <= nrows - psf_size/2 (:284-295)
+ *   ROI k  = imphot(y-dx:y+dx, x-dx:x+dx), columns reversed if mirror
+ *            (:296-307); peaks(k, :) = [x, y, frame] as peakcoordinates.
+ * Frames are [nrows, ncols, nframes] column-major; frame f is numbered
+ * first_frame + f (the GUI counts from 1).  The filter2 sum is accumulated
+ * in FP32 with fused multiply-adds, kernel column by column (v outer, row u
+ * inner). */
+#define BSF_MAX_PEAK_FILTER 51
+
+enum bsf_pixel_type { BSF_PIXEL_U16 = 0, BSF_PIXEL_F32 = 1 };
+
+typedef struct bsf_roi_request {
+    const void *frames;     /* host (bsf_find_rois_host) or device pointer */
+    int pixel_type;         /* enum bsf_pixel_type */
+    int nrows, ncols, nframes;
+    int first_frame;
+    float offset;           /* p.offset, ADU */
+    float conversion;       /* p.conversion, photons per ADU */
+    const float *filter;    /* [filter_rows, filter_cols] column-major; host or device as frames */
+    int filter_rows, filter_cols;
+    int subtract_min;       /* filter imphot - min(imphot(:,1)) (DoG and PSF modes) */
+    double cutoff;          /* p.peakcutoff */
+    int roi;                /* p.roifit, odd, 3..51 */
+    int mirror;             /* p.mirror */
+    double psf_size;        /* PSF-convolution mode: size(cal.PSF, 1); 0 = off */
+    int capacity;           /* ROIs the outputs hold */
+} bsf_roi_request;
+
+typedef struct bsf_roi_result {
+    float *imstack;         /* [roi, roi, capacity] photons */
+    int *peaks;             /* [capacity, 3] column-major: x, y, frame number */
+    float *value;           /* [capacity] filtered value at the maximum, or NULL */
+    int count;              /* bsf_find_rois_host: ROIs found; when above capacity
+                               only the first capacity are written */
+} bsf_roi_result;
+
+/* Host frames in, host ROI stack out (copies to and from the device). */
+int bsf_find_rois_host(const bsf_roi_request *req, bsf_roi_result *res);
+
+/* Device frames, filter and outputs; the number of ROIs found goes to the
+ * device int *count; work is device scratch of bsf_find_rois_work_bytes
+ * bytes.  Queued on stream; allocates nothing. */
+size_t bsf_find_rois_work_bytes(const bsf_roi_request *req);
+int bsf_find_rois_device(const bsf_roi_request *req, bsf_roi_result *res, int *count, void *work, size_t work_bytes,
+                         void *stream);
+
+/* The GUI's peak-finding kernels (simplefitter_spline.m:195-212), computed
+ * in FP64 and rounded to FP32:
+ *   BSF_PEAK_GAUSS  fspecial('gaussian', max(3, ceil(3 sigma + 1)), sigma)
+ *   BSF_PEAK_DOG    fspecial('gaussian', n, sigma) - fspecial('gaussian', n,
+ *                   max(1, 2.5 sigma)), n = max(ceil(6 sigma + 1), 3)
+ * *size receives n; h must hold n * n floats (cap). */
+enum bsf_peak_filter_kind {
+    BSF_PEAK_GAUSS = 0, BSF_PEAK_DOG = 1, BSF_PEAK_PSF_WINDOW = 2, BSF_PEAK_PSF_MEAN = 3
+};
+int bsf_peak_filter(int kind, double sigma, float *h, int cap, int *size);
+
+/* Kernels from the calibration PSF [n, n, nz] (column-major):
+ *   BSF_PEAK_PSF_WINDOW  p.conv (:199-207): the z-mean of rows/columns
+ *                        floor((n-roi)/2)..floor((n+roi)/2) (1-based), minus
+ *                        its mean, scaled to 0.05 at its maximum
+ *   BSF_PEAK_PSF_MEAN    'Conv PSF' background mode (:262-268): the z-mean of
+ *                        5 PSF(:,:,z) minus its column means (one filter with
+ *                        the mean kernel instead of the mean of nz filters). */
+int bsf_peak_filter_psf(int kind, const float *psf, const int *psf_dims, int roi, float *h, int cap, int *rows,
+                        int *cols);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,7 +7,9 @@ leg may import this package, and only as the checker: the shipped library in
 ``liborc.so`` (built by ``oracle/Makefile`` from ``bsf_oracle.c``) holds two
 restatements of the reference's CPU fitter
 (``MultiD_bspline/visual code/CPUmleFit_LM/CPUmleFit_LM.cpp:2411-3620``):
-the float "parity" build (``fit``) and the all-double build (``fit_f64``).
+the float "parity" build (``fit``) and the all-double build (``fit_f64``),
+and the frame -> ROI front-end of the GUI fitter (``find_rois`` and its
+steps ``photons``, ``filter2``, ``find_maxima``; ``bsf_frontend.c``).
 
 ``_ref/`` (built by ``make ref`` where the reference tree exists) holds the
 reference's own ``CPUsplineLib.cpp`` and ``MatInvLib.h`` compiled from source;
@@ -20,4 +22,5 @@ from .loader import (  # noqa: F401
     cholesky, lu_eval, matinv, com2d, maxmin2d, eval_pass, fit_4pi, nv_4pi,
     ref_available, ref_bspline, ref_delta_tables, ref_deriv_eval,
     ref_cholesky, ref_lu_eval, ref_matinv, NV_OF, D_OF,
+    photons, filter2, find_maxima, find_rois,
 )

@@ -337,3 +337,83 @@ def fit_4pi(imstack, shared, iterations, coeff, phaseShift, dT=None, initZ=None,
     if rc != 0:
         raise ValueError("oracle: bad 4Pi arguments")
     return P, CR, LL
+
+
+# ------------------------------------------------------- frames -> ROI stack
+def _frontend():
+    lib = _L()
+    if not getattr(lib, "_fe_bound", False):
+        lib.orc_photons.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_float, C.c_float, C.c_void_p]
+        lib.orc_photons.restype = None
+        lib.orc_filter2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        lib.orc_filter2.restype = None
+        lib.orc_find_maxima.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
+        lib.orc_find_maxima.restype = C.c_long
+        lib.orc_find_rois.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                      C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                      C.c_int, C.c_double, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_find_rois.restype = C.c_long
+        lib._fe_bound = True
+    return lib
+
+
+def _pixels(frames):
+    if frames.dtype == np.uint16:
+        return 0
+    if frames.dtype == np.float32:
+        return 1
+    raise ValueError("frames must be uint16 or float32")
+
+
+def photons(raw, offset, conversion):
+    """(single(raw) - offset) * conversion, elementwise (bsf_frontend.c)."""
+    raw = np.asfortranarray(raw)
+    out = np.empty(raw.shape, np.float32, order="F")
+    _frontend().orc_photons(raw.ctypes.data, _pixels(raw), raw.size, float(offset), float(conversion),
+                            out.ctypes.data)
+    return out
+
+
+def filter2(h, x):
+    """filter2(h, x, 'same') in the front-end's FP32 fmaf order; x, h 2-D."""
+    x = np.asfortranarray(x, np.float32)
+    h = np.asfortranarray(h, np.float32)
+    out = np.empty(x.shape, np.float32, order="F")
+    _frontend().orc_filter2(x.ctypes.data, x.shape[0], x.shape[1], h.ctypes.data, h.shape[0], h.shape[1],
+                            out.ctypes.data)
+    return out
+
+
+def find_maxima(F, cutoff, dx, psf_size=0.0):
+    """maximumfind2's maxima of F plus the GUI's filters: (n, 3) [x, y, value]."""
+    F = np.asfortranarray(F, np.float32)
+    cap = F.size // 4 + 8
+    xs = np.zeros(cap, np.int32)
+    ys = np.zeros(cap, np.int32)
+    vs = np.zeros(cap, np.float32)
+    n = _frontend().orc_find_maxima(F.ctypes.data, F.shape[0], F.shape[1], float(cutoff), int(dx), float(psf_size),
+                                    xs.ctypes.data, ys.ctypes.data, vs.ctypes.data, cap)
+    return np.stack([xs[:n], ys[:n], vs[:n].astype(np.float64)], 1)
+
+
+def find_rois(frames, roi, filt, cutoff, offset=0.0, conversion=1.0, subtract_min=False, mirror=False,
+              psf_size=0.0, first_frame=1, capacity=None):
+    """The whole front-end (bsf_frontend.c: orc_find_rois); same arguments and
+    returns as bsfit.frames.find_rois."""
+    frames = np.asfortranarray(frames)
+    if frames.ndim == 2:
+        frames = frames[:, :, None]
+    h = np.asfortranarray(filt, np.float32)
+    nr, nc, nf = frames.shape
+    if capacity is None:
+        capacity = max(1, nf * ((nr + 1) // 2) * ((nc + 1) // 2) // 16)
+    stack = np.zeros((roi, roi, capacity), np.float32, order="F")
+    peaks = np.zeros((capacity, 3), np.int32, order="F")
+    value = np.zeros(capacity, np.float32)
+    n = _frontend().orc_find_rois(frames.ctypes.data, _pixels(frames), nr, nc, nf, int(first_frame), float(offset),
+                                  float(conversion), h.ctypes.data, h.shape[0], h.shape[1], int(bool(subtract_min)),
+                                  float(cutoff), int(roi), int(bool(mirror)), float(psf_size), capacity,
+                                  stack.ctypes.data, peaks.ctypes.data, value.ctypes.data)
+    k = min(n, capacity)
+    return stack[:, :, :k], np.asfortranarray(peaks[:k]), value[:k], n

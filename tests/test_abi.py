@@ -98,3 +98,29 @@ def test_multistart_argument_errors():
         with pytest.raises(bsfit.BsfError) as e:
             bsfit.fit_multistart(7, w.imstack, 50, w.coeff, np.ones((2, 2), np.float32))
         assert e.value.code == 6
+
+
+def test_front_end_argument_errors_and_host_kernels():
+    import torch
+    import bsfit
+    from bsfit import frames as FR
+    h = FR.peak_filter("dog", 1.0)                                 # host code: no device needed
+    assert h.shape == (7, 7)
+    fr = np.zeros((32, 32, 1), np.uint16)
+    for kw, code in ((dict(roi=12), 3), (dict(roi=53), 3), (dict(roi=7, filt=np.zeros((52, 3), np.float32)), 4),
+                     (dict(roi=7, capacity=-1), 1)):
+        args = dict(filt=h, cutoff=1.0, capacity=4)
+        args.update(kw)
+        roi = args.pop("roi")
+        with pytest.raises(bsfit.BsfError) as e:
+            FR.find_rois(fr, roi, **args)
+        assert e.value.code == code, kw
+    with pytest.raises(bsfit.BsfError) as e:
+        FR.find_rois(np.zeros((2, 40, 1), np.uint16), 7, h, 1.0, capacity=4)   # frames below 3 x 3
+    assert e.value.code == 4
+    with pytest.raises(bsfit.BsfError):
+        FR.find_rois(np.zeros((8, 8, 1), np.int32), 7, h, 1.0)                # pixel type
+    if not torch.cuda.is_available():
+        with pytest.raises(bsfit.BsfError) as e:
+            FR.find_rois(fr, 7, h, 1.0, capacity=4)
+        assert e.value.code == 6

@@ -10,6 +10,7 @@ import pytest
 import bsfit
 from bsfit import gen
 from oracle import bsarray as B
+from parity_util import Hip as _Hip
 
 pytestmark = pytest.mark.gpu
 
@@ -47,37 +48,6 @@ def test_builder_on_the_fit_models():
     xc = gen.psf_color4d().astype(np.float32)
     lam = [0.0, 0.0, 0.1, 0.2]
     _close(bsfit.coeff_build(xc, lam), B.bsarray(xc.astype(np.float64), lam))
-
-
-class _Hip:
-    """Device buffers through the HIP runtime libbsfit.so links (not torch's
-    bundled one: two runtimes in a process only get along when torch's
-    initialises first)."""
-
-    def __init__(self):
-        bsfit.load()
-        self.h = C.CDLL("libamdhip64.so.7")
-        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize"):
-            getattr(self.h, f).restype = C.c_int
-        self.h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.h.hipFree.argtypes = [C.c_void_p]
-        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-
-    def alloc(self, n):
-        p = C.c_void_p()
-        assert self.h.hipMalloc(C.byref(p), n) == 0
-        return p.value
-
-    def put(self, a):
-        a = np.ascontiguousarray(a)
-        p = self.alloc(a.nbytes)
-        assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0   # host to device
-        return p
-
-    def get(self, p, n, dtype):
-        out = np.empty(n, dtype)
-        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0   # device to host
-        return out
 
 
 def test_device_entry_point_matches_host():

@@ -1,0 +1,152 @@
+"""The GPU front-end (bsf_find_rois_*) against the oracle restatement of the
+GUI fitter's frame -> ROI steps (oracle/bsf_frontend.c): the same FP32
+operations in the same order, so the ROI count, the peak rows
+(x, y, frame), the photon ROIs and the filtered values must be identical bit
+for bit.  Covers both pixel types, Gaussian / DoG / PSF / random kernels
+(odd, even, rectangular; the unrolled sizes up to 17 rows and the generic
+kernel beyond), frame sizes that are not multiples of the 64 x 40 tile, the
+DoG mode's first-column minimum, mirroring, the PSF-mode border test,
+capacity truncation, empty results and a 2048 x 2048 frame; then frames ->
+ROIs -> fits equals fitting the oracle's ROIs."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import bsfit
+import oracle
+from bsfit import frames as FR
+from bsfit import gen
+from parity_util import Hip
+
+pytestmark = pytest.mark.gpu
+
+
+def _kernel(kind, rng):
+    if kind[0] in ("dog", "gauss"):
+        return FR.peak_filter(kind[0], kind[1])
+    if kind[0] == "psf":
+        psf = np.transpose(gen.psf_astig3d(), (0, 1, 2)).astype(np.float32)
+        return FR.peak_filter_psf("window", psf, kind[1])
+    return np.asfortranarray(rng.uniform(-0.05, 0.2, kind[1]).astype(np.float32))
+
+
+CASES = [
+    # (nrows, ncols, nframes, dtype, kernel, subtract_min, mirror, psf_size, roi, cutoff)
+    (256, 200, 4, np.uint16, ("dog", 1.0), True, False, 0.0, 13, 15.0),
+    (130, 97, 3, np.float32, ("gauss", 1.0), False, True, 0.0, 11, 40.0),
+    (64, 64, 2, np.uint16, ("rand", (5, 8)), True, False, 27.0, 9, 20.0),
+    (100, 301, 2, np.uint16, ("rand", (19, 19)), False, False, 0.0, 15, 60.0),
+    (67, 45, 1, np.float32, ("rand", (1, 1)), False, False, 0.0, 7, 30.0),
+    (300, 70, 1, np.uint16, ("rand", (31, 25)), True, True, 0.0, 13, 100.0),
+    (200, 180, 2, np.uint16, ("psf", 13), True, False, 27.0, 13, 0.5),
+    (129, 81, 2, np.float32, ("dog", 2.6), True, False, 0.0, 17, 5.0),
+    (70, 66, 1, np.uint16, ("rand", (51, 3)), False, False, 0.0, 3, 10.0),
+]
+
+
+def _compare(got, want):
+    st, pk, val, n = got
+    st_o, pk_o, val_o, n_o = want
+    assert n == n_o
+    np.testing.assert_array_equal(pk, pk_o)
+    np.testing.assert_array_equal(st, st_o)
+    np.testing.assert_array_equal(val, val_o)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_find_rois_matches_oracle(case):
+    nr, nc, nf, dt, kind, sub, mirror, psf, roi, cutoff = CASES[case]
+    rng = np.random.default_rng(100 + case)
+    frames, _ = gen.make_frames(nf, nr, nc, max(2, nr * nc // 2500), 500 + case, dtype=dt)
+    h = _kernel(kind, rng)
+    args = dict(cutoff=cutoff, offset=100.0, conversion=0.5, subtract_min=sub, mirror=mirror, psf_size=psf,
+                first_frame=1, capacity=20000)
+    got = FR.find_rois(frames, roi, h, **args)
+    want = oracle.find_rois(frames, roi, h, **args)
+    print("case %d: %dx%dx%d %s kernel %s: %d ROIs" % (case, nr, nc, nf, np.dtype(dt).name, h.shape, got[3]),
+          flush=True)
+    assert got[3] > 0
+    _compare(got, want)
+
+
+def test_capacity_truncation_and_empty_frames():
+    frames, _ = gen.make_frames(3, 96, 96, 8, 77)
+    h = FR.peak_filter("dog", 1.0)
+    args = dict(cutoff=10.0, offset=100.0, conversion=0.5, subtract_min=True)
+    full = FR.find_rois(frames, 11, h, capacity=5000, **args)
+    part = FR.find_rois(frames, 11, h, capacity=7, **args)
+    assert part[3] == full[3] > 7
+    np.testing.assert_array_equal(part[1], full[1][:7])
+    np.testing.assert_array_equal(part[0], full[0][:, :, :7])
+    _compare(part, oracle.find_rois(frames, 11, h, capacity=7, **args))
+    flat = np.full((80, 90, 2), 300, np.uint16)               # no strict maximum anywhere
+    got = FR.find_rois(flat, 11, h, capacity=10, **args)
+    assert got[3] == 0 and got[1].shape == (0, 3)
+    none = FR.find_rois(np.zeros((80, 90, 0), np.uint16), 11, h, capacity=10, **args)
+    assert none[3] == 0
+
+
+def test_large_frame():
+    frames, truth = gen.make_frames(1, 2048, 2048, 600, 9)
+    h = FR.peak_filter("dog", 1.0)
+    args = dict(cutoff=15.0, offset=100.0, conversion=0.5, subtract_min=True, capacity=20000)
+    got = FR.find_rois(frames, 13, h, **args)
+    _compare(got, oracle.find_rois(frames, 13, h, **args))
+    assert got[3] >= 0.8 * len(truth)
+
+
+def test_device_entry_point_matches_host():
+    lib = FR._bind()
+    frames, _ = gen.make_frames(3, 150, 130, 10, 31)
+    h = FR.peak_filter("dog", 1.2)
+    cap, roi = 400, 13
+    host = FR.find_rois(frames, roi, h, cutoff=12.0, offset=100.0, conversion=0.5, subtract_min=True, capacity=cap)
+    hip = Hip()
+    d_fr, d_h = hip.put(frames), hip.put(h)
+    q = FR.make_request(d_fr, FR.PIXEL_U16, frames.shape, roi, d_h, h.shape, 12.0, 100.0, 0.5, True, False, 0.0, 1,
+                        cap)
+    wb = lib.bsf_find_rois_work_bytes(C.byref(q))
+    d_w, d_st, d_pk, d_v, d_n = (hip.alloc(wb), hip.alloc(roi * roi * cap * 4), hip.alloc(cap * 12),
+                                 hip.alloc(cap * 4), hip.alloc(4))
+    try:
+        res = FR.RoiResult(d_st, d_pk, d_v, 0)
+        rc = lib.bsf_find_rois_device(C.byref(q), C.byref(res), d_n, d_w, wb, None)
+        assert rc == 0, lib.bsf_last_error()
+        assert hip.h.hipDeviceSynchronize() == 0
+        n = int(hip.get(d_n, 1, np.int32)[0])
+        pk = hip.get(d_pk, cap * 3, np.int32).reshape(cap, 3, order="F")[:n]
+        st = hip.get(d_st, roi * roi * cap, np.float32).reshape(roi, roi, cap, order="F")[:, :, :n]
+        v = hip.get(d_v, cap, np.float32)[:n]
+        # a work buffer that is too small is refused before any launch
+        rc = lib.bsf_find_rois_device(C.byref(q), C.byref(res), d_n, d_w, wb - 8, None)
+        assert rc == 1
+    finally:
+        hip.free(d_fr, d_h, d_w, d_st, d_pk, d_v, d_n)
+    assert n == host[3]
+    np.testing.assert_array_equal(pk, host[1])
+    np.testing.assert_array_equal(st, host[0])
+    np.testing.assert_array_equal(v, host[2])
+
+
+def test_frames_to_fits():
+    """The GUI's path on raw frames: ROIs from the GPU front-end fitted with
+    fittype 6 give the same results as the oracle's ROIs fitted the same way
+    (the ROIs are identical), and the fits land on the planted emitters."""
+    coeff = gen.model_coeffs("astig3d")
+    frames, truth = gen.make_frames(4, 192, 192, 20, 55, coeff=coeff)
+    h = FR.peak_filter("dog", 1.0)
+    args = dict(cutoff=15.0, offset=100.0, conversion=0.5, subtract_min=True, capacity=2000)
+    st, pk, _, n = FR.find_rois(frames, 13, h, **args)
+    st_o = oracle.find_rois(frames, 13, h, **args)[0]
+    P, CR, LL = bsfit.fit(6, st, 50, coeff)
+    Po, CRo, LLo = bsfit.fit(6, st_o, 50, coeff)
+    np.testing.assert_array_equal(P, Po)
+    # localisations in frame pixels: ROI corner (x - 1 - dx, y - 1 - dx) plus the fitted column/row position
+    xs = pk[:, 0] - 1 - 6 + P[:, 0]
+    ys = pk[:, 1] - 1 - 6 + P[:, 1]
+    hit = 0
+    for f, x, y in truth[:, :3]:
+        m = pk[:, 2] == f + 1
+        hit += bool(m.any() and np.hypot(xs[m] - x, ys[m] - y).min() < 0.3)
+    assert hit >= 0.7 * len(truth), (hit, len(truth))
