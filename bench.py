@@ -288,6 +288,112 @@ def coeff_bench(args):
         "cpu_baseline": cpu}))
 
 
+def frames_bench(args):
+    """Frame -> ROI front-end on device-resident camera frames: 32 frames of
+    1024x1024 uint16 ADU, 400 emitters each (3D astigmatic model), the GUI's
+    DoG mode (sigma 1, 7x7 kernel, first-column minimum subtracted), 13x13
+    ROIs; one bsf_find_rois_device call over all frames per step.  Then the
+    GUI's whole path on those frames: ROIs found, their count read back, and
+    the ROIs fitted with fittype 6 (50 iterations) on the device.
+    Roofline: the detect kernel's FP32 filter arithmetic, 2 mr mc flop per
+    pixel (the fused NMS, the halo recomputation and the compaction are not
+    counted).  CPU baseline: the oracle's front-end (oracle/bsf_frontend.c)
+    on a sample of the frames, one thread."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import bsfit
+    from bsfit import frames as FR
+    from bsfit import gen
+    lib = FR._bind()
+    nf, nr, nc, per, roi, cutoff = args.nframes, 1024, 1024, 400, 13, 15.0
+    t0 = time.perf_counter()
+    coeff = gen.model_coeffs("astig3d")
+    frames, truth = gen.make_frames(nf, nr, nc, per, 20261203 + 5000, coeff=coeff)
+    print("bench: generated %d frames %dx%d in %.1f s" % (nf, nr, nc, time.perf_counter() - t0), file=sys.stderr,
+          flush=True)
+    h = FR.peak_filter("dog", 1.0)
+    dev = torch.device("cuda:0")
+    d_fr = torch.from_numpy(np.ascontiguousarray(frames.ravel(order="F"))).to(dev)
+    d_h = torch.from_numpy(np.ascontiguousarray(h.ravel(order="F"))).to(dev)
+    cap = nf * 2 * per
+    q = FR.make_request(d_fr.data_ptr(), FR.PIXEL_U16, frames.shape, roi, d_h.data_ptr(), h.shape, cutoff, 100.0,
+                        0.5, True, False, 0.0, 1, cap)
+    wb = lib.bsf_find_rois_work_bytes(C.byref(q))
+    d_w = torch.empty(wb, dtype=torch.uint8, device=dev)
+    d_st = torch.empty(roi * roi * cap, dtype=torch.float32, device=dev)
+    d_pk = torch.empty(3 * cap, dtype=torch.int32, device=dev)
+    d_n = torch.zeros(1, dtype=torch.int32, device=dev)
+    res = FR.RoiResult(d_st.data_ptr(), d_pk.data_ptr(), None, 0)
+    st = torch.cuda.current_stream()
+
+    def detect():
+        rc = lib.bsf_find_rois_device(C.byref(q), C.byref(res), d_n.data_ptr(), d_w.data_ptr(), wb, st.cuda_stream)
+        if rc:
+            raise bsfit.BsfError(rc, lib.bsf_last_error().decode())
+
+    for _ in range(args.warmup):
+        detect()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record(st)
+    for _ in range(args.steps):
+        detect()
+    b.record(st)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    ms = a.elapsed_time(b) / args.steps
+    n = int(d_n.item())
+    # the GUI's whole path: frames -> ROIs -> count to the host -> fits
+    NV = 5
+    d_P = torch.empty((NV + 1) * n, dtype=torch.float32, device=dev)
+    d_CR = torch.empty(NV * n, dtype=torch.float32, device=dev)
+    d_LL = torch.empty(n, dtype=torch.float32, device=dev)
+    d_c = torch.from_numpy(np.ascontiguousarray(coeff.ravel(order="F"))).to(dev)
+
+    def end_to_end():
+        detect()
+        m = int(d_n.item())
+        bsfit.fit_device(6, d_st.data_ptr(), roi, min(m, cap), 50, d_c.data_ptr(), coeff.shape, d_P.data_ptr(),
+                         d_CR.data_ptr(), d_LL.data_ptr(), stream=st.cuda_stream)
+
+    end_to_end()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        end_to_end()
+    torch.cuda.synchronize()
+    e2e = (time.perf_counter() - t0) / args.steps
+    flops = 2.0 * h.size * nr * nc * nf
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        k = max(1, min(nf, args.cpu_frames))
+        t1 = time.perf_counter()
+        co = oracle.find_rois(frames[:, :, :k], roi, h, cutoff=cutoff, offset=100.0, conversion=0.5,
+                              subtract_min=True, capacity=k * 2 * per)
+        tc = time.perf_counter() - t1
+        cpu = {"value": k / tc, "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": "%d of the %d frames (oracle/bsf_frontend.c, %d ROIs), %.1f s" % (k, nf, co[3], tc)}
+    print("bench: front-end %.3f ms for %d frames (%d ROIs); frames -> fits %.3f ms" % (ms, nf, n, e2e * 1e3),
+          file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "camera frames/sec through the frame -> ROI front-end (1024x1024 uint16)", "value": nf / (ms * 1e-3),
+        "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded spline-rendered emitters, Poisson noise, ADU = photons/0.5 + 100)",
+        "config": {"workload": "%d frames of %dx%d uint16, %d emitters per frame, DoG sigma 1 (7x7), 13x13 ROIs"
+                               % (nf, nr, nc, per), "rois_found": n,
+                   "frames_to_fits_ms": e2e * 1e3, "frames_to_fits_per_s": n / e2e},
+        "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": FP32_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / FP32_PEAK, "traffic": None,
+                     "note": "FP32 vector-ALU bound (filter2: 2 x 49 flop per pixel against 2 B of raw pixel); "
+                             "time = the whole front-end call (6 kernels), flops = the filter's only"},
+        "cpu_baseline": cpu,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,9 +404,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy (for scaling rocprofv3 byte counters)")
-    ap.add_argument("--workload", choices=["fit", "serve", "coeff"], default="fit",
+    ap.add_argument("--workload", choices=["fit", "serve", "coeff", "frames"], default="fit",
                     help="fit: the colour-4D batch fit (default); serve: the streamed mix through a session; "
-                         "coeff: the B-spline coefficient build of the 5D model")
+                         "coeff: the B-spline coefficient build of the 5D model; frames: camera frames -> ROIs "
+                         "(-> fits)")
+    ap.add_argument("--nframes", type=int, default=32, help="frames: camera frames per step")
+    ap.add_argument("--cpu-frames", type=int, default=4, help="frames: frames in the CPU baseline sample")
     ap.add_argument("--frames", type=int, default=3000, help="serve: frames")
     ap.add_argument("--rate", type=float, default=1000.0, help="serve: frame arrivals per second")
     ap.add_argument("--rois", type=int, default=2000, help="serve: mean ROIs per frame")
@@ -316,6 +425,9 @@ def main():
         sys.exit(2)
     if args.workload == "coeff":
         coeff_bench(args)
+        return
+    if args.workload == "frames":
+        frames_bench(args)
         return
     if args.workload == "serve":
         # the session's kernel streams plus its two copy streams each need a

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <utility>
 
 #include "bsf_kernels.h"
@@ -55,10 +56,26 @@ constexpr int kRG = kFR / kR;     // row groups (6)
 static_assert(kRG * kR == kFR, "row groups must tile the filtered rows");
 static_assert(kRG * kFC <= kThreads, "one filter thread per (row group, column)");
 constexpr int kMaxTemplR = 17;    // kernels with mr <= this get an unrolled filter
+constexpr int kLoadBatch = 2;     // lower-halo loads a thread keeps in flight
+constexpr int kColBatch = 12;     // region columns a wave stages per round (loads in flight)
+static_assert(kTC % 4 == 0, "tile columns split evenly over the four waves");
 
 constexpr int kMaxGridZ = 65535;          // frames per detect launch
 constexpr int kCW = 8;                    // mask words per thread in the compaction
 constexpr int kCB = kThreads * kCW;       // mask words per compaction block
+
+template <typename T>
+__device__ __forceinline__ T load_px(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ uint16_t load_px<uint16_t>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ float load_px<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
 
 template <typename T>
 __device__ __forceinline__ float photon(T v, float offset, float conv)
@@ -74,7 +91,7 @@ struct DetectArgs {
     const float *h;       // device [mr, mc]
     int mr, mc;
     const float *sub;     // per-frame value subtracted from the photons, or NULL
-    double cutoff;
+    float cutoff_f;       // the cutoff rounded down to FP32
     int dx;
     double psf_half;      // > 0: border test of the PSF-convolution mode
     uint64_t *mask;
@@ -118,19 +135,68 @@ __global__ __launch_bounds__(kThreads) void detect_kernel(DetectArgs A)
     const T *fr = (const T *)A.raw + (size_t)f * A.nr * A.nc;
     const float sub = A.sub ? A.sub[f] : 0.0f;
 
-    // photons of the tile, the filter halo and the NMS ring; zero outside the frame
+    // photons of the tile, the filter halo and the NMS ring; zero outside the
+    // frame.  Region rows [0, 64): lane = row, wave w takes columns w, w+4,
+    // ..., kColBatch of them per round with their loads in flight together;
+    // the column tests are wave-uniform and the rows' are fixed per lane.
+    // Rows [64, XR) (the lower halo): a flat list over all threads.  Buffer
+    // loads return 0 outside the frame's bytes; every load address is clamped
+    // into the frame and the padding is applied by a select, because
+    // predicated loads would each be waited for on their own.
     const int gr0 = i0 - 1 - ar, gc0 = j0 - 1 - ac;
-    for (int c = wave; c < XC; c += kThreads / 64) {
-        const int gc = gc0 + c;
-        const bool cin = gc >= 0 && gc < A.nc;
-        for (int r = lane; r < XR; r += 64) {
-            const int gr = gr0 + r;
-            float x = 0.0f;
-            if (cin && gr >= 0 && gr < A.nr) {
-                x = photon(fr[(size_t)gc * A.nr + gr], A.offset, A.conv);
-                if (A.sub) x = x - sub;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)fr, (short)0, (int)(sizeof(T) * (size_t)A.nr * A.nc), 0x00020000);
+    {
+        const int gr = gr0 + lane;
+        const bool row_in = gr >= 0 && gr < A.nr;
+        const int voff = (gr < 0 ? 0 : gr >= A.nr ? A.nr - 1 : gr) * (int)sizeof(T);
+        const int w = __builtin_amdgcn_readfirstlane(wave);
+        for (int c0 = w; c0 < XC; c0 += 4 * kColBatch) {
+            T v[kColBatch];
+            // columns past the region are clamped to its last one, which is
+            // then written again with the same value: no branch, so the
+            // compiler keeps the loads together
+#pragma unroll
+            for (int k = 0; k < kColBatch; k++) {
+                const int c = min(c0 + 4 * k, XC - 1), gc = gc0 + c;
+                const int cgc = gc < 0 ? 0 : gc >= A.nc ? A.nc - 1 : gc;
+                v[k] = load_px<T>(rs, voff, cgc * A.nr * (int)sizeof(T));
             }
-            X[c * LDX + r] = x;
+#pragma unroll
+            for (int k = 0; k < kColBatch; k++) {
+                const int c = min(c0 + 4 * k, XC - 1), gc = gc0 + c;
+                float x = photon(v[k], A.offset, A.conv);
+                if (A.sub) x = x - sub;
+                X[c * LDX + lane] = row_in && gc >= 0 && gc < A.nc ? x : 0.0f;
+            }
+        }
+    }
+    {
+        const int nB = XR - 64;                                       // >= 2
+        const int nQ = nB * XC;
+        const float inv = 1.0f / (float)nB;
+        for (int q0 = threadIdx.x; q0 < nQ; q0 += kThreads * kLoadBatch) {
+            T v[kLoadBatch];
+            int at[kLoadBatch];
+            bool in[kLoadBatch];
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; k++) {
+                const int q = q0 + k * kThreads;
+                const int c = (int)(((float)q + 0.5f) * inv), r = 64 + q - c * nB;   // exact: q < 2^14, nB <= 52
+                const int gc = gc0 + c, gr = gr0 + r;
+                at[k] = q < nQ ? c * LDX + r : -1;
+                in[k] = q < nQ && gc >= 0 && gc < A.nc && gr >= 0 && gr < A.nr;
+                const int cgc = gc < 0 ? 0 : gc >= A.nc ? A.nc - 1 : gc;
+                const int cgr = gr < 0 ? 0 : gr >= A.nr ? A.nr - 1 : gr;
+                v[k] = load_px<T>(rs, (cgc * A.nr + cgr) * (int)sizeof(T), 0);
+            }
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; k++) {
+                if (at[k] < 0) continue;
+                float x = photon(v[k], A.offset, A.conv);
+                if (A.sub) x = x - sub;
+                X[at[k]] = in[k] ? x : 0.0f;
+            }
         }
     }
     __syncthreads();
@@ -169,25 +235,34 @@ __global__ __launch_bounds__(kThreads) void detect_kernel(DetectArgs A)
     }
     __syncthreads();
 
-    // maxima: wave w tests tile columns w, w+4, ...; lane = row in the tile
-    const int gi = i0 + lane;
-    for (int cc = wave; cc < kTC; cc += kThreads / 64) {
+    // maxima: lane = row of the tile, wave w takes tile columns
+    // [w * kTC/4, (w+1) * kTC/4) with a sliding window of three filtered
+    // columns (rows lane-1, lane, lane+1 of each).  Row tests are fixed per
+    // lane, column tests are wave-uniform; m > cutoff_f in FP32 is the same
+    // test as (double)m > cutoff (cutoff_f = cutoff rounded down to FP32).
+    const int gi = i0 + lane, y = gi + 1;
+    bool row_ok = gi >= 1 && gi <= A.nr - 2 && y > A.dx && y <= A.nr - A.dx;
+    if (A.psf_half > 0.0) row_ok = row_ok && !((double)y < A.psf_half || (double)y > A.nr - A.psf_half);
+    constexpr int kCPW = kTC / (kThreads / 64);                      // tile columns per wave
+    const int cb = __builtin_amdgcn_readfirstlane(wave) * kCPW;
+    const float *pl = Fl + lane;                                      // row lane-1 of the tile, column 0 of Fl
+    float l0 = pl[cb * kFLD], l1 = pl[cb * kFLD + 1], l2 = pl[cb * kFLD + 2];
+    float c0 = pl[(cb + 1) * kFLD], c1 = pl[(cb + 1) * kFLD + 1], c2 = pl[(cb + 1) * kFLD + 2];
+    for (int cc = cb; cc < cb + kCPW; cc++) {
         const int gj = j0 + cc;
-        if (gj >= A.nc) break;                    // wave-uniform
-        const float *p = Fl + (cc + 1) * kFLD + lane + 1;
-        const float m = p[0];
-        bool keep = gi >= 1 && gi <= A.nr - 2 && gj >= 1 && gj <= A.nc - 2;
-        keep = keep && m > p[-kFLD - 1] && m > p[-kFLD] && m > p[-kFLD + 1] && m > p[-1] && m > p[1] &&
-               m > p[kFLD - 1] && m > p[kFLD] && m > p[kFLD + 1];
-        keep = keep && (double)m > A.cutoff;
-        const int x = gj + 1, y = gi + 1;
-        keep = keep && x > A.dx && x <= A.nc - A.dx && y > A.dx && y <= A.nr - A.dx;
-        if (A.psf_half > 0.0) {
-            const double hp = A.psf_half;
-            keep = keep && !((double)x < hp || (double)x > A.nr - hp || (double)y < hp || (double)y > A.nr - hp);
-        }
+        if (gj >= A.nc) break;                                        // wave-uniform
+        const float *pr = pl + (cc + 2) * kFLD;
+        const float r0 = pr[0], r1 = pr[1], r2 = pr[2];
+        const int x = gj + 1;
+        bool col_ok = gj >= 1 && gj <= A.nc - 2 && x > A.dx && x <= A.nc - A.dx;
+        if (A.psf_half > 0.0) col_ok = col_ok && !((double)x < A.psf_half || (double)x > A.nr - A.psf_half);
+        const float m = c1;
+        const bool keep = col_ok && row_ok && m > A.cutoff_f && m > c0 && m > c2 && m > l0 && m > l1 && m > l2 &&
+                          m > r0 && m > r1 && m > r2;
         const uint64_t bits = __ballot(keep);
         if (lane == 0) A.mask[((size_t)f * A.nc + gj) * A.wpc + blockIdx.y] = bits;
+        l0 = c0, l1 = c1, l2 = c2;
+        c0 = r0, c1 = r1, c2 = r2;
     }
 }
 
@@ -383,7 +458,10 @@ hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
     A.mr = L.mr;
     A.mc = L.mc;
     A.sub = sub;
-    A.cutoff = L.cutoff;
+    // the largest float <= cutoff: m > it  <=>  (double)m > cutoff
+    float cf = (float)L.cutoff;
+    if ((double)cf > L.cutoff) cf = std::nextafter(cf, -HUGE_VALF);
+    A.cutoff_f = cf;
     A.dx = L.S / 2;
     A.psf_half = L.psf_size > 0.0 ? L.psf_size / 2.0 : 0.0;
     A.mask = mask;
