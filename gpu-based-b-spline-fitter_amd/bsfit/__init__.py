@@ -15,4 +15,5 @@ from .api import (  # noqa: F401
     lib_path, load, num_params, num_params_4pi, PREC_MIX, PREC_F64,
 )
 from .frames import find_rois, peak_filter, peak_filter_psf  # noqa: F401
+from .gui import color_class, gui_results  # noqa: F401
 from .session import FT_4PI, Group, Session  # noqa: F401

@@ -409,7 +409,9 @@ def make_frames(nframes, nrows, ncols, per_frame, seed, coeff=None, offset=100.0
     ADU = photons / conversion + offset, rounded (uint16) or not (float32).
 
     Returns (frames, truth) with truth rows (frame index from 0, x, y, z
-    spline coordinate, photons, bg), x = column and y = row in pixels from 0."""
+    spline coordinate, photons, bg[, colour coordinate]), x = column and y =
+    row in pixels from 0.  A 4D (colour) coefficient array renders each
+    emitter at a colour coordinate ~ U(1, 3) as well."""
     if coeff is None:
         coeff = model_coeffs("astig3d")
     rng = np.random.default_rng(seed)
@@ -424,7 +426,11 @@ def make_frames(nframes, nrows, ncols, per_frame, seed, coeff=None, offset=100.0
     bgf = rng.uniform(5, 30, nframes)
     cj, ci = np.rint(x).astype(int), np.rint(y).astype(int)
     c0 = float(h)
-    f = eval_spline_rois(coeff, patch, np.stack([c0 + x - cj, c0 + y - ci], 1), zc[:, None])
+    nonxy = zc[:, None]
+    if coeff.ndim == 4:                        # colour model: a colour coordinate per emitter
+        wc = 1 + rng.uniform(0, 2, n)
+        nonxy = np.stack([zc, wc], 1)
+    f = eval_spline_rois(coeff, patch, np.stack([c0 + x - cj, c0 + y - ci], 1), nonxy)
     mu = np.repeat(bgf[None, None, :], nrows, 0).repeat(ncols, 1).astype(np.float64)
     for k in range(n):
         mu[ci[k] - h:ci[k] + h + 1, cj[k] - h:cj[k] + h + 1, fi[k]] += ph[k] * np.maximum(f[k], 0.0)
@@ -434,5 +440,5 @@ def make_frames(nframes, nrows, ncols, per_frame, seed, coeff=None, offset=100.0
         frames = np.clip(np.rint(adu), 0, 65535).astype(np.uint16)
     else:
         frames = adu.astype(np.float32)
-    truth = np.stack([fi, x, y, zc, ph, bgf[fi]], 1)
+    truth = np.stack([fi, x, y, zc, ph, bgf[fi]] + ([nonxy[:, 1]] if coeff.ndim == 4 else []), 1)
     return np.asfortranarray(frames), truth

@@ -388,6 +388,28 @@ int bsf_peak_filter(int kind, double sigma, float *h, int cap, int *size);
 int bsf_peak_filter_psf(int kind, const float *psf, const int *psf_dims, int roi, float *h, int cap, int *rows,
                         int *cols);
 
+/* ---------------------------------------------------- GUI result table
+ * The GUI fitter's localisation table from one fit block
+ * (MultiD_fitter/simplefitter_spline.m:550-630): results [N, ncols]
+ * column-major double, ncols = bsf_gui_results_cols(fittype) (12, 17, 19),
+ * from P [N, NV+1], CRLB [N, NV], LL [N] (the fit's outputs) and peaks
+ * [N, 3] double (the GUI's peakcoordinates: x, y, frame).  dx = p.dx, z0 =
+ * p.z0, dz = p.dz, du = p.du, dv = p.dv (fittype 8), mirror = p.mirror.
+ *   column 1 frame; 2 x = P(:,1) (fittypes 7, 8) or P(:,2) (fittype 6) - dx +
+ *   peak x, or dx - P(:,2) + peak x when mirrored; 3 y; then z (fittypes 6,
+ *   7: -(z - z0) dz, fittype 8: (z - z0) dz), photons, background, errors
+ *   as real(sqrt(CRLB)), LL, ... exactly as the GUI lays them out, each value
+ *   computed in single precision as MATLAB does with single P and CRLB.
+ * Host arithmetic only (no device needed). */
+int bsf_gui_results_cols(int fittype);
+int bsf_gui_results(int fittype, int mirror, double dx, double z0, double dz, double du, double dv, const float *P,
+                    const float *CRLB, const float *LL, const double *peaks, int N, double *results);
+
+/* The colour demo's class from the fitted colour coordinate w = P(:,6)
+ * (demo_color_classification/bspline_based_color_classification_v3.m:
+ * 115-117): (min(3, max(2, w)) - 2.5) * 2 in single precision. */
+int bsf_color_class(const float *w, int N, float *cls);
+
 #ifdef __cplusplus
 }
 #endif

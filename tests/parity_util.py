@@ -64,3 +64,80 @@ class Hip:
         for p in ps:
             if p:
                 self.h.hipFree(p)
+
+
+class Mex:
+    """One MEX gateway of gpu-based-b-spline-fitter_amd/matlab/ compiled with
+    the test MX API (tests/mex_api) into a library of its own and called as
+    MATLAB would call it: numpy arrays in, numpy arrays out; a
+    mexErrMsgIdAndTxt comes back as MexError(id, message)."""
+
+    _CLS = {np.dtype(np.float64): 6, np.dtype(np.float32): 7, np.dtype(np.uint16): 11, np.dtype(np.uint32): 13}
+    _DT = {v: k for k, v in _CLS.items()}
+
+    def __init__(self, name, outdir):
+        import os
+        import subprocess
+        import bsfit
+        root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+        src = os.path.join(root, "gpu-based-b-spline-fitter_amd", "matlab", name + ".c")
+        so = os.path.join(str(outdir), name + ".so")
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-shared", "-fPIC", src,
+                        os.path.join(root, "tests", "mex_api", "mock_mx.c"), "-I", os.path.join(root, "include"),
+                        "-I", os.path.join(root, "tests", "mex_api"), "-o", so], check=True)
+        C.CDLL(bsfit.lib_path(), mode=C.RTLD_GLOBAL)      # the gateway's bsf_* symbols
+        self.lib = C.CDLL(so)
+        self.lib.mock_new.restype = C.c_void_p
+        self.lib.mock_new.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        self.lib.mock_data.restype = C.c_void_p
+        self.lib.mock_data.argtypes = [C.c_void_p]
+        self.lib.mock_free.argtypes = [C.c_void_p]
+        self.lib.mock_call.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        self.lib.mock_error_id.restype = C.c_char_p
+        self.lib.mock_error_msg.restype = C.c_char_p
+        for f in ("mxGetNumberOfDimensions", "mxGetClassID"):
+            getattr(self.lib, f).argtypes = [C.c_void_p]
+            getattr(self.lib, f).restype = C.c_size_t
+        self.lib.mxGetDimensions.argtypes = [C.c_void_p]
+        self.lib.mxGetDimensions.restype = C.POINTER(C.c_size_t)
+
+    def _arr(self, a):
+        a = np.asfortranarray(np.atleast_1d(a))
+        dims = (C.c_size_t * max(a.ndim, 1))(*a.shape)
+        p = self.lib.mock_new(self._CLS[a.dtype], a.ndim, dims)
+        C.memmove(self.lib.mock_data(p), a.ctypes.data, a.nbytes)
+        return p
+
+    def _np(self, p):
+        nd = self.lib.mxGetNumberOfDimensions(p)
+        d = self.lib.mxGetDimensions(p)
+        shape = tuple(d[k] for k in range(nd))
+        dt = self._DT[int(self.lib.mxGetClassID(p))]
+        n = int(np.prod(shape))
+        out = np.empty(shape, dt, order="F")
+        if n:
+            C.memmove(out.ctypes.data, self.lib.mock_data(p), n * dt.itemsize)
+        return out
+
+    def __call__(self, nlhs, *args):
+        ins = [self._arr(a) for a in args]
+        prhs = (C.c_void_p * max(len(ins), 1))(*ins)
+        plhs = (C.c_void_p * max(nlhs, 1))()
+        try:
+            rc = self.lib.mock_call(nlhs, plhs, len(ins), prhs)
+            if rc:
+                raise MexError(self.lib.mock_error_id().decode(), self.lib.mock_error_msg().decode())
+            outs = [self._np(plhs[k]) for k in range(max(nlhs, 1))]
+        finally:
+            for p in ins:
+                self.lib.mock_free(p)
+        for k in range(max(nlhs, 1)):
+            if plhs[k]:
+                self.lib.mock_free(plhs[k])
+        return outs
+
+
+class MexError(RuntimeError):
+    def __init__(self, ident, msg):
+        super().__init__("%s: %s" % (ident, msg))
+        self.ident = ident
