@@ -15,7 +15,9 @@ from parity_util import Hip as _Hip
 pytestmark = pytest.mark.gpu
 
 CASES = [((7,), 0.0), ((12, 9), 0.0), ((12, 9), 0.2), ((8, 6, 5), 0.1), ((6, 5, 4, 7), [0.0, 0.0, 0.1, 0.2]),
-         ((5, 4, 4, 3, 3), 1.0), ((2, 3, 2), 0.0), ((33, 17, 9), 5.0)]
+         ((5, 4, 4, 3, 3), 1.0), ((2, 3, 2), 0.0), ((33, 17, 9), 5.0),
+         # axes longer than 48 samples take the in-place global-memory path
+         ((60, 5), 0.0), ((50, 3, 2), [0.3, 0.0, 0.1]), ((100,), 1.0), ((7, 64), [0.0, 2.0])]
 
 
 def _close(gpu, ref):
