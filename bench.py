@@ -293,8 +293,9 @@ def frames_bench(args):
     1024x1024 uint16 ADU, 400 emitters each (3D astigmatic model), the GUI's
     DoG mode (sigma 1, 7x7 kernel, first-column minimum subtracted), 13x13
     ROIs; one bsf_find_rois_device call over all frames per step.  Then the
-    GUI's whole path on those frames: ROIs found, their count read back, and
-    the ROIs fitted with fittype 6 (50 iterations) on the device.
+    GUI's whole path on those frames with bsf_fit_frames_device: ROIs found
+    and fitted with fittype 6 (50 iterations), the fit reading the ROI count
+    on the device (no host round trip).
     Roofline: the detect kernel's FP32 filter arithmetic, 2 mr mc flop per
     pixel (the fused NMS, the halo recomputation and the compaction are not
     counted).  CPU baseline: the oracle's front-end (oracle/bsf_frontend.c)
@@ -345,18 +346,22 @@ def frames_bench(args):
     wall = (time.perf_counter() - t0) / args.steps
     ms = a.elapsed_time(b) / args.steps
     n = int(d_n.item())
-    # the GUI's whole path: frames -> ROIs -> count to the host -> fits
+    # the GUI's whole path: frames -> ROIs -> fits (fittype 6, 50 iterations)
     NV = 5
-    d_P = torch.empty((NV + 1) * n, dtype=torch.float32, device=dev)
-    d_CR = torch.empty(NV * n, dtype=torch.float32, device=dev)
-    d_LL = torch.empty(n, dtype=torch.float32, device=dev)
     d_c = torch.from_numpy(np.ascontiguousarray(coeff.ravel(order="F"))).to(dev)
 
+    d_P = torch.empty((NV + 1) * cap, dtype=torch.float32, device=dev)
+    d_CR = torch.empty(NV * cap, dtype=torch.float32, device=dev)
+    d_LL = torch.empty(cap, dtype=torch.float32, device=dev)
+    fq = bsfit.api._request(6, roi, cap, 50, d_c.data_ptr(), coeff.shape, None, None, bsfit.PREC_MIX, None)
+    fres = bsfit.api.Result(d_P.data_ptr(), d_CR.data_ptr(), d_LL.data_ptr())
+
     def end_to_end():
-        detect()
-        m = int(d_n.item())
-        bsfit.fit_device(6, d_st.data_ptr(), roi, min(m, cap), 50, d_c.data_ptr(), coeff.shape, d_P.data_ptr(),
-                         d_CR.data_ptr(), d_LL.data_ptr(), stream=st.cuda_stream)
+        # frames -> ROIs -> fits queued on the stream; the fit reads the ROI count on the device
+        rc = lib.bsf_fit_frames_device(C.byref(q), C.byref(fq), C.byref(res), d_n.data_ptr(), C.byref(fres),
+                                       d_w.data_ptr(), wb, st.cuda_stream)
+        if rc:
+            raise bsfit.BsfError(rc, lib.bsf_last_error().decode())
 
     end_to_end()
     torch.cuda.synchronize()

@@ -16,7 +16,7 @@ import ctypes as C
 
 import numpy as np
 
-from .api import BsfError, _check, load
+from .api import _NV, BsfError, Request, Result, _check, _f32, _request, load
 
 PIXEL_U16 = 0
 PIXEL_F32 = 1
@@ -63,6 +63,12 @@ def _bind():
         lib.bsf_find_rois_device.argtypes = [C.POINTER(RoiRequest), C.POINTER(RoiResult), C.c_void_p, C.c_void_p,
                                              C.c_size_t, C.c_void_p]
         lib.bsf_find_rois_device.restype = C.c_int
+        lib.bsf_fit_frames_host.argtypes = [C.POINTER(RoiRequest), C.POINTER(Request), C.POINTER(RoiResult),
+                                            C.POINTER(Result)]
+        lib.bsf_fit_frames_host.restype = C.c_int
+        lib.bsf_fit_frames_device.argtypes = [C.POINTER(RoiRequest), C.POINTER(Request), C.POINTER(RoiResult),
+                                              C.c_void_p, C.POINTER(Result), C.c_void_p, C.c_size_t, C.c_void_p]
+        lib.bsf_fit_frames_device.restype = C.c_int
         lib.bsf_peak_filter.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         lib.bsf_peak_filter.restype = C.c_int
         lib.bsf_peak_filter_psf.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
@@ -154,3 +160,42 @@ def find_rois(frames, roi, filt, cutoff, offset=0.0, conversion=1.0, subtract_mi
     _check(lib.bsf_find_rois_host(C.byref(q), C.byref(res)))
     n = min(res.count, ncap)
     return (stack[:, :, :n], np.asfortranarray(peaks[:n]), value[:n] if value is not None else None, res.count)
+
+
+def fit_frames(frames, roi, filt, cutoff, fittype, iterations, coeff, offset=0.0, conversion=1.0,
+               subtract_min=False, mirror=False, psf_size=0.0, first_frame=1, capacity=None, initP=None,
+               coeff3=None, precision=0):
+    """Run bsf_fit_frames_host: the ROIs of find_rois and their fits in one
+    call.  Returns (imstack, peaks, value, P, CRLB, LL, count) with
+    n = min(count, capacity) ROIs; P (n, NV+1) and CRLB (n, NV) as fit()."""
+    lib = _bind()
+    if frames.dtype not in (np.uint16, np.float32):
+        raise BsfError(1, "frames must be uint16 or float32 (got %s)" % frames.dtype)
+    pt = PIXEL_U16 if frames.dtype == np.uint16 else PIXEL_F32
+    frames = np.asfortranarray(frames)
+    if frames.ndim == 2:
+        frames = frames[:, :, None]
+    filt = np.asfortranarray(filt, np.float32)
+    if capacity is None:
+        capacity = max(1, frames.shape[2] * ((frames.shape[0] + 1) // 2) * ((frames.shape[1] + 1) // 2) // 16)
+    ncap = max(int(capacity), 0)
+    NV = _NV.get(int(fittype), 1)
+    stack = np.zeros((roi, roi, ncap), np.float32, order="F")
+    peaks = np.zeros((ncap, 3), np.int32, order="F")
+    value = np.zeros(ncap, np.float32)
+    P = np.zeros((ncap, NV + 1), np.float32, order="F")
+    CR = np.zeros((ncap, NV), np.float32, order="F")
+    LL = np.zeros(ncap, np.float32)
+    coeff = _f32(coeff)
+    c3 = _f32(coeff3) if coeff3 is not None else None
+    ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
+    q = make_request(frames.ctypes.data, pt, frames.shape, roi, filt.ctypes.data, filt.shape, cutoff, offset,
+                     conversion, subtract_min, mirror, psf_size, first_frame, capacity)
+    f = _request(fittype, roi, ncap, iterations, coeff.ctypes.data, coeff.shape,
+                 c3.ctypes.data if c3 is not None else None, ip, precision, None)
+    r = RoiResult(stack.ctypes.data, peaks.ctypes.data, value.ctypes.data, 0)
+    res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
+    _check(lib.bsf_fit_frames_host(C.byref(q), C.byref(f), C.byref(r), C.byref(res)))
+    n = min(r.count, ncap)
+    return (stack[:, :, :n], np.asfortranarray(peaks[:n]), value[:n], np.asfortranarray(P[:n]),
+            np.asfortranarray(CR[:n]), LL[:n], r.count)

@@ -96,6 +96,7 @@ struct Dev {
     const RoiDesc *rois;   // per-ROI placement, or NULL: uniform S, ROI n at data + n*S*S
     float *P, *CRLB, *LL;
     int N, S, iterations, roi_offset;
+    const int *ncount;     // device: ROIs to fit (fewer than N), or NULL: all N
     int nd[5];
     long s1, s2, s3, s4;   // coefficient strides of dims 1..4
     int cbytes, c3bytes;   // coefficient array sizes (buffer record counts)

@@ -35,7 +35,15 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     constexpr int NRED = 1 + NV + NH;
     extern __shared__ __align__(16) unsigned char smem[];
 
-    const int n = xcd_block(blockIdx.x, gridDim.x) + A.roi_offset;
+    // with a device-side ROI count, the blocks below it are spread over the
+    // XCDs as the whole grid otherwise is (the rest exit at once)
+    int G = (int)gridDim.x;
+    if (A.ncount) {
+        const int c = *A.ncount - A.roi_offset;
+        G = c < 0 ? 0 : c < G ? c : G;
+        if ((int)blockIdx.x >= G) return;
+    }
+    const int n = xcd_block(blockIdx.x, G) + A.roi_offset;
     if (n >= A.N) return;
     const int lane = threadIdx.x;
     const float *dptr = A.data + (size_t)n * A.S * A.S;
@@ -420,6 +428,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     A.S = L.S;
     A.iterations = L.iterations;
     A.roi_offset = L.roi_offset;
+    A.ncount = L.ncount;
     for (int a = 0; a < 5; a++) A.nd[a] = L.cdims[a] - 3;
     A.s1 = L.cdims[0];
     A.s2 = A.s1 * L.cdims[1];

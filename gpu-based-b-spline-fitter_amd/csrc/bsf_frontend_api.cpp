@@ -271,4 +271,116 @@ int bsf_find_rois_host(const bsf_roi_request *r, bsf_roi_result *res)
     return rc;
 }
 
+int bsf_fit_frames_device(const bsf_roi_request *rq, const bsf_request *fq, bsf_roi_result *rois, int *count,
+                          bsf_result *res, void *work, size_t work_bytes, void *stream)
+{
+    bsf::clear_error();
+    int rc = validate_rois(rq, rois);
+    if (rc) return rc;
+    if (!fq || !count) return fail(BSF_ERR_ARG, "bsf: NULL fit request or count");
+    if (rq->nframes > 0 && (!work || work_bytes < bsf::front_work_bytes(rq->nrows, rq->ncols, rq->nframes)))
+        return fail(BSF_ERR_ARG, "bsf: work buffer of %zu B is smaller than bsf_find_rois_work_bytes (%zu B)",
+                    work_bytes, bsf::front_work_bytes(rq->nrows, rq->ncols, rq->nframes));
+    // the fit reads the ROI stack in place: sz = roi, nfits = capacity (the
+    // outputs' stride); only the first *count ROIs are fitted
+    bsf_request q = *fq;
+    q.imstack = rois->imstack;
+    q.sz = rq->roi;
+    q.nfits = rq->capacity;
+    bsf::FitLaunch L;
+    if ((rc = bsf::prepare_fit_launch(&q, res, &L))) return rc;
+    L.ncount = count;
+    const hipStream_t st = (hipStream_t)stream;
+    bsf::FrontLaunch F = make_front(rq, rq->frames, rq->filter, rois->imstack, rois->peaks, rois->value, count, work);
+    hipError_t e = bsf::launch_find_rois(F, st);
+    if (e == hipSuccess && rq->capacity > 0) e = bsf::launch_fit_all(L, st);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: launch failed: %s", hipGetErrorString(e));
+    return BSF_OK;
+}
+
+int bsf_fit_frames_host(const bsf_roi_request *rq, const bsf_request *fq, bsf_roi_result *rois, bsf_result *res)
+{
+    bsf::clear_error();
+    int rc = validate_rois(rq, rois);
+    if (rc) return rc;
+    if (!fq || !res || !fq->coeff) return fail(BSF_ERR_ARG, "bsf: NULL fit request, result or coeff");
+    const int NV = bsf_num_params(fq->fittype);
+    if (NV < 0) return fail(BSF_ERR_FITTYPE, "bsf: fittype %d is not a B-spline fittype (6, 7, 8)", fq->fittype);
+    if (fq->coeff_ndim < 1 || fq->coeff_ndim > 5) return fail(BSF_ERR_SHAPE, "bsf: bad coeff_ndim %d", fq->coeff_ndim);
+    if (rq->capacity > 0 && (!res->P || !res->CRLB || !res->LL)) return fail(BSF_ERR_ARG, "bsf: NULL fit output");
+    if ((rc = check_gfx950())) return rc;
+    const size_t px = rq->pixel_type == BSF_PIXEL_U16 ? 2 : 4;
+    const size_t fbytes = px * (size_t)rq->nrows * rq->ncols * rq->nframes;
+    const size_t hbytes = sizeof(float) * (size_t)rq->filter_rows * rq->filter_cols;
+    const size_t S2 = (size_t)rq->roi * rq->roi, cap = (size_t)rq->capacity;
+    const size_t wbytes = bsf::front_work_bytes(rq->nrows, rq->ncols, rq->nframes);
+    size_t cn = 1;
+    for (int a = 0; a < fq->coeff_ndim; a++) cn *= (size_t)(fq->coeff_dims[a] > 0 ? fq->coeff_dims[a] : 0);
+    const size_t c3n = fq->fittype == 8 && fq->coeff_ndim >= 3
+                           ? (size_t)fq->coeff_dims[0] * fq->coeff_dims[1] * fq->coeff_dims[2] : 0;
+    void *d_frames = nullptr, *d_work = nullptr;
+    float *d_h = nullptr, *d_stack = nullptr, *d_value = nullptr, *d_c = nullptr, *d_c3 = nullptr;
+    float *d_P = nullptr, *d_CR = nullptr, *d_LL = nullptr;
+    int *d_peaks = nullptr, *d_count = nullptr;
+    hipError_t e = hipSuccess;
+    auto alloc = [&](void **p, size_t b) {
+        if (e == hipSuccess) e = hipMalloc(p, b ? b : 4);
+    };
+    alloc(&d_frames, fbytes);
+    alloc((void **)&d_h, hbytes);
+    alloc(&d_work, wbytes);
+    alloc((void **)&d_stack, sizeof(float) * S2 * cap);
+    alloc((void **)&d_peaks, sizeof(int) * 3 * cap);
+    if (rois->value) alloc((void **)&d_value, sizeof(float) * cap);
+    alloc((void **)&d_count, sizeof(int));
+    alloc((void **)&d_c, sizeof(float) * cn);
+    if (c3n && fq->coeff3) alloc((void **)&d_c3, sizeof(float) * c3n);
+    alloc((void **)&d_P, sizeof(float) * cap * (NV + 1));
+    alloc((void **)&d_CR, sizeof(float) * cap * NV);
+    alloc((void **)&d_LL, sizeof(float) * cap);
+    int count = 0;
+    if (e != hipSuccess) {
+        rc = fail(BSF_ERR_NOMEM, "bsf: device allocation failed: %s", hipGetErrorString(e));
+    } else {
+        if (fbytes) e = hipMemcpy(d_frames, rq->frames, fbytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_h, rq->filter, hbytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_c, fq->coeff, sizeof(float) * cn, hipMemcpyHostToDevice);
+        if (e == hipSuccess && d_c3) e = hipMemcpy(d_c3, fq->coeff3, sizeof(float) * c3n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            rc = fail(BSF_ERR_DEVICE, "bsf: HIP error: %s", hipGetErrorString(e));
+        } else {
+            bsf_roi_request q = *rq;
+            q.frames = d_frames;
+            q.filter = d_h;
+            bsf_request f = *fq;
+            f.coeff = d_c;
+            f.coeff3 = d_c3;
+            bsf_roi_result dr = {d_stack, d_peaks, d_value, 0};
+            bsf_result dres = {d_P, d_CR, d_LL};
+            rc = bsf_fit_frames_device(&q, &f, &dr, d_count, &dres, d_work, wbytes, nullptr);
+            if (!rc) {
+                e = hipMemcpy(&count, d_count, sizeof(int), hipMemcpyDeviceToHost);
+                const size_t n = (size_t)count < cap ? (size_t)count : cap;
+                if (e == hipSuccess && n) e = hipMemcpy(rois->imstack, d_stack, sizeof(float) * S2 * n,
+                                                        hipMemcpyDeviceToHost);
+                for (int c = 0; c < 3 && e == hipSuccess && n; c++)
+                    e = hipMemcpy(rois->peaks + c * cap, d_peaks + c * cap, sizeof(int) * n, hipMemcpyDeviceToHost);
+                if (e == hipSuccess && n && rois->value)
+                    e = hipMemcpy(rois->value, d_value, sizeof(float) * n, hipMemcpyDeviceToHost);
+                for (int c = 0; c <= NV && e == hipSuccess && n; c++)
+                    e = hipMemcpy(res->P + c * cap, d_P + c * cap, sizeof(float) * n, hipMemcpyDeviceToHost);
+                for (int c = 0; c < NV && e == hipSuccess && n; c++)
+                    e = hipMemcpy(res->CRLB + c * cap, d_CR + c * cap, sizeof(float) * n, hipMemcpyDeviceToHost);
+                if (e == hipSuccess && n) e = hipMemcpy(res->LL, d_LL, sizeof(float) * n, hipMemcpyDeviceToHost);
+                if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: HIP error: %s", hipGetErrorString(e));
+            }
+        }
+    }
+    for (void *p : {d_frames, (void *)d_h, d_work, (void *)d_stack, (void *)d_peaks, (void *)d_value, (void *)d_count,
+                    (void *)d_c, (void *)d_c3, (void *)d_P, (void *)d_CR, (void *)d_LL})
+        (void)hipFree(p);
+    rois->count = rc ? 0 : count;
+    return rc;
+}
+
 }  // extern "C"

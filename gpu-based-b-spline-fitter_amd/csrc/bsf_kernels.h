@@ -26,6 +26,7 @@ struct FitLaunch {
     float *P, *CRLB, *LL;   // device outputs, column-major with stride N
     int N, S, iterations;   // S: the ROI size, or the largest one when rois is set
     int roi_offset, count;  // ROIs [roi_offset, roi_offset + count)
+    const int *ncount;      // device: only ROIs below this count are fitted (outputs keep stride N), or NULL
     const RoiDesc *rois;    // device, per-ROI placement, or NULL
     int cdims[5], ndim;
     float initP[4];
@@ -92,6 +93,11 @@ struct FrontLaunch {
 hipError_t launch_find_rois(const FrontLaunch &L, hipStream_t st);
 size_t front_work_bytes(int nr, int nc, int nframes);
 size_t front_detect_lds_bytes(int mr, int mc);
+
+// Checks a fit request as bsf_fit_device does and builds its launch (the
+// request's imstack and outputs are device pointers).
+int prepare_fit_launch(const struct bsf_request *q, const struct bsf_result *res, FitLaunch *L);
+hipError_t launch_fit_all(const FitLaunch &L, hipStream_t st);
 
 int fail(int code, const char *fmt, ...);
 void clear_error();

@@ -141,6 +141,21 @@ hipError_t launch_all(bsf::FitLaunch L, hipStream_t st)
 
 }  // namespace
 
+namespace bsf {
+
+int prepare_fit_launch(const bsf_request *q, const bsf_result *res, FitLaunch *L)
+{
+    int rc = validate(q, res);
+    if (rc) return rc;
+    if ((rc = check_device(q->sz, q->fittype, q->precision == BSF_PREC_F64))) return rc;
+    *L = make_launch(q, q->imstack, q->coeff, q->coeff3, res->P, res->CRLB, res->LL);
+    return BSF_OK;
+}
+
+hipError_t launch_fit_all(const FitLaunch &L, hipStream_t st) { return launch_all(L, st); }
+
+}  // namespace bsf
+
 extern "C" {
 
 int bsf_num_params(int fittype) { return nv_of(fittype); }

@@ -367,6 +367,22 @@ size_t bsf_find_rois_work_bytes(const bsf_roi_request *req);
 int bsf_find_rois_device(const bsf_roi_request *req, bsf_roi_result *res, int *count, void *work, size_t work_bytes,
                          void *stream);
 
+/* Frames -> ROIs -> fits in one call: the GUI's per-frame loop followed by
+ * its fitspline call on the ROI stack (simplefitter_spline.m:197-316, 479-546)
+ * without a round trip to the host.  fit gives fittype, iterations, coeff
+ * (and coeff3, initP, precision) as for bsf_fit_device; its imstack, sz and
+ * nfits are taken from the ROI stack (sz = roi, nfits = capacity).  The fit
+ * outputs res->P [capacity, NV+1], CRLB [capacity, NV], LL [capacity] keep
+ * stride capacity; the first *count rows (at most capacity) are the fits of
+ * the ROIs in rois, in the same order.  Device pointers, queued on stream,
+ * the count read by the fit on the device; allocates nothing. */
+int bsf_fit_frames_device(const bsf_roi_request *rq, const bsf_request *fit, bsf_roi_result *rois, int *count,
+                          bsf_result *res, void *work, size_t work_bytes, void *stream);
+
+/* The same from host frames and host coefficients into host outputs (sized
+ * for capacity ROIs; rois->count receives the number found). */
+int bsf_fit_frames_host(const bsf_roi_request *rq, const bsf_request *fit, bsf_roi_result *rois, bsf_result *res);
+
 /* The GUI's peak-finding kernels (simplefitter_spline.m:195-212), computed
  * in FP64 and rounded to FP32:
  *   BSF_PEAK_GAUSS  fspecial('gaussian', max(3, ceil(3 sigma + 1)), sigma)

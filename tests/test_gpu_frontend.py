@@ -150,3 +150,27 @@ def test_frames_to_fits():
         m = pk[:, 2] == f + 1
         hit += bool(m.any() and np.hypot(xs[m] - x, ys[m] - y).min() < 0.3)
     assert hit >= 0.7 * len(truth), (hit, len(truth))
+
+
+@pytest.mark.parametrize("ft", [6, 7])
+def test_fit_frames_equals_find_then_fit(ft):
+    """bsf_fit_frames_host (ROIs and fits in one call, the fit reading the
+    ROI count on the device) equals bsf_find_rois_host followed by
+    bsf_fit_host on its ROIs, bit for bit; a capacity below the count
+    fits the first capacity ROIs."""
+    coeff = gen.model_coeffs("astig3d" if ft == 6 else "color4d")
+    frames, _ = gen.make_frames(3, 128, 120, 10, 40 + ft, coeff=coeff)
+    h = FR.peak_filter("dog", 1.0)
+    initP = None if ft == 6 else np.array([30.5, 1.5], np.float32)
+    for cap in (1000, 9):
+        args = dict(cutoff=15.0, offset=100.0, conversion=0.5, subtract_min=True, capacity=cap)
+        st, pk, val, P, CR, LL, n = FR.fit_frames(frames, 13, h, fittype=ft, iterations=50, coeff=coeff,
+                                                  initP=initP, **args)
+        st2, pk2, val2, n2 = FR.find_rois(frames, 13, h, **args)
+        assert n == n2 > 9
+        np.testing.assert_array_equal(st, st2)
+        np.testing.assert_array_equal(pk, pk2)
+        P2, CR2, LL2 = bsfit.fit(ft, st2, 50, coeff, initP)
+        np.testing.assert_array_equal(P, P2)
+        np.testing.assert_array_equal(CR, CR2)
+        np.testing.assert_array_equal(LL, LL2)
