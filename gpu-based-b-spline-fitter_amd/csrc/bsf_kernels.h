@@ -53,10 +53,16 @@ struct Fit4PiLaunch {
     float phaseShift[8];      // w units
     const float *dT;          // device [6, nch, N] or NULL
     const float *initZ, *initPhi;  // device [N] or NULL
+    int wpc;                  // 1: a workgroup of nch waves per ROI, one per channel; 0: one wave per ROI
 };
 
 hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st);
 size_t lds_bytes_4pi(int f64, int S, int nch, int nv);
+size_t lds_bytes_4pi_wpc(int f64, int S, int nch, int nv);
+// 4Pi kernel choice: -1 automatic, 0 one wave per ROI, 1 one wave per channel
+extern int g_4pi_kernel;
+// the wave-per-channel kernel when asked for (prefer = 1) or forced, and its LDS fits
+int use_wpc_4pi(int prefer, int f64, int S, int nch, int nv, size_t lds_limit);
 
 // Cubic B-spline coefficients (bsarray, degree 3): samples [dims] column-major
 // (device, FP32), lambda one per axis or NULL, coeff [dims + 3] (device,

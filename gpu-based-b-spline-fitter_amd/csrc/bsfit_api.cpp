@@ -386,6 +386,11 @@ static int check_device_4pi(const bsf_request_4pi *r)
     return BSF_OK;
 }
 
+// The wave-per-channel 4Pi kernel finishes a ROI sooner but moves fewer
+// ROIs per second once the GPU is full: launches of fewer ROIs than this use
+// it (measured crossover between 400 and 1,600 ROIs of 4 x 13 x 13).
+constexpr int kWpcMaxRois = 1024;
+
 static bsf::Fit4PiLaunch make_launch_4pi(const bsf_request_4pi *r, const float *data, const float *coeff,
                                          const float *dT, const float *iz, const float *ip, float *P, float *CR,
                                          float *LL)
@@ -408,6 +413,11 @@ static bsf::Fit4PiLaunch make_launch_4pi(const bsf_request_4pi *r, const float *
     L.dT = dT;
     L.initZ = iz;
     L.initPhi = ip;
+    int dev = 0;
+    hipDeviceProp_t p;
+    const size_t limit = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess
+                             ? (size_t)p.sharedMemPerBlock : 0;
+    L.wpc = bsf::use_wpc_4pi(r->nfits < kWpcMaxRois, L.f64, L.S, L.nch, nv_4pi(r->shared, r->nch), limit);
     return L;
 }
 
@@ -421,6 +431,14 @@ static hipError_t launch_all_4pi(bsf::Fit4PiLaunch L, hipStream_t st)
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// Test hook (not in the header): force the 4Pi kernel (-1 automatic, 0 one
+// wave per ROI, 1 one wave per channel) for every later launch.
+int bsf_test_set_4pi_kernel(int mode)
+{
+    bsf::g_4pi_kernel = mode < 0 ? -1 : mode ? 1 : 0;
+    return BSF_OK;
 }
 
 int bsf_fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, void *stream)

@@ -50,3 +50,32 @@ def test_4pi_matches_oracle(orc, shared):
     ll_ok = np.mean(gm[2] >= of[2] - 1e-3 * np.abs(of[2]) - 1e-3)
     print("LL(gpu MIX) >= LL(oracle F32) on %.3f" % ll_ok, flush=True)
     assert ll_ok >= 0.75
+
+
+@pytest.mark.parametrize("shared,nch,S", [((1, 1, 1, 1, 1, 1), 4, 13), ((1, 1, 1, 0, 1, 1), 4, 11),
+                                          ((0, 0, 1, 0, 1, 1), 2, 15), ((1, 1, 0, 0, 1, 1), 3, 13),
+                                          ((0, 0, 0, 0, 1, 1), 4, 13)])
+def test_wave_per_channel_kernel_equals_one_wave_kernel(shared, nch, S):
+    """The two 4Pi kernels (one wave per ROI; one wave per channel, used for
+    small launches and by sessions) give identical results, MIX and FP64:
+    the second adds the same channel blocks in the same order."""
+    import ctypes as C
+    import bsfit
+    from bsfit import gen
+    lib = bsfit.load()
+    lib.bsf_test_set_4pi_kernel.argtypes = [C.c_int]
+    w = gen.make_rois_4pi(80, S, 77 + nch, gen.model_coeffs("4pi"), shared=shared)
+    im = np.asfortranarray(w.imstack[:, :, :, :nch])
+    ps = np.ascontiguousarray(w.phaseShift[:nch])
+    print("case 4Pi shared=%s nch=%d S=%d" % (shared, nch, S), flush=True)
+    try:
+        for prec in (bsfit.PREC_MIX, bsfit.PREC_F64):
+            out = []
+            for mode in (0, 1):
+                lib.bsf_test_set_4pi_kernel(mode)
+                out.append(bsfit.fit_4pi(im, shared, 50, w.coeff, ps, initZ=w.initZ, initPhi=w.initPhi,
+                                         precision=prec))
+            for a, b in zip(*out):
+                np.testing.assert_array_equal(a, b)
+    finally:
+        lib.bsf_test_set_4pi_kernel(-1)
