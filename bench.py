@@ -418,8 +418,8 @@ def main():
     ap.add_argument("--frames", type=int, default=3000, help="serve: frames")
     ap.add_argument("--rate", type=float, default=1000.0, help="serve: frame arrivals per second")
     ap.add_argument("--rois", type=int, default=2000, help="serve: mean ROIs per frame")
-    ap.add_argument("--kernel-streams", type=int, default=6, help="serve: session kernel streams")
-    ap.add_argument("--depth", type=int, default=8, help="serve: frames in flight in the closed-loop phase")
+    ap.add_argument("--kernel-streams", type=int, default=16, help="serve: session kernel streams")
+    ap.add_argument("--depth", type=int, default=16, help="serve: frames in flight in the closed-loop phase")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()

@@ -52,6 +52,8 @@ struct Model {
 };
 
 // One kernel launch of a frame: the groups sharing model, iterations and start.
+constexpr int kMaxKernelStreams = 16;
+
 struct Launch {
     int model = 0, iterations = 0, n = 0, Smax = 0, nch = 1, nv = 0;
     float initP[4] = {0, 0, 0, 0};
@@ -97,11 +99,11 @@ struct DeviceGuard {
 
 struct bsf_session {
     int device = 0, precision = BSF_PREC_MIX, nks = 2;
-    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // queued, undelivered launch cost per kernel stream
+    double load[kMaxKernelStreams] = {};           // queued, undelivered launch cost per kernel stream
     size_t budget = 0, arena = 0, model_bytes = 0, lds_limit = 0;
     char *d_arena = nullptr, *h_arena = nullptr;
     hipStream_t s_in = nullptr, s_out = nullptr;
-    hipStream_t ks[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipStream_t ks[kMaxKernelStreams] = {};
     std::vector<Model> models;
     std::deque<Frame> inflight;
     std::vector<hipEvent_t> event_pool;
@@ -287,7 +289,8 @@ int bsf_session_create(const bsf_session_config *cfg, bsf_session **out)
     if (cfg->precision != BSF_PREC_MIX && cfg->precision != BSF_PREC_F64)
         return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", cfg->precision);
     const int nks = cfg->kernel_streams ? cfg->kernel_streams : 2;
-    if (nks < 1 || nks > 8) return fail(BSF_ERR_ARG, "bsf: kernel_streams must be 1..8 (got %d)", nks);
+    if (nks < 1 || nks > kMaxKernelStreams)
+        return fail(BSF_ERR_ARG, "bsf: kernel_streams must be 1..%d (got %d)", kMaxKernelStreams, nks);
     const size_t budget = cfg->hbm_budget ? cfg->hbm_budget : ((size_t)2 << 30);
     const size_t arena = up(cfg->arena_bytes ? cfg->arena_bytes : ((size_t)256 << 20));
     if (arena > budget)

@@ -218,7 +218,7 @@ typedef struct bsf_session_config {
     int device;             /* HIP device ordinal */
     size_t hbm_budget;      /* device bytes for models + arena (0: 2 GiB) */
     size_t arena_bytes;     /* frame arena, part of the budget (0: 256 MiB) */
-    int kernel_streams;     /* streams that run fit launches, 1..8 (0: 2); with the two
+    int kernel_streams;     /* streams that run fit launches, 1..16 (0: 2); with the two
                                copy streams, keep within the HIP runtime's hardware
                                queues (GPU_MAX_HW_QUEUES, default 4) */
     int precision;          /* enum bsf_precision for submitted frames */

@@ -54,7 +54,7 @@ def test_session_errors_without_device_or_bad_args():
     import bsfit
     # configuration errors are reported before any device work
     with pytest.raises(bsfit.BsfError) as e:
-        bsfit.Session(kernel_streams=9)
+        bsfit.Session(kernel_streams=17)
     assert e.value.code == 1
     with pytest.raises(bsfit.BsfError) as e:
         bsfit.Session(hbm_budget=1 << 20, arena_bytes=2 << 20)
