@@ -61,12 +61,12 @@ def cpu_baseline(w, budget_s=15.0):
                       % (n, w.N, t)}
 
 
-def traffic_bytes():
-    """HBM bytes per launch of the fit kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy),
-    or None when no summary is present."""
+def traffic_bytes(path=None):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy;
+    default: the fit kernel's), or None when no summary is present."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path or TRAFFIC_FILE):
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(float(line.split(":", 1)[1]))
     except OSError:
@@ -392,9 +392,12 @@ def frames_bench(args):
                                % (nf, nr, nc, per), "rois_found": n,
                    "frames_to_fits_ms": e2e * 1e3, "frames_to_fits_per_s": n / e2e},
         "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": FP32_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / FP32_PEAK, "traffic": None,
+                     "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / FP32_PEAK,
+                     "traffic": traffic_bytes(os.path.join(ROOT, "profiles", "detect_kernel_frames_traffic.md")),
                      "note": "FP32 vector-ALU bound (filter2: 2 x 49 flop per pixel against 2 B of raw pixel); "
-                             "time = the whole front-end call (6 kernels), flops = the filter's only"},
+                             "time = the whole front-end call (6 kernels), flops = the filter's only; traffic = "
+                             "the detect kernel's HBM bytes per call (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
+                             "copy-calibrated; algorithmic 71,303,168 B)"},
         "cpu_baseline": cpu,
     }))
 

@@ -16,7 +16,8 @@
 //
 // Kernels, all on the caller's stream:
 //   column_min_kernel  one workgroup per frame: the first column's minimum.
-//   detect_kernel      one workgroup per 64-row x 40-column tile: raw pixels
+//   detect_kernel      one workgroup per 64-row x 40-column tile (tiles dealt
+//                      out in contiguous runs per XCD): raw pixels
 //                      of the tile plus its filter and NMS halo -> photons in
 //                      LDS -> filtered values of the tile plus a one-pixel
 //                      ring in LDS -> maxima test; each wave writes one
@@ -35,6 +36,7 @@
 #include <cmath>
 #include <utility>
 
+#include "bsf_device.h"
 #include "bsf_kernels.h"
 
 // photons and differences are rounded separately, as MATLAB rounds each
@@ -60,7 +62,6 @@ constexpr int kLoadBatch = 2;     // lower-halo loads a thread keeps in flight
 constexpr int kColBatch = 12;     // region columns a wave stages per round (loads in flight)
 static_assert(kTC % 4 == 0, "tile columns split evenly over the four waves");
 
-constexpr int kMaxGridZ = 65535;          // frames per detect launch
 constexpr int kCW = 8;                    // mask words per thread in the compaction
 constexpr int kCB = kThreads * kCW;       // mask words per compaction block
 
@@ -96,6 +97,7 @@ struct DetectArgs {
     double psf_half;      // > 0: border test of the PSF-convolution mode
     uint64_t *mask;
     int wpc;              // mask words per column
+    int tiles_x;          // column tiles per frame
     int f0;               // first frame of this launch
 };
 
@@ -129,8 +131,14 @@ __global__ __launch_bounds__(kThreads) void detect_kernel(DetectArgs A)
     float *X = lds;                               // photons [XC][LDX]
     float *Fl = lds + XC * LDX;                   // filtered [kFC][kFLD]
 
-    const int f = A.f0 + (int)blockIdx.z;
-    const int i0 = blockIdx.y * kTR, j0 = blockIdx.x * kTC;
+    // tile of this workgroup: the flat grid is dealt out so that each XCD
+    // (blocks go round-robin over the 8) gets a contiguous run of tiles,
+    // column tiles fastest, and neighbouring tiles share halos and mask
+    // lines in one L2
+    const int tq = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int tx = tq % A.tiles_x, tr = tq / A.tiles_x;
+    const int ty = tr % A.wpc, f = A.f0 + tr / A.wpc;
+    const int i0 = ty * kTR, j0 = tx * kTC;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const T *fr = (const T *)A.raw + (size_t)f * A.nr * A.nc;
     const float sub = A.sub ? A.sub[f] : 0.0f;
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(kThreads) void detect_kernel(DetectArgs A)
         const bool keep = col_ok && row_ok && m > A.cutoff_f && m > c0 && m > c2 && m > l0 && m > l1 && m > l2 &&
                           m > r0 && m > r1 && m > r2;
         const uint64_t bits = __ballot(keep);
-        if (lane == 0) A.mask[((size_t)f * A.nc + gj) * A.wpc + blockIdx.y] = bits;
+        if (lane == 0) A.mask[((size_t)f * A.nc + gj) * A.wpc + ty] = bits;
         l0 = c0, l1 = c1, l2 = c2;
         c0 = r0, c1 = r1, c2 = r2;
     }
@@ -468,10 +476,13 @@ hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
     A.wpc = w.wpc;
     const size_t lds = front_detect_lds_bytes(L.mr, L.mc);
     const DetectFn fn = detect_fn<T>(L.mr, std::make_index_sequence<kMaxTemplR + 1>{});
-    for (int f0 = 0; f0 < L.nframes; f0 += kMaxGridZ) {
+    A.tiles_x = (L.nc + kTC - 1) / kTC;
+    const int per_frame = A.tiles_x * w.wpc;
+    const int fchunk = (int)((1L << 30) / per_frame) > 0 ? (int)((1L << 30) / per_frame) : 1;   // grid < 2^30
+    for (int f0 = 0; f0 < L.nframes; f0 += fchunk) {
         A.f0 = f0;
-        const int nf = L.nframes - f0 < kMaxGridZ ? L.nframes - f0 : kMaxGridZ;
-        const dim3 grid((unsigned)((L.nc + kTC - 1) / kTC), (unsigned)w.wpc, (unsigned)nf);
+        const int nf = L.nframes - f0 < fchunk ? L.nframes - f0 : fchunk;
+        const dim3 grid((unsigned)(per_frame * nf));
         void *args[] = {&A};
         const hipError_t e = hipLaunchKernel((const void *)fn, grid, dim3(kThreads), args, lds, st);
         if (e != hipSuccess) return e;
