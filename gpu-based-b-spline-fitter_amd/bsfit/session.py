@@ -44,7 +44,8 @@ class RoiGroup(C.Structure):
     _fields_ = [("model", C.c_int), ("sz", C.c_int), ("count", C.c_int), ("iterations", C.c_int),
                 ("data", C.c_void_p), ("chan_stride", C.c_size_t), ("initP", C.c_void_p), ("n_initP", C.c_int),
                 ("dT", C.c_void_p), ("initZ", C.c_void_p), ("initPhi", C.c_void_p),
-                ("P", C.c_void_p), ("CRLB", C.c_void_p), ("LL", C.c_void_p)]
+                ("P", C.c_void_p), ("CRLB", C.c_void_p), ("LL", C.c_void_p),
+                ("nstarts", C.c_int), ("strict", C.c_int)]
 
 
 class Frame(C.Structure):
@@ -98,12 +99,21 @@ def _roi_view(a, nch):
 
 
 class Group:
-    """One group of a frame: ROIs of one model and one size."""
+    """One group of a frame: ROIs of one model and one size.  starts (K, 2)
+    for fittype 7 or (K, 4) for fittype 8 fits every ROI from K starts and
+    keeps, start by start, the result whose log-likelihood is higher by at
+    least 1e-4 (strict: more than 1e-4), as bsfit.fit_multistart does."""
 
-    def __init__(self, model, imstack, iterations=50, initP=None, dT=None, initZ=None, initPhi=None):
+    def __init__(self, model, imstack, iterations=50, initP=None, dT=None, initZ=None, initPhi=None,
+                 starts=None, strict=False):
         self.model = model
         self.imstack = imstack
         self.iterations = int(iterations)
+        self.nstarts, self.strict, self.row = 0, int(bool(strict)), 0
+        if starts is not None:
+            st = np.ascontiguousarray(np.atleast_2d(starts), np.float32)
+            self.nstarts, self.row = st.shape[0], st.shape[1]
+            initP = st
         self.initP = None if initP is None else np.ascontiguousarray(np.ravel(initP), np.float32)
         self.dT = None if dT is None else np.asfortranarray(dT, np.float32)
         self.initZ = None if initZ is None else np.ascontiguousarray(initZ, np.float32)
@@ -222,7 +232,8 @@ class Session:
             r.model, r.sz, r.count, r.iterations = int(g.model), int(S), int(N), g.iterations
             r.data, r.chan_stride = ptr, int(cst)
             if g.initP is not None:
-                r.initP, r.n_initP = g.initP.ctypes.data, int(g.initP.size)
+                r.initP, r.n_initP = g.initP.ctypes.data, int(g.row if g.nstarts else g.initP.size)
+                r.nstarts, r.strict = g.nstarts, g.strict
             r.dT = g.dT.ctypes.data if g.dT is not None else None
             r.initZ = g.initZ.ctypes.data if g.initZ is not None else None
             r.initPhi = g.initPhi.ctypes.data if g.initPhi is not None else None

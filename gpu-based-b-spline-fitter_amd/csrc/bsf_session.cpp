@@ -53,10 +53,12 @@ struct Model {
 
 // One kernel launch of a frame: the groups sharing model, iterations and start.
 constexpr int kMaxKernelStreams = 16;
+constexpr int kMaxGroupStarts = 8;
 
 struct Launch {
     int model = 0, iterations = 0, n = 0, Smax = 0, nch = 1, nv = 0;
-    float initP[4] = {0, 0, 0, 0};
+    float initP[kMaxGroupStarts][4] = {};   // start rows (multi-start: one launch per row)
+    int nstarts = 1, strict = 0;
     int has_initP = 0, has_dT = 0, has_iz = 0, has_ip = 0;
     int wpc = 0;               // 4Pi: one wave per channel
     std::vector<int> groups;   // launch order (largest ROI size first)
@@ -494,19 +496,28 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
         if (use_initP && g.n_initP < need)
             return fail(BSF_ERR_ARG, "bsf: group %d: initP for fittype %d needs %d values (got %d)", gi, M.ft, need,
                         g.n_initP);
+        const int nst = g.nstarts > 1 ? g.nstarts : 1;
+        if (g.nstarts < 0 || nst > kMaxGroupStarts)
+            return fail(BSF_ERR_ARG, "bsf: group %d: nstarts must be 0..%d (got %d)", gi, kMaxGroupStarts, g.nstarts);
+        if (nst > 1 && !use_initP)
+            return fail(BSF_ERR_ARG, "bsf: group %d: several starts need fittype 7 or 8 and initP rows", gi);
         if (M.ft == BSF_FT_4PI && g.chan_stride && g.chan_stride < (size_t)g.count * g.sz * g.sz)
             return fail(BSF_ERR_ARG, "bsf: group %d: channel stride %zu is below count*sz*sz", gi, g.chan_stride);
         Launch key;
         key.model = g.model;
         key.iterations = g.iterations;
         key.has_initP = use_initP ? 1 : 0;
-        for (int i = 0; i < need && use_initP; i++) key.initP[i] = g.initP[i];
+        key.nstarts = nst;
+        key.strict = nst > 1 && g.strict ? 1 : 0;
+        for (int k = 0; k < nst && use_initP; k++)
+            for (int i = 0; i < need; i++) key.initP[k][i] = g.initP[(size_t)k * g.n_initP + i];
         key.has_dT = M.ft == BSF_FT_4PI && g.dT;
         key.has_iz = M.ft == BSF_FT_4PI && g.initZ;
         key.has_ip = M.ft == BSF_FT_4PI && g.initPhi;
         Launch *hit = nullptr;
         for (Launch &l : L)
             if (l.model == key.model && l.iterations == key.iterations && l.has_initP == key.has_initP &&
+                l.nstarts == key.nstarts && l.strict == key.strict &&
                 std::memcmp(l.initP, key.initP, sizeof key.initP) == 0 && l.has_dT == key.has_dT &&
                 l.has_iz == key.has_iz && l.has_ip == key.has_ip) {
                 hit = &l;
@@ -553,7 +564,7 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
             nd += (size_t)g.count * g.sz * g.sz * l.nch;
             cost += (double)g.count * g.sz * g.sz * l.nch * (double)(1 << (2 * (dim_of(M.ft) - 2)));
         }
-        l.cost = cost * (l.iterations + 2);
+        l.cost = cost * (l.iterations + 2) * l.nstarts;
         l.o_desc = o;
         o += up((size_t)l.n * sizeof(bsf::RoiDesc));
         l.o_data = o;
@@ -699,8 +710,13 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
             K.ndim = M.ndim;
             for (int a = 0; a < 5; a++) K.cdims[a] = M.cdims[a];
             K.has_initP = l.has_initP;
-            for (int q = 0; q < 4; q++) K.initP[q] = l.initP[q];
-            e = bsf::launch_fit(K, ks);
+            // multi-start: one launch per start on this stream; after the
+            // first, each keeps the results whose log-likelihood it beats
+            for (int k = 0; k < l.nstarts && e == hipSuccess; k++) {
+                for (int q = 0; q < 4; q++) K.initP[q] = l.initP[k][q];
+                K.keep = k == 0 ? 0 : (l.strict ? 2 : 1);
+                e = bsf::launch_fit(K, ks);
+            }
         }
         if (e != hipSuccess) break;
         hipEvent_t ev = take_event(s);

@@ -250,6 +250,10 @@ typedef struct bsf_roi_group {
     float *P, *CRLB, *LL;       /* host outputs as bsf_result (stride count);
                                    written by bsf_poll / a blocking submit,
                                    must stay valid until the frame completes */
+    int nstarts;                /* fittypes 7, 8: starts in initP (nstarts rows of
+                                   n_initP values), 0 or 1: one; up to 8.  Kept
+                                   as bsf_fit_multistart_* keeps them */
+    int strict;                 /* multi-start rule: 0 LL gain >= 1e-4, 1 > 1e-4 */
 } bsf_roi_group;
 
 typedef struct bsf_frame {

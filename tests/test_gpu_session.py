@@ -186,6 +186,14 @@ def test_session_argument_errors(pools):
         with pytest.raises(bsfit.BsfError) as e:
             s.set_precision(3)
         assert e.value.code == 1
+        with pytest.raises(bsfit.BsfError) as e:        # several starts: fittypes 7 and 8 only
+            s.submit([bsfit.Group(m, w.imstack, 10, starts=np.zeros((2, 2), np.float32))])
+        assert e.value.code == 1
+        m7 = s.register_model(7, gen.model_coeffs("color4d"))
+        w7 = gen.config(1, N=4)
+        with pytest.raises(bsfit.BsfError) as e:        # at most 8 starts per group
+            s.submit([bsfit.Group(m7, w7.imstack, 10, starts=np.ones((9, 2), np.float32))])
+        assert e.value.code == 1
         t = s.submit([])                                # nothing to fit: complete at once
         assert s.poll(t) == []
         with pytest.raises(bsfit.BsfError) as e:
@@ -195,3 +203,28 @@ def test_session_argument_errors(pools):
         with pytest.raises(bsfit.BsfError) as e:
             s.submit([bsfit.Group(m, w.imstack, 10)])
         assert e.value.code == 1
+
+
+def test_multistart_groups_equal_standalone_multistart():
+    """A session group with several starts equals bsf_fit_multistart_host on
+    the same ROIs bit for bit (fittype 7 strict, fittype 8 not), next to a
+    single-start group of the same model in the same frame."""
+    w7 = gen.config(1, N=90)
+    st7 = np.array([[30.5, 1.5], [24.4, 2.5], [36.6, 1.0]], np.float32)
+    w8 = gen.config(3, N=60, S=13)[0]
+    c5, c3 = gen.model_coeffs("dipole5d")
+    st8 = np.array([[10.5, 6.0, 4.5, 0.75], [10.5, 12.0, 4.5, 0.75]], np.float32)
+    with bsfit.Session(arena_bytes=32 << 20) as s:
+        m7 = s.register_model(7, w7.coeff)
+        m8 = s.register_model(8, c5, c3)
+        t = s.submit([bsfit.Group(m7, w7.imstack, 50, starts=st7, strict=True),
+                      bsfit.Group(m7, w7.imstack, 50, initP=st7[0]),
+                      bsfit.Group(m8, w8.imstack, 50, starts=st8)])
+        r = s.poll(t, wait=True)
+    want = [bsfit.fit_multistart(7, w7.imstack, 50, w7.coeff, st7, strict=True),
+            bsfit.fit(7, w7.imstack, 50, w7.coeff, st7[0]),
+            bsfit.fit_multistart(8, w8.imstack, 50, c5, st8, coeff3=c3)]
+    for got, exp in zip(r, want):
+        for a, b in zip(got, exp):
+            np.testing.assert_array_equal(a, b)
+    assert np.any(r[0][2] != r[1][2])          # the later starts changed some ROIs
