@@ -174,3 +174,31 @@ def test_fit_frames_equals_find_then_fit(ft):
         np.testing.assert_array_equal(P, P2)
         np.testing.assert_array_equal(CR, CR2)
         np.testing.assert_array_equal(LL, LL2)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_configurations_match_oracle(seed):
+    """Randomly drawn configurations (frame shape, kernel shape 1..51 on each
+    side, pixel type, first-column minimum, mirror, PSF border, ROI size,
+    cutoff, capacity) against the oracle, bit for bit."""
+    rng = np.random.default_rng(7000 + seed)
+    nr, nc = int(rng.integers(3, 260)), int(rng.integers(3, 260))
+    nf = int(rng.integers(1, 4))
+    dt = np.uint16 if rng.random() < 0.6 else np.float32
+    mr, mc = int(rng.integers(1, 52)), int(rng.integers(1, 52))
+    roi = int(rng.choice(np.arange(3, 52, 2)))
+    sub, mirror = bool(rng.random() < 0.5), bool(rng.random() < 0.3)
+    psf = float(rng.choice([0.0, 0.0, 13.0, 27.0]))
+    cap = int(rng.integers(0, 400))
+    if nr > 2 * 11 + 2 and nc > 2 * 11 + 2:
+        frames, _ = gen.make_frames(nf, nr, nc, max(1, nr * nc // 3000), 8000 + seed, dtype=dt)
+    else:                                       # tiny frames: noise only
+        frames = (rng.poisson(30, (nr, nc, nf)) / 0.5 + 100).astype(dt)
+        frames = np.asfortranarray(frames)
+    h = np.asfortranarray(rng.uniform(-0.02, 0.05, (mr, mc)).astype(np.float32))
+    cutoff = float(rng.uniform(-5, 40))
+    args = dict(cutoff=cutoff, offset=100.0, conversion=0.5, subtract_min=sub, mirror=mirror, psf_size=psf,
+                first_frame=int(rng.integers(0, 5)), capacity=cap)
+    print("seed %d: %dx%dx%d %s kernel %dx%d roi %d sub %d mirror %d psf %g cap %d" %
+          (seed, nr, nc, nf, np.dtype(dt).name, mr, mc, roi, sub, mirror, psf, cap), flush=True)
+    _compare(FR.find_rois(frames, roi, h, **args), oracle.find_rois(frames, roi, h, **args))
