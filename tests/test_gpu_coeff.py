@@ -102,3 +102,22 @@ def test_session_registers_models_from_samples():
         for u, v in zip(a, b):
             np.testing.assert_array_equal(u, v)
     assert np.isfinite(r[0][0]).all()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_shapes_match_oracle(seed):
+    """Randomly drawn ranks (1..5), lengths (2..60: both the staged and the
+    in-place paths) and per-axis lambda (0 or up to 5) against the oracle."""
+    rng = np.random.default_rng(900 + seed)
+    nd = int(rng.integers(1, 6))
+    budget = 400000
+    shape = []
+    for a in range(nd):
+        hi = max(3, min(61, int(budget ** (1.0 / (nd - a)))))
+        d = int(rng.integers(2, hi))
+        shape.append(d)
+        budget = max(8, budget // d)
+    lam = [0.0 if rng.random() < 0.5 else float(rng.uniform(0.01, 5.0)) for _ in shape]
+    x = np.asfortranarray(rng.standard_normal(tuple(shape)).astype(np.float32))
+    print("seed %d shape %s lambda %s" % (seed, tuple(shape), np.round(lam, 3).tolist()), flush=True)
+    _close(bsfit.coeff_build(x, lam), B.bsarray(x.astype(np.float64), lam))
