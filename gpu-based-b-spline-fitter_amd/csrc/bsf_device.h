@@ -485,6 +485,104 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
     }
 }
 
+// Stage B in two passes (fittypes 6, 7).  The xy stencil of pixel (i, j)
+// sums over window columns (tx) and then window rows (ty); the column sums
+// depend only on the pixel column j and the window row, so the first pass
+// forms them once per (j, row) -- shared by the S pixels of column j -- and
+// the second pass applies the row weights per pixel: 16 + 4 multiply-adds
+// per value instead of 16 + 4 per pixel and row tap.  Both passes keep
+// pixel_model's operation order, so every value is the same as there.
+//   h[m * S*Wd + j * Wd + row], Wd = S + 4, in LDS after the planes:
+//   m = 0: sum_tx W[tx] p0;  1: sum_tx Dv[tx] p0;  1 + k: sum_tx W[tx] pk
+// FP32 always uses it (a kernel that also holds the one-pass form needs
+// more than 128 VGPRs for fittype 7); FP64, at 2 waves per SIMD, up to
+// S = 15, where the extra LDS leaves the occupancy as it is; larger FP64
+// ROIs use pixel_model.  (Keeping the sums in registers and writing them over
+// the planes instead needs ~20 more VGPRs for S = 15: a wave per SIMD less.)
+__host__ __device__ inline bool stage_b_split(bool f64, int S) { return !f64 || S <= 15; }
+
+template <typename E, int FT>
+__device__ __forceinline__ void col_sums(const Geo<E, FT> &g, const E *planes, int plane_stride, const int *ctap,
+                                         int Wd, float invWd, int q, E *hv)
+{
+    constexpr int K = Ft<FT>::K;
+    const int PR = g.PR;
+    const Axis<E> &ax = g.ax[1];
+    const int j = (int)(((float)q + 0.5f) * invWd);
+    int a = q - j * Wd;
+    a = a < PR ? a : PR - 1;   // rows past the window: a copy of the last (never read)
+    const E *p0 = planes + ctap[j] * PR + a;
+    E sw = E(0), sd = E(0);
+#pragma unroll
+    for (int tx = 0; tx < 4; tx++) {
+        E gv = p0[tx * PR];
+        sw += ax.W[tx] * gv;
+        sd += ax.Dv[tx] * gv;
+    }
+    hv[0] = sw;
+    hv[1] = sd;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const E *pk = p0 + (k + 1) * plane_stride;
+        E s = E(0);
+#pragma unroll
+        for (int tx = 0; tx < 4; tx++) s += ax.W[tx] * pk[tx * PR];
+        hv[2 + k] = s;
+    }
+}
+
+// First pass.  Every lane must call it; it ends with the sums visible to
+// the wave.
+template <typename E, int FT>
+__device__ __forceinline__ void stage_b_cols(const Geo<E, FT> &g, const E *planes, int plane_stride, const int *ctap,
+                                             int S, float invWd, E *h, int lane)
+{
+    constexpr int NH = 2 + Ft<FT>::K;
+    const int Wd = S + 4, items = S * Wd;
+    for (int q = lane; q < items; q += 64) {
+        E hv[NH];
+        col_sums<E, FT>(g, planes, plane_stride, ctap, Wd, invWd, q, hv);
+#pragma unroll
+        for (int m = 0; m < NH; m++) h[m * items + q] = hv[m];
+    }
+    __syncthreads();
+}
+
+// Second pass: model and dudt at pixel (row r's taps, column j) from the
+// column sums; hj = h + j * Wd, hs = S * Wd.
+template <typename E, int FT>
+__device__ __forceinline__ void pixel_model_h(const Geo<E, FT> &g, const E *hj, int hs, int r, E Nph, E bg, E *dudt,
+                                              E &model)
+{
+    constexpr int K = Ft<FT>::K;
+    const Axis<E> &ay = g.ax[0];
+    const E *h0 = hj + r;
+    E f = E(0), dcol = E(0), drow = E(0);
+#pragma unroll
+    for (int ty = 0; ty < 4; ty++) {
+        const E sw = h0[ty], sd = h0[hs + ty];
+        f += ay.W[ty] * sw;
+        dcol += ay.W[ty] * sd;
+        drow += ay.Dv[ty] * sw;
+    }
+    E dn[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const E *hk = h0 + (2 + k) * hs;
+        E s = E(0);
+#pragma unroll
+        for (int ty = 0; ty < 4; ty++) s += ay.W[ty] * hk[ty];
+        dn[k] = s;
+    }
+    dudt[0] = E(-1) * Nph * dcol;
+    dudt[1] = E(-1) * Nph * drow;
+    dudt[2] = f;
+    dudt[3] = E(1);
+#pragma unroll
+    for (int k = 0; k < K; k++) dudt[4 + k] = Nph * dn[k];
+    model = bg + Nph * f;
+}
+
 // Poisson deviance term of one pixel, 2[(mu - d) - d ln(mu/d)] for d > 0 and
 // 2 mu otherwise (d <= 0 counts as 0, as the reference), without a branch.
 // FP32: mu * (1/d) with 1/d precomputed per pixel and the hardware log2 (the

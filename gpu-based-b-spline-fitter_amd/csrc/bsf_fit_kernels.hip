@@ -26,6 +26,33 @@
 
 namespace bsf {
 
+// Initial x, y, N, bg of one ROI (CPUgaussLib.cpp:172-223, CPUmleFit_LM.cpp:
+// 2478-2482): centre of mass and max / min, summed in the reference's
+// sequential order, each product and sum rounded on its own.
+template <typename E>
+__device__ __forceinline__ void roi_init4(const float *d, int S, E *v)
+{
+#pragma clang fp contract(off)
+    E tx = E(0), ty = E(0), ts = E(0), mx = E(0), mn = E(10e10);
+    for (int ii = 0; ii < S; ii++)
+        for (int jj = 0; jj < S; jj++) {
+            E dd = (E)d[S * jj + ii];
+            tx += dd * (E)ii;
+            ty += dd * (E)jj;
+            ts += dd;
+        }
+    for (int p = 0; p < S * S; p++) {
+        E dd = (E)d[p];
+        mx = (mx > dd) ? mx : dd;
+        mn = (mn < dd) ? mn : dd;
+    }
+    v[0] = tx / ts;
+    v[1] = ty / ts;
+    double t = (double)(((mx - mn) * E(2)) * (E)3.141592) * 1.5 * 1.5;   // PI, definitions.h:15
+    v[2] = (E)((0.0 > t) ? 0.0 : t);
+    v[3] = dmaxr<E>(mn, 0.01);
+}
+
 // ----------------------------------------------------------------- kernel
 template <int FT, typename E>
 __global__ __launch_bounds__(64) void fit_kernel(Dev A)
@@ -47,6 +74,8 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     if (n >= A.N) return;
     const int lane = threadIdx.x;
     const float *dptr = A.data + (size_t)n * A.S * A.S;
+    // the launch's largest ROI size, which the LDS allocation is sized for
+    const int Smax = A.S;
     if (A.rois) {
         const RoiDesc d = A.rois[n];
         A.S = d.S;
@@ -64,7 +93,11 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     // its damped diagonal), so they live in LDS across iterations
     double *s_jh = (double *)(smem + LL.hj);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
-    const float invS = 1.0f / (float)S;
+    // column sums of the two-pass stage B (fittypes 6, 7), after the planes
+    const bool split = FT != 8 && (sizeof(E) == 4 || stage_b_split(true, Smax));
+    E *s_h = s_pl + Ft<FT>::NPL * plane_stride;
+    const int hs = S * W;
+    const float invS = 1.0f / (float)S, invW = 1.0f / (float)W;
 
     for (int p = lane; p < S2; p += 64) {
         const float d = dptr[p];
@@ -80,25 +113,8 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     const int myl = lane < NV ? lane : NV - 1;
     E th;
     {
-        E tx = E(0), ty = E(0), ts = E(0), mx = E(0), mn = E(10e10);
-        for (int ii = 0; ii < S; ii++)
-            for (int jj = 0; jj < S; jj++) {
-                E d = (E)s_data[S * jj + ii];
-                tx += d * (E)ii;
-                ty += d * (E)jj;
-                ts += d;
-            }
-        for (int p = 0; p < S2; p++) {
-            E d = (E)s_data[p];
-            mx = (mx > d) ? mx : d;
-            mn = (mn < d) ? mn : d;
-        }
         E init[8];
-        init[0] = tx / ts;
-        init[1] = ty / ts;
-        double t = (double)(((mx - mn) * E(2)) * (E)3.141592) * 1.5 * 1.5;   // PI, definitions.h:15
-        init[2] = (E)((0.0 > t) ? 0.0 : t);
-        init[3] = dmaxr<E>(mn, 0.01);
+        roi_init4<E>(s_data, S, init);
         init[4] = init[5] = init[6] = init[7] = E(0);
         if constexpr (FT == 6) {
             init[4] = (E)A.nd[2] / E(2);
@@ -157,6 +173,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         make_geo<E, FT>(A, cth, g, lane, gl);
         stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
         __syncthreads();
+        if (split) stage_b_cols<E, FT>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, lane);
         E acc[NRED];
 #pragma unroll
         for (int k = 0; k < NRED; k++) acc[k] = E(0);
@@ -164,7 +181,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             int ii, jj;
             pixel_rc(p, invS, S, ii, jj);
             E dudt[8], model;
-            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
+            if (split)
+                pixel_model_h<E, FT>(g, s_h + jj * W, hs, gl.rtap[ii], cth[2], cth[3], dudt, model);
+            else
+                pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
             if constexpr (FT == 7) {
                 if (first) dudt[5] = dudt[4];
             }
@@ -341,6 +361,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     __syncthreads();
     stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
     __syncthreads();
+    if (split) stage_b_cols<E, FT>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, lane);
     constexpr int NF = 1 + NH;
     E acc[NF];
 #pragma unroll
@@ -349,7 +370,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         int ii, jj;
         pixel_rc(p, invS, S, ii, jj);
         E dudt[8], model;
-        pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
+        if (split)
+            pixel_model_h<E, FT>(g, s_h + jj * W, hs, gl.rtap[ii], cth[2], cth[3], dudt, model);
+        else
+            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
         E data = (E)s_data[p];
         const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
         int h = 1;
@@ -390,20 +414,24 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 
 // ------------------------------------------------------------ launchers
 // planes, shared with the reduction scratch
-static size_t plane_elems(int fittype, int S)
+// planes, the column sums of the two-pass stage B after them, and the
+// reduction scratch over both
+static size_t plane_elems(int fittype, int f64, int S)
 {
     const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
     const int NPL = fittype == 6 ? 2 : fittype == 7 ? 3 : 8;
     const size_t W = S + 4;
     const size_t pl = (size_t)NPL * W * W;
+    const size_t h = fittype == 8 ? 0 : (size_t)(fittype == 6 ? 3 : 4) * S * W;
     const size_t red = (size_t)red_elems(1 + NV + NV * (NV + 1) / 2);
-    return pl > red ? pl : red;
+    const size_t e = stage_b_split(f64, S) ? pl + h : pl;
+    return e > red ? e : red;
 }
 
 size_t lds_bytes(int fittype, int f64, int S)
 {
     const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
-    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, S), (NV + NV * (NV + 1) / 2) * (f64 ? 1 : 2)).total;
+    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, f64, S), (NV + NV * (NV + 1) / 2) * (f64 ? 1 : 2)).total;
 }
 
 template <int FT, typename E>
