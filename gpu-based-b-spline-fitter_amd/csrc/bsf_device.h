@@ -394,6 +394,29 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
 }
 
 // --------------------------------------------------------- stage B + C
+// Fittype 8: blend the 5D model (dudt, model) with the 3D one (f3 and its
+// column, row and z derivatives) by g2 (CPUmleFit_LM.cpp:3318-3322), every
+// product rounded on its own as the reference computes it -- so the one- and
+// the two-pass stage B, whose code leaves the compiler different fusion
+// choices, give the same bits.
+template <typename E>
+__device__ __forceinline__ void blend_5n3(E Nph, E bg, E g2, E f3, E dc3, E dr3, E dz3, E *dudt, E &model)
+{
+#pragma clang fp contract(off)
+    E d3[5];
+    d3[0] = E(-1) * Nph * dc3;
+    d3[1] = E(-1) * Nph * dr3;
+    d3[2] = f3;
+    d3[3] = E(1);
+    d3[4] = Nph * dz3;
+    const E m3 = bg + Nph * f3;
+    const E m5 = model;
+#pragma unroll
+    for (int l = 0; l < 5; l++) dudt[l] = dudt[l] * g2 + (E(1) - g2) * d3[l];
+    model = m3 * (E(1) - g2) + m5 * g2;
+    dudt[7] = m5 - m3;
+}
+
 // Model and dudt at one pixel from the planes.
 template <typename E, int FT>
 __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes, int plane_stride, int r, int c,
@@ -470,36 +493,43 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
             dr3 += ay.Dv[ty] * sw;
             dz3 += ay.W[ty] * sz;
         }
-        E d3[5];
-        d3[0] = E(-1) * Nph * dc3;
-        d3[1] = E(-1) * Nph * dr3;
-        d3[2] = f3;
-        d3[3] = E(1);
-        d3[4] = Nph * dz3;
-        E m3 = bg + Nph * f3;
-        E m5 = model;
-#pragma unroll
-        for (int l = 0; l < 5; l++) dudt[l] = dudt[l] * g2 + (E(1) - g2) * d3[l];
-        model = m3 * (E(1) - g2) + m5 * g2;
-        dudt[7] = m5 - m3;
+        blend_5n3<E>(Nph, bg, g2, f3, dc3, dr3, dz3, dudt, model);
     }
 }
 
-// Stage B in two passes (fittypes 6, 7).  The xy stencil of pixel (i, j)
-// sums over window columns (tx) and then window rows (ty); the column sums
-// depend only on the pixel column j and the window row, so the first pass
-// forms them once per (j, row) -- shared by the S pixels of column j -- and
-// the second pass applies the row weights per pixel: 16 + 4 multiply-adds
-// per value instead of 16 + 4 per pixel and row tap.  Both passes keep
-// pixel_model's operation order, so every value is the same as there.
-//   h[m * S*Wd + j * Wd + row], Wd = S + 4, in LDS after the planes:
-//   m = 0: sum_tx W[tx] p0;  1: sum_tx Dv[tx] p0;  1 + k: sum_tx W[tx] pk
-// FP32 always uses it (a kernel that also holds the one-pass form needs
-// more than 128 VGPRs for fittype 7); FP64, at 2 waves per SIMD, up to
-// S = 15, where the extra LDS leaves the occupancy as it is; larger FP64
-// ROIs use pixel_model.  (Keeping the sums in registers and writing them over
-// the planes instead needs ~20 more VGPRs for S = 15: a wave per SIMD less.)
-__host__ __device__ inline bool stage_b_split(bool f64, int S) { return !f64 || S <= 15; }
+// Stage B in two passes.  The xy stencil of pixel (i, j) sums over window
+// columns (tx) and then window rows (ty); the column sums depend only on the
+// pixel column j and the window row, so the first pass forms them once per
+// (j, row) -- shared by the S pixels of column j -- and the second pass
+// applies the row weights per pixel: 16 + 4 multiply-adds per value instead
+// of 16 + 4 per pixel and row tap.  Both passes keep pixel_model's
+// operation order, so every value is the same as there.
+// Column sums of item (j, row) at h[((j * Wd) + row) * NHP + m], Wd = S + 4,
+// in LDS after the planes (NHP values per item, 16-byte groups):
+//   ft6/7 (NHP 4): W.p0, Dv.p0, W.pk (k = 1..K)      [ft6: one pad]
+//   ft8   (NHP 8): W.p0, Dv.p0, W.p1, W.p2, W.p3, W.p6, Dv.p6, W.p7
+// (W. = sum_tx W[tx] x, Dv. likewise; the irregular 2x2 planes 4, 5 of ft8
+// stay per pixel.)
+// Where it is used: FP32 fittypes 6/7 always (a kernel that also holds the
+// one-pass form needs more than 128 VGPRs for fittype 7); FP64 fittypes 6/7
+// up to S = 15 (2 waves per SIMD: the extra LDS leaves occupancy as it is);
+// FP64 fittype 8 up to S = 13 (1 wave per SIMD; the kernel holding both
+// forms also needs 310 instead of 382 registers, which speeds up every S).
+// FP32 fittype 8 keeps one pass: two gained nothing at S <= 13 and the
+// kernel holding both lost 3 % at S >= 15.  Elsewhere pixel_model does one
+// pass.  (Keeping the sums in registers and writing them over the planes
+// instead needs ~20 more VGPRs for fittype 7 at S = 15: a wave per SIMD less.)
+template <int FT>
+struct ColSums { static constexpr int NHP = FT == 8 ? 8 : 4; };
+
+__host__ __device__ inline bool stage_b_split(int ft, bool f64, int S)
+{
+    if (ft == 8) return f64 && S <= 13;
+    return !f64 || S <= 15;
+}
+
+// LDS elements of the column sums for ROI size S
+__host__ __device__ inline size_t col_sum_elems(int ft, int S) { return (size_t)(ft == 8 ? 8 : 4) * S * (S + 4); }
 
 template <typename E, int FT>
 __device__ __forceinline__ void col_sums(const Geo<E, FT> &g, const E *planes, int plane_stride, const int *ctap,
@@ -512,24 +542,35 @@ __device__ __forceinline__ void col_sums(const Geo<E, FT> &g, const E *planes, i
     int a = q - j * Wd;
     a = a < PR ? a : PR - 1;   // rows past the window: a copy of the last (never read)
     const E *p0 = planes + ctap[j] * PR + a;
-    E sw = E(0), sd = E(0);
+    auto wd = [&](const E *pp, E &sw, E &sd) __attribute__((always_inline)) {
+        sw = E(0);
+        sd = E(0);
 #pragma unroll
-    for (int tx = 0; tx < 4; tx++) {
-        E gv = p0[tx * PR];
-        sw += ax.W[tx] * gv;
-        sd += ax.Dv[tx] * gv;
-    }
-    hv[0] = sw;
-    hv[1] = sd;
+        for (int tx = 0; tx < 4; tx++) {
+            E gv = pp[tx * PR];
+            sw += ax.W[tx] * gv;
+            sd += ax.Dv[tx] * gv;
+        }
+    };
+    auto w = [&](const E *pp) __attribute__((always_inline)) -> E {
+        E sw = E(0);
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const E *pk = p0 + (k + 1) * plane_stride;
-        E s = E(0);
+        for (int tx = 0; tx < 4; tx++) sw += ax.W[tx] * pp[tx * PR];
+        return sw;
+    };
+    wd(p0, hv[0], hv[1]);
 #pragma unroll
-        for (int tx = 0; tx < 4; tx++) s += ax.W[tx] * pk[tx * PR];
-        hv[2 + k] = s;
+    for (int k = 0; k < K; k++) hv[2 + k] = w(p0 + (k + 1) * plane_stride);
+    if constexpr (FT == 8) {
+        wd(p0 + 6 * plane_stride, hv[5], hv[6]);
+        hv[7] = w(p0 + 7 * plane_stride);
+    } else if constexpr (K == 1) {
+        hv[3] = E(0);
     }
 }
+
+template <typename E>
+struct alignas(16) Vec4 { E v[4]; };
 
 // First pass.  Every lane must call it; it ends with the sums visible to
 // the wave.
@@ -537,42 +578,66 @@ template <typename E, int FT>
 __device__ __forceinline__ void stage_b_cols(const Geo<E, FT> &g, const E *planes, int plane_stride, const int *ctap,
                                              int S, float invWd, E *h, int lane)
 {
-    constexpr int NH = 2 + Ft<FT>::K;
+    constexpr int NHP = ColSums<FT>::NHP;
     const int Wd = S + 4, items = S * Wd;
     for (int q = lane; q < items; q += 64) {
-        E hv[NH];
+        E hv[NHP];
         col_sums<E, FT>(g, planes, plane_stride, ctap, Wd, invWd, q, hv);
+        Vec4<E> *dst = (Vec4<E> *)(h + (size_t)q * NHP);
 #pragma unroll
-        for (int m = 0; m < NH; m++) h[m * items + q] = hv[m];
+        for (int m = 0; m < NHP / 4; m++) dst[m] = Vec4<E>{{hv[4 * m], hv[4 * m + 1], hv[4 * m + 2], hv[4 * m + 3]}};
     }
     __syncthreads();
 }
 
 // Second pass: model and dudt at pixel (row r's taps, column j) from the
-// column sums; hj = h + j * Wd, hs = S * Wd.
+// column sums (hj = h + j * Wd * NHP), and for fittype 8 the irregular 2x2
+// planes (pz, pu at the pixel's window origin, row stride 1, column stride PR).
 template <typename E, int FT>
-__device__ __forceinline__ void pixel_model_h(const Geo<E, FT> &g, const E *hj, int hs, int r, E Nph, E bg, E *dudt,
-                                              E &model)
+__device__ __forceinline__ void pixel_model_h(const Geo<E, FT> &g, const E *hj, int r, const E *pz, const E *pu,
+                                              E Nph, E bg, E g2, E *dudt, E &model)
 {
-    constexpr int K = Ft<FT>::K;
+    constexpr int K = Ft<FT>::K, NHP = ColSums<FT>::NHP;
     const Axis<E> &ay = g.ax[0];
-    const E *h0 = hj + r;
+    E hv[4][NHP];
+#pragma unroll
+    for (int ty = 0; ty < 4; ty++) {
+        const Vec4<E> *src = (const Vec4<E> *)(hj + (size_t)(r + ty) * NHP);
+#pragma unroll
+        for (int m = 0; m < NHP / 4; m++) {
+            const Vec4<E> t = src[m];
+#pragma unroll
+            for (int i = 0; i < 4; i++) hv[ty][4 * m + i] = t.v[i];
+        }
+    }
     E f = E(0), dcol = E(0), drow = E(0);
 #pragma unroll
     for (int ty = 0; ty < 4; ty++) {
-        const E sw = h0[ty], sd = h0[hs + ty];
-        f += ay.W[ty] * sw;
-        dcol += ay.W[ty] * sd;
-        drow += ay.Dv[ty] * sw;
+        f += ay.W[ty] * hv[ty][0];
+        dcol += ay.W[ty] * hv[ty][1];
+        drow += ay.Dv[ty] * hv[ty][0];
     }
     E dn[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const E *hk = h0 + (2 + k) * hs;
         E s = E(0);
 #pragma unroll
-        for (int ty = 0; ty < 4; ty++) s += ay.W[ty] * hk[ty];
+        for (int ty = 0; ty < 4; ty++) s += ay.W[ty] * hv[ty][2 + k];
         dn[k] = s;
+    }
+    if constexpr (FT == 8) {
+        const int PR = g.PR;
+        const Axis<E> &ax = g.ax[1];
+        E zb = E(0), ub = E(0);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                zb += ay.W[2 + i] * ax.W[2 + j] * pz[(2 + j) * PR + 2 + i];
+                ub += ay.W[i] * ax.W[j] * pu[j * PR + i];
+            }
+        dn[0] += zb;
+        dn[2] += ub;
     }
     dudt[0] = E(-1) * Nph * dcol;
     dudt[1] = E(-1) * Nph * drow;
@@ -581,6 +646,17 @@ __device__ __forceinline__ void pixel_model_h(const Geo<E, FT> &g, const E *hj, 
 #pragma unroll
     for (int k = 0; k < K; k++) dudt[4 + k] = Nph * dn[k];
     model = bg + Nph * f;
+    if constexpr (FT == 8) {
+        E f3 = E(0), dc3 = E(0), dr3 = E(0), dz3 = E(0);
+#pragma unroll
+        for (int ty = 0; ty < 4; ty++) {
+            f3 += ay.W[ty] * hv[ty][5];
+            dc3 += ay.W[ty] * hv[ty][6];
+            dr3 += ay.Dv[ty] * hv[ty][5];
+            dz3 += ay.W[ty] * hv[ty][7];
+        }
+        blend_5n3<E>(Nph, bg, g2, f3, dc3, dr3, dz3, dudt, model);
+    }
 }
 
 // Poisson deviance term of one pixel, 2[(mu - d) - d ln(mu/d)] for d > 0 and

@@ -47,7 +47,7 @@ __device__ __forceinline__ int packed6(int p, int q)
     return a * 6 - (a * (a - 1)) / 2 + (b - a);
 }
 
-__host__ __device__ inline bool split_4pi(bool f64, int S) { return f64 && stage_b_split(true, S); }
+__host__ __device__ inline bool split_4pi(bool f64, int S) { return f64 && stage_b_split(7, true, S); }
 
 template <typename E, int NVMAX>
 __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
@@ -78,11 +78,11 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     // scratch over both (FP64 only: with FP32 the two-pass form measured 6 %
     // slower here, 3.02 vs 2.84 M fits/s on 200k 4x13x13 ROIs; FP64 8 % faster)
     const bool split = split_4pi(sizeof(E) == 8, Smax);
-    const size_t pls = split ? pl + (size_t)4 * S * W : pl;
+    const size_t pls = split ? pl + col_sum_elems(7, S) + 3 : pl;
     const size_t plred = pls > (size_t)red_elems(28) ? pls : (size_t)red_elems(28);
     const float invS = 1.0f / (float)S, invW = 1.0f / (float)W;
     E *s_red = s_pl;
-    E *s_h = s_pl + pl;
+    E *s_h = s_pl + ((pl + 3) & ~(size_t)3);   // 16-byte groups
     E *s_H = s_pl + plred;          // nv * nv
     E *s_J = s_H + nv * nv;         // nv
     E *s_err = s_J + nv;            // 1
@@ -177,7 +177,8 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
                 pixel_rc(p, invS, S, ii, jj);
                 E dudt[8], model;
                 if (split)
-                    pixel_model_h<E, 7>(g, s_h + jj * W, S * W, gl.rtap[ii], cth[2], cth[3], dudt, model);
+                    pixel_model_h<E, 7>(g, s_h + jj * W * 4, gl.rtap[ii], nullptr, nullptr, cth[2], cth[3], E(0),
+                                        dudt, model);
                 else
                     pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], E(0), dudt,
                                       model);
@@ -290,7 +291,8 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
             pixel_rc(p, invS, S, ii, jj);
             E dudt[8], model;
             if (split)
-                pixel_model_h<E, 7>(g, s_h + jj * W, S * W, gl.rtap[ii], cth[2], cth[3], dudt, model);
+                pixel_model_h<E, 7>(g, s_h + jj * W * 4, gl.rtap[ii], nullptr, nullptr, cth[2], cth[3], E(0), dudt,
+                                    model);
             else
                 pixel_model<E, 7>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], E(0), dudt, model);
             E data = (E)s_data[k * S2 + p];
@@ -677,7 +679,7 @@ size_t lds_bytes_4pi(int f64, int S, int nch, int nv)
 {
     const size_t W = S + 4;
     const size_t pl = 3 * W * W;
-    const size_t pls = split_4pi(f64, S) ? pl + 4 * S * W : pl;
+    const size_t pls = split_4pi(f64, S) ? pl + col_sum_elems(7, S) + 3 : pl;
     const size_t plred = pls > (size_t)red_elems(28) ? pls : (size_t)red_elems(28);
     return lds_layout(S, nch, f64 ? 8 : 4, plred + (size_t)nv * nv + nv + 1).total;
 }

@@ -94,9 +94,9 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     double *s_jh = (double *)(smem + LL.hj);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
     // column sums of the two-pass stage B (fittypes 6, 7), after the planes
-    const bool split = FT != 8 && (sizeof(E) == 4 || stage_b_split(true, Smax));
-    E *s_h = s_pl + Ft<FT>::NPL * plane_stride;
-    const int hs = S * W;
+    const bool split = (FT != 8 && sizeof(E) == 4) || stage_b_split(FT, sizeof(E) == 8, Smax);
+    E *s_h = s_pl + ((Ft<FT>::NPL * plane_stride + 3) & ~3);   // 16-byte groups
+    constexpr int NHP = ColSums<FT>::NHP;
     const float invS = 1.0f / (float)S, invW = 1.0f / (float)W;
 
     for (int p = lane; p < S2; p += 64) {
@@ -181,8 +181,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             int ii, jj;
             pixel_rc(p, invS, S, ii, jj);
             E dudt[8], model;
-            if (split)
-                pixel_model_h<E, FT>(g, s_h + jj * W, hs, gl.rtap[ii], cth[2], cth[3], dudt, model);
+            if (split) {
+                const int r = gl.rtap[ii];
+                const E *pp = s_pl + gl.ctap[jj] * g.PR + r;
+                pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
+                                     cth[3], g2, dudt, model);
+            }
             else
                 pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
             if constexpr (FT == 7) {
@@ -370,8 +374,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         int ii, jj;
         pixel_rc(p, invS, S, ii, jj);
         E dudt[8], model;
-        if (split)
-            pixel_model_h<E, FT>(g, s_h + jj * W, hs, gl.rtap[ii], cth[2], cth[3], dudt, model);
+        if (split) {
+            const int r = gl.rtap[ii];
+            const E *pp = s_pl + gl.ctap[jj] * g.PR + r;
+            pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
+                                 cth[3], g2, dudt, model);
+        }
         else
             pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
         E data = (E)s_data[p];
@@ -422,9 +430,9 @@ static size_t plane_elems(int fittype, int f64, int S)
     const int NPL = fittype == 6 ? 2 : fittype == 7 ? 3 : 8;
     const size_t W = S + 4;
     const size_t pl = (size_t)NPL * W * W;
-    const size_t h = fittype == 8 ? 0 : (size_t)(fittype == 6 ? 3 : 4) * S * W;
+    const size_t h = col_sum_elems(fittype, S) + 3;   // after the planes, 16-byte aligned
     const size_t red = (size_t)red_elems(1 + NV + NV * (NV + 1) / 2);
-    const size_t e = stage_b_split(f64, S) ? pl + h : pl;
+    const size_t e = (fittype != 8 && !f64) || stage_b_split(fittype, f64, S) ? pl + h : pl;
     return e > red ? e : red;
 }
 
