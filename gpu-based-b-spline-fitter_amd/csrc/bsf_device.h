@@ -79,6 +79,7 @@ struct Geo {
     int knx[3];        // base taps of the non-xy axes
     int row0, col0;    // window origin (0-based coefficient index)
     int PR, PC;        // window extent
+    int CS;            // plane column stride (PR, or S + 4 in size-specialised kernels)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
@@ -150,7 +151,8 @@ struct GeoLds {
 // (CPUsplineLib.cpp:1203-1225, per pixel exactly as the reference), and the
 // window spans the taps of all rows / columns.  Every lane must call it.
 template <typename E, int FT>
-__device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> &g, int lane, const GeoLds<E> &L)
+__device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> &g, int lane, const GeoLds<E> &L,
+                                         bool fixed_cs = false)
 {
     constexpr int D = Ft<FT>::D;
     const int S = A.S;
@@ -192,6 +194,9 @@ __device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> 
     g.row0 = rmin - 2;
     g.col0 = cmin - 2;
     g.PR = rmax - rmin + 4;
+    // planes column stride: the window height, or with a compile-time ROI
+    // size its largest value S + 4, so that plane offsets are constants
+    g.CS = fixed_cs ? S + 4 : g.PR;
     g.PC = cmax - cmin + 4;
     if (lane < S) {
         L.rtap[lane] = kr - rmin;
@@ -235,7 +240,7 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int s
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
-// Contract the non-xy axes over the window; planes[p][b*PR + a].
+// Contract the non-xy axes over the window; planes[p][b*CS + a].
 template <typename E, int FT>
 __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *planes, int plane_stride, int lane)
 {
@@ -251,8 +256,10 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
     // FP32 quotient (q + 0.5) / PR stays within 1e-4 of its exact value,
     // which is at least 0.5/PR away from an integer, so the floor is exact
     const float invPR = 1.0f / (float)PR;
+#pragma nounroll
     for (int q = lane; q < npts; q += 64) {
         const int b = (int)(((float)q + 0.5f) * invPR), a = q - b * PR;
+        const int pq = b * g.CS + a;
         const int vo = ((g.row0 + a) + (g.col0 + b) * (int)A.s1) * 4;
         if constexpr (K == 1) {
             const Axis<E> &z = g.ax[2];
@@ -265,8 +272,8 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
                 v += z.W[t] * cv[t];
                 dz += z.Dv[t] * cv[t];
             }
-            planes[q] = v;
-            planes[plane_stride + q] = dz;
+            planes[pq] = v;
+            planes[plane_stride + pq] = dz;
         } else if constexpr (K == 2) {
             const Axis<E> &z = g.ax[2], &w = g.ax[3];
             E v0[4], v1[4];
@@ -291,9 +298,9 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
                 dz += z.Dv[tz] * v0[tz];
                 dw += z.W[tz] * v1[tz];
             }
-            planes[q] = v;
-            planes[plane_stride + q] = dz;
-            planes[2 * plane_stride + q] = dw;
+            planes[pq] = v;
+            planes[plane_stride + pq] = dz;
+            planes[2 * plane_stride + pq] = dw;
         } else {
             // Slice by slice along z (16 taps at a time) to bound live
             // registers.  The reference's 5D contraction has two irregular
@@ -370,12 +377,12 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
 #pragma unroll
                     for (int s = 0; s < 2; s++) prev[r][s] = cs[2 + r][2 + s];
             }
-            planes[q] = v;
-            planes[plane_stride + q] = dz;
-            planes[2 * plane_stride + q] = dw;
-            planes[3 * plane_stride + q] = du;
-            planes[4 * plane_stride + q] = zb;
-            planes[5 * plane_stride + q] = ub;
+            planes[pq] = v;
+            planes[plane_stride + pq] = dz;
+            planes[2 * plane_stride + pq] = dw;
+            planes[3 * plane_stride + pq] = du;
+            planes[4 * plane_stride + pq] = zb;
+            planes[5 * plane_stride + pq] = ub;
             // the 3D model (coeff3) shares the xy window and the z taps
             const __amdgpu_buffer_rsrc_t r3 =
                 __builtin_amdgcn_make_buffer_rsrc((void *)A.coeff3, (short)0, A.c3bytes, 0x00020000);
@@ -387,8 +394,8 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
                 v3 += z.W[t] * cc;
                 dz3 += z.Dv[t] * cc;
             }
-            planes[6 * plane_stride + q] = v3;
-            planes[7 * plane_stride + q] = dz3;
+            planes[6 * plane_stride + pq] = v3;
+            planes[7 * plane_stride + pq] = dz3;
         }
     }
 }
@@ -423,9 +430,9 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
                                             E Nph, E bg, E g2, E *dudt, E &model)
 {
     constexpr int D = Ft<FT>::D;
-    const int PR = g.PR;
+    const int CS = g.CS;   // plane column stride
     const Axis<E> &ay = g.ax[0], &ax = g.ax[1];
-    const E *p0 = planes + c * PR + r;
+    const E *p0 = planes + c * CS + r;
     // value plane: f, d/dcol, d/drow
     E f = E(0), dcol = E(0), drow = E(0);
 #pragma unroll
@@ -433,7 +440,7 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
         E sw = E(0), sd = E(0);
 #pragma unroll
         for (int tx = 0; tx < 4; tx++) {
-            E gv = p0[tx * PR + ty];
+            E gv = p0[tx * CS + ty];
             sw += ax.W[tx] * gv;
             sd += ax.Dv[tx] * gv;
         }
@@ -450,7 +457,7 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
         for (int ty = 0; ty < 4; ty++) {
             E sw = E(0);
 #pragma unroll
-            for (int tx = 0; tx < 4; tx++) sw += ax.W[tx] * pk[tx * PR + ty];
+            for (int tx = 0; tx < 4; tx++) sw += ax.W[tx] * pk[tx * CS + ty];
             s += ay.W[ty] * sw;
         }
         dn[k] = s;
@@ -462,8 +469,8 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                zb += ay.W[2 + i] * ax.W[2 + j] * pz[(2 + j) * PR + 2 + i];
-                ub += ay.W[i] * ax.W[j] * pu[j * PR + i];
+                zb += ay.W[2 + i] * ax.W[2 + j] * pz[(2 + j) * CS + 2 + i];
+                ub += ay.W[i] * ax.W[j] * pu[j * CS + i];
             }
         dn[0] += zb;
         dn[2] += ub;
@@ -483,10 +490,10 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
             E sw = E(0), sd = E(0), sz = E(0);
 #pragma unroll
             for (int tx = 0; tx < 4; tx++) {
-                E gv = q0[tx * PR + ty];
+                E gv = q0[tx * CS + ty];
                 sw += ax.W[tx] * gv;
                 sd += ax.Dv[tx] * gv;
-                sz += ax.W[tx] * q1[tx * PR + ty];
+                sz += ax.W[tx] * q1[tx * CS + ty];
             }
             f3 += ay.W[ty] * sw;
             dc3 += ay.W[ty] * sd;
@@ -541,13 +548,14 @@ __device__ __forceinline__ void col_sums(const Geo<E, FT> &g, const E *planes, i
     const int j = (int)(((float)q + 0.5f) * invWd);
     int a = q - j * Wd;
     a = a < PR ? a : PR - 1;   // rows past the window: a copy of the last (never read)
-    const E *p0 = planes + ctap[j] * PR + a;
+    const int CS = g.CS;
+    const E *p0 = planes + ctap[j] * CS + a;
     auto wd = [&](const E *pp, E &sw, E &sd) __attribute__((always_inline)) {
         sw = E(0);
         sd = E(0);
 #pragma unroll
         for (int tx = 0; tx < 4; tx++) {
-            E gv = pp[tx * PR];
+            E gv = pp[tx * CS];
             sw += ax.W[tx] * gv;
             sd += ax.Dv[tx] * gv;
         }
@@ -555,7 +563,7 @@ __device__ __forceinline__ void col_sums(const Geo<E, FT> &g, const E *planes, i
     auto w = [&](const E *pp) __attribute__((always_inline)) -> E {
         E sw = E(0);
 #pragma unroll
-        for (int tx = 0; tx < 4; tx++) sw += ax.W[tx] * pp[tx * PR];
+        for (int tx = 0; tx < 4; tx++) sw += ax.W[tx] * pp[tx * CS];
         return sw;
     };
     wd(p0, hv[0], hv[1]);
@@ -580,6 +588,7 @@ __device__ __forceinline__ void stage_b_cols(const Geo<E, FT> &g, const E *plane
 {
     constexpr int NHP = ColSums<FT>::NHP;
     const int Wd = S + 4, items = S * Wd;
+#pragma nounroll
     for (int q = lane; q < items; q += 64) {
         E hv[NHP];
         col_sums<E, FT>(g, planes, plane_stride, ctap, Wd, invWd, q, hv);
@@ -592,7 +601,7 @@ __device__ __forceinline__ void stage_b_cols(const Geo<E, FT> &g, const E *plane
 
 // Second pass: model and dudt at pixel (row r's taps, column j) from the
 // column sums (hj = h + j * Wd * NHP), and for fittype 8 the irregular 2x2
-// planes (pz, pu at the pixel's window origin, row stride 1, column stride PR).
+// planes (pz, pu at the pixel's window origin, row stride 1, column stride CS).
 template <typename E, int FT>
 __device__ __forceinline__ void pixel_model_h(const Geo<E, FT> &g, const E *hj, int r, const E *pz, const E *pu,
                                               E Nph, E bg, E g2, E *dudt, E &model)
@@ -626,15 +635,15 @@ __device__ __forceinline__ void pixel_model_h(const Geo<E, FT> &g, const E *hj, 
         dn[k] = s;
     }
     if constexpr (FT == 8) {
-        const int PR = g.PR;
+        const int CS = g.CS;   // plane column stride
         const Axis<E> &ax = g.ax[1];
         E zb = E(0), ub = E(0);
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                zb += ay.W[2 + i] * ax.W[2 + j] * pz[(2 + j) * PR + 2 + i];
-                ub += ay.W[i] * ax.W[j] * pu[j * PR + i];
+                zb += ay.W[2 + i] * ax.W[2 + j] * pz[(2 + j) * CS + 2 + i];
+                ub += ay.W[i] * ax.W[j] * pu[j * CS + i];
             }
         dn[0] += zb;
         dn[2] += ub;

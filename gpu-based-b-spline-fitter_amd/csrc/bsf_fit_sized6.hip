@@ -1,0 +1,19 @@
+// bsf_fit_sized6.hip -- fittype-6 FP32 fit kernels specialised on the ROI
+// size (see sized_kernel in bsf_fit_kernel.h); a translation unit of their
+// own so that they compile in parallel with the generic kernels.
+#include "bsf_fit_kernel.h"
+
+namespace bsf {
+
+hipError_t launch_sized6(const Dev &A, int nblocks, hipStream_t st, int S)
+{
+    switch (S) {
+    case 11: return launch_t<6, float, 11>(A, nblocks, st);
+    case 13: return launch_t<6, float, 13>(A, nblocks, st);
+    case 15: return launch_t<6, float, 15>(A, nblocks, st);
+    case 17: return launch_t<6, float, 17>(A, nblocks, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace bsf

@@ -28,6 +28,7 @@ struct FitLaunch {
     int roi_offset, count;  // ROIs [roi_offset, roi_offset + count)
     const int *ncount;      // device: only ROIs below this count are fitted (outputs keep stride N), or NULL
     const RoiDesc *rois;    // device, per-ROI placement, or NULL
+    int uniform;            // with rois: every ROI has size S
     int cdims[5], ndim;
     float initP[4];
     int has_initP;

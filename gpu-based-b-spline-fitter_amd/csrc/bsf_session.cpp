@@ -56,7 +56,7 @@ constexpr int kMaxKernelStreams = 16;
 constexpr int kMaxGroupStarts = 8;
 
 struct Launch {
-    int model = 0, iterations = 0, n = 0, Smax = 0, nch = 1, nv = 0;
+    int model = 0, iterations = 0, n = 0, Smax = 0, Smin = 0, nch = 1, nv = 0;
     float initP[kMaxGroupStarts][4] = {};   // start rows (multi-start: one launch per row)
     int nstarts = 1, strict = 0;
     int has_initP = 0, has_dT = 0, has_iz = 0, has_ip = 0;
@@ -534,6 +534,7 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
         hit->groups.push_back(gi);
         hit->n += g.count;
         hit->Smax = std::max(hit->Smax, g.sz);
+        hit->Smin = hit->Smin ? std::min(hit->Smin, g.sz) : g.sz;
     }
 
     // ---- layout of the slot: inputs, then outputs
@@ -707,6 +708,7 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
             K.roi_offset = 0;
             K.count = l.n;
             K.rois = d_desc;
+            K.uniform = l.Smin == l.Smax;
             K.ndim = M.ndim;
             for (int a = 0; a < 5; a++) K.cdims[a] = M.cdims[a];
             K.has_initP = l.has_initP;
