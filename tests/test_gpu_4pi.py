@@ -21,12 +21,20 @@ def _cmp(P, CR, Po, CRo, NV):
     return float((ok_p & ok_kk & ok_cr).mean()), float(ok_p.mean())
 
 
-@pytest.mark.parametrize("shared", [(1, 1, 1, 1, 1, 1), (1, 1, 1, 0, 1, 1)])
-def test_4pi_matches_oracle(orc, shared):
+# (shared, S, N, min FP64 agreement, min FP64 parameter agreement); S = 25
+# and the maximum 51 run the generic path with the largest LDS footprints
+# (4 channels of data), the model's 27 px xy grid clamping most taps
+@pytest.mark.parametrize("shared,S,N,a64_min,p64_min", [
+    ((1, 1, 1, 1, 1, 1), 13, 96, 0.3, 0.5),
+    ((1, 1, 1, 0, 1, 1), 13, 96, 0.3, 0.5),
+    ((1, 1, 1, 1, 1, 1), 25, 16, 0.2, 0.4),   # measured 0.50 / 0.69
+    ((1, 1, 1, 0, 1, 1), 51, 8, 0.2, 0.5),    # measured 0.50 / 1.00
+])
+def test_4pi_matches_oracle(orc, shared, S, N, a64_min, p64_min):
     import bsfit
     from bsfit import gen
-    w = gen.make_rois_4pi(96, 13, 4242, gen.model_coeffs("4pi"), shared=shared)
-    print("case 4Pi shared=%s N=%d" % (shared, w.N), flush=True)
+    w = gen.make_rois_4pi(N, S, 4242, gen.model_coeffs("4pi"), shared=shared)
+    print("case 4Pi shared=%s S=%d N=%d" % (shared, S, w.N), flush=True)
     NV = bsfit.num_params_4pi(shared, 4)
     assert NV == (6 if all(shared) else 9)
     args = (w.imstack, shared, 50, w.coeff, w.phaseShift)
@@ -41,7 +49,7 @@ def test_4pi_matches_oracle(orc, shared):
     print("agree F64 %.3f (P %.3f)  MIX %.3f (P %.3f)  oracle F32 vs F64 %.3f (P %.3f)"
           % (a64, p64, amx, pmx, aref, pref), flush=True)
     assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0]))
-    assert a64 >= 0.3 and p64 >= 0.5
+    assert a64 >= a64_min and p64 >= p64_min
     t = w.truth
     eg = np.median(np.abs(gm[0][:, :2] - np.stack([t["x"], t["y"]], 1)), 0)
     eo = np.median(np.abs(of[0][:, :2] - np.stack([t["x"], t["y"]], 1)), 0)
