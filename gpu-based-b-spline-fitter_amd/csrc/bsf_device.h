@@ -137,6 +137,39 @@ __host__ __device__ inline LdsLayout lds_layout(int S, int nch, size_t esz, size
     return L;
 }
 
+// LDS layout of the fit kernel (fittypes 6-8, one ROI per wave): [ROI data,]
+// 1/data per pixel (FP32 evaluation only), tap tables, J / H, then the
+// planes.  Fittypes 6 / 7 keep the ROI data in global memory (each pixel pass
+// reads its pixel; they stay cached while the wave fits the ROI), and the
+// per-axis factors and the ROI copy the initial values are computed from live
+// in the plane region (both are read back before stage A writes the planes):
+// for fittype 7 at S = 15 this is 10.2 KB, so 16 waves fit a CU (11.0 KB: 14).
+// Fittype 8 runs 2 waves per SIMD (registers), so LDS does not bind it and
+// it reads the data from LDS (global reads measured 2 % slower).
+__host__ __device__ constexpr bool fit_data_in_lds(int ft) { return ft == 8; }
+
+__host__ __device__ inline LdsLayout fit_lds_layout(int S, size_t esz, size_t pl_elems, size_t hj_elems,
+                                                    bool data_in_lds)
+{
+    LdsLayout L;
+    size_t o = data_in_lds ? lds_align16((size_t)S * S * 4) : 0;
+    L.rdat = o;
+    o = lds_align16(o + (esz == 4 ? (size_t)S * S * 4 : 0));
+    L.rtap = o;
+    o = lds_align16(o + (size_t)S * 4);
+    L.ctap = o;
+    o = lds_align16(o + (size_t)S * 4);
+    L.hj = o;
+    o = lds_align16(o + hj_elems * esz);
+    L.pl = o;
+    L.fac = o;
+    size_t need = pl_elems * esz;
+    need = need > 40 * esz ? need : 40 * esz;
+    need = need > (size_t)S * S * 4 ? need : (size_t)S * S * 4;
+    L.total = o + need;
+    return L;
+}
+
 // Per-wave LDS workspace of one evaluation's geometry.
 template <typename E>
 struct GeoLds {

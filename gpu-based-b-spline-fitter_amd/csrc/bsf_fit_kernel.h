@@ -37,6 +37,9 @@ __device__ __forceinline__ void roi_init4(const float *d, int S, E *v)
 {
 #pragma clang fp contract(off)
     E tx = E(0), ty = E(0), ts = E(0), mx = E(0), mn = E(10e10);
+    // partly unrolled only: fully unrolled (a compile-time S) these loops
+    // hold every loaded pixel in a register
+#pragma unroll 2
     for (int ii = 0; ii < S; ii++)
         for (int jj = 0; jj < S; jj++) {
             E dd = (E)d[S * jj + ii];
@@ -44,6 +47,7 @@ __device__ __forceinline__ void roi_init4(const float *d, int S, E *v)
             ty += dd * (E)jj;
             ts += dd;
         }
+#pragma unroll 8
     for (int p = 0; p < S * S; p++) {
         E dd = (E)d[p];
         mx = (mx > dd) ? mx : dd;
@@ -89,8 +93,14 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     if (SC) A.S = SC;
     const int S = A.S, S2 = S * S;
     const int W = S + 4, plane_stride = W * W;
-    const LdsLayout LL = lds_layout(S, 1, sizeof(E), 0, (NV + NH) * 8 / sizeof(E));
-    float *s_data = (float *)smem;
+    constexpr bool DL = fit_data_in_lds(FT);
+    const LdsLayout LL = fit_lds_layout(S, sizeof(E), 0, (NV + NH) * 8 / sizeof(E), DL);
+    // ROI data in LDS (fittype 8), or a copy for the initial values only
+    float *s_data = DL ? (float *)smem : (float *)(smem + LL.pl);
+    // the pixel passes read the data through a buffer descriptor (32-bit
+    // offsets, no 64-bit address arithmetic per load)
+    const __amdgpu_buffer_rsrc_t rsd =
+        __builtin_amdgcn_make_buffer_rsrc((void *)dptr, (short)0, S2 * 4, 0x00020000);
     float *s_rdat = (float *)(smem + LL.rdat);
     const GeoLds<E> gl = {(E *)(smem + LL.fac), (int *)(smem + LL.rtap), (int *)(smem + LL.ctap)};
     E *s_pl = (E *)(smem + LL.pl);
@@ -200,7 +210,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
                 if (first) dudt[5] = dudt[4];
             }
             E data;
-            acc[0] += deviance_term<E>(model, (E)s_data[p], s_rdat[p], data);
+            acc[0] += deviance_term<E>(model, DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0), s_rdat[p], data);
             const E rm = recip<E>(model);
             const E dr = data * rm;
             E t1 = E(1) - dr;
@@ -390,7 +400,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         }
         else
             pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
-        E data = (E)s_data[p];
+        E data = DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0);
         const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
         int h = 1;
 #pragma unroll
@@ -440,9 +450,10 @@ inline hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
 // FP32 fittypes 6 / 7 where every ROI of a launch has one of the common
 // sizes: a kernel specialised on it, whose size-dependent index arithmetic
 // and LDS offsets fold to constants (4.6 % faster on 100k 15x15 fittype-7
-// ROIs, 7-11 % on fittype 6).  Fittype 7 at S = 11 stays generic:
-// specialised, it needs 131 VGPRs (3 waves per SIMD instead of 4).
-inline bool sized_kernel(int ft, int S) { return S == 13 || S == 15 || S == 17 || (ft == 6 && S == 11); }
+// ROIs, 7-11 % on fittype 6).  (The initial-value loops are only partly
+// unrolled: fully unrolled at a compile-time S they held every pixel in a
+// register, up to 234 VGPRs.)
+inline bool sized_kernel(int, int S) { return S == 11 || S == 13 || S == 15 || S == 17; }
 // Launch the specialised kernel for S (sized_kernel(ft, S) must hold);
 // in bsf_fit_sized6.hip / bsf_fit_sized7.hip.
 hipError_t launch_sized6(const Dev &A, int nblocks, hipStream_t st, int S);

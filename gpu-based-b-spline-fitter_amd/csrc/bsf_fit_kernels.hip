@@ -26,7 +26,9 @@ static size_t plane_elems(int fittype, int f64, int S)
 size_t lds_bytes(int fittype, int f64, int S)
 {
     const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
-    return lds_layout(S, 1, f64 ? 8 : 4, plane_elems(fittype, f64, S), (NV + NV * (NV + 1) / 2) * (f64 ? 1 : 2)).total;
+    return fit_lds_layout(S, f64 ? 8 : 4, plane_elems(fittype, f64, S), (NV + NV * (NV + 1) / 2) * (f64 ? 1 : 2),
+                          fit_data_in_lds(fittype))
+        .total;
 }
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st)

@@ -8,6 +8,7 @@ namespace bsf {
 hipError_t launch_sized7(const Dev &A, int nblocks, hipStream_t st, int S)
 {
     switch (S) {
+    case 11: return launch_t<7, float, 11>(A, nblocks, st);
     case 13: return launch_t<7, float, 13>(A, nblocks, st);
     case 15: return launch_t<7, float, 15>(A, nblocks, st);
     case 17: return launch_t<7, float, 17>(A, nblocks, st);
