@@ -193,35 +193,43 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         E acc[NRED];
 #pragma unroll
         for (int k = 0; k < NRED; k++) acc[k] = E(0);
-        #pragma nounroll   // with a fixed size, unrolling costs registers (a wave per SIMD)
-        for (int p = lane; p < S2; p += 64) {
-            int ii, jj;
-            pixel_rc(p, invS, S, ii, jj);
-            E dudt[8], model;
-            if (split) {
-                const int r = gl.rtap[ii];
-                const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
-                pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
-                                     cth[3], g2, dudt, model);
-            }
-            else
-                pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
-            if constexpr (FT == 7) {
-                if (first) dudt[5] = dudt[4];
-            }
-            E data;
-            acc[0] += deviance_term<E>(model, DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0), s_rdat[p], data);
-            const E rm = recip<E>(model);
-            const E dr = data * rm;
-            E t1 = E(1) - dr;
-            E t2 = dr * rm;
+        // with a compile-time size every lane has a first pixel (S * S > 64), so
+        // the loop needs no entry test (with one, the compiler zeroes the
+        // accumulators a second time under the entry mask); not unrolled, which
+        // with a fixed size would cost registers (a wave per SIMD)
+        int p = lane;
+        if (SC || p < S2) {
+#pragma nounroll
+            do {
+                int ii, jj;
+                pixel_rc(p, invS, S, ii, jj);
+                E dudt[8], model;
+                if (split) {
+                    const int r = gl.rtap[ii];
+                    const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
+                    pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
+                                         cth[3], g2, dudt, model);
+                }
+                else
+                    pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
+                if constexpr (FT == 7) {
+                    if (first) dudt[5] = dudt[4];
+                }
+                E data;
+                acc[0] += deviance_term<E>(model, DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0), s_rdat[p], data);
+                const E rm = recip<E>(model);
+                const E dr = data * rm;
+                E t1 = E(1) - dr;
+                E t2 = dr * rm;
 #pragma unroll
-            for (int l = 0; l < NV; l++) acc[1 + l] += t1 * dudt[l];
-            int h = 1 + NV;
+                for (int l = 0; l < NV; l++) acc[1 + l] += t1 * dudt[l];
+                int h = 1 + NV;
 #pragma unroll
-            for (int l = 0; l < NV; l++)
+                for (int l = 0; l < NV; l++)
 #pragma unroll
-                for (int m = l; m < NV; m++) acc[h++] += t2 * dudt[l] * dudt[m];
+                    for (int m = l; m < NV; m++) acc[h++] += t2 * dudt[l] * dudt[m];
+                p += 64;
+            } while (p < S2);
         }
         __syncthreads();   // planes are dead; the scratch reuses them
         wave_sum_lds<E, NRED>(acc, s_red, lane);
@@ -387,30 +395,38 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     E acc[NF];
 #pragma unroll
     for (int k = 0; k < NF; k++) acc[k] = E(0);
-    #pragma nounroll   // with a fixed size, unrolling costs registers (a wave per SIMD)
-    for (int p = lane; p < S2; p += 64) {
-        int ii, jj;
-        pixel_rc(p, invS, S, ii, jj);
-        E dudt[8], model;
-        if (split) {
-            const int r = gl.rtap[ii];
-            const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
-            pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
-                                 cth[3], g2, dudt, model);
-        }
-        else
-            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
-        E data = DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0);
-        const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
-        int h = 1;
+    // with a compile-time size every lane has a first pixel (S * S > 64), so
+    // the loop needs no entry test (with one, the compiler zeroes the
+    // accumulators a second time under the entry mask); not unrolled, which
+    // with a fixed size would cost registers (a wave per SIMD)
+    int p = lane;
+    if (SC || p < S2) {
+#pragma nounroll
+        do {
+            int ii, jj;
+            pixel_rc(p, invS, S, ii, jj);
+            E dudt[8], model;
+            if (split) {
+                const int r = gl.rtap[ii];
+                const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
+                pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
+                                     cth[3], g2, dudt, model);
+            }
+            else
+                pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
+            E data = DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0);
+            const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
+            int h = 1;
 #pragma unroll
-        for (int a = 0; a < NV; a++)
+            for (int a = 0; a < NV; a++)
 #pragma unroll
-            for (int b = a; b < NV; b++) acc[h++] += dudt[b] * dudt[a] * rmf;
-        if (model > E(0)) {
-            if (data > E(0)) acc[0] += data * log(model) - model - data * log(data) + data;
-            else acc[0] += -model;
-        }
+                for (int b = a; b < NV; b++) acc[h++] += dudt[b] * dudt[a] * rmf;
+            if (model > E(0)) {
+                if (data > E(0)) acc[0] += data * log(model) - model - data * log(data) + data;
+                else acc[0] += -model;
+            }
+            p += 64;
+        } while (p < S2);
     }
     __syncthreads();
     wave_sum_lds<E, NF>(acc, s_red, lane);
