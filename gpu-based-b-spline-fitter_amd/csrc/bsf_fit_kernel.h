@@ -109,12 +109,14 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     // its damped diagonal), so they live in LDS across iterations
     double *s_jh = (double *)(smem + LL.hj);
     E *s_red = s_pl;   // reductions reuse the plane storage (never live together)
-    // column sums of the two-pass stage B (fittypes 6, 7), after the planes
+    // column sums of the two-pass stage B (FP32 fittypes 6 / 7; FP64 where
+    // stage_b_split allows it for the launch's largest size), after the planes
     const bool split = (FT != 8 && sizeof(E) == 4) || stage_b_split(FT, sizeof(E) == 8, Smax);
     E *s_h = s_pl + ((Ft<FT>::NPL * plane_stride + 3) & ~3);   // 16-byte groups
     constexpr int NHP = ColSums<FT>::NHP;
     const float invS = 1.0f / (float)S, invW = 1.0f / (float)W;
 
+    // stage the ROI (fittypes 6 / 7: only for the initial values) and 1/data
     for (int p = lane; p < S2; p += 64) {
         const float d = dptr[p];
         s_data[p] = d;
