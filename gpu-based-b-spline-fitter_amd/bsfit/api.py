@@ -100,9 +100,27 @@ def _request(fittype, S, N, iterations, coeff_ptr, coeff_shape, coeff3_ptr, init
     return r
 
 
+# per-ROI status flags (enum bsf_roi_flag)
+ROI_MAX_ITER = 1
+ROI_SOLVE_FAILED = 2
+ROI_FISHER_SINGULAR = 4
+ROI_NONFINITE = 8
+
+
 def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=PREC_MIX):
     """Fit an ROI stack (S, S, N) with host arrays; returns (P, CRLB, LL) with
     P (N, NV+1) and CRLB (N, NV) in Fortran order, as the MEX returns them."""
+    return _fit(fittype, imstack, iterations, coeff, initP, coeff3, precision, False)
+
+
+def fit_status(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=PREC_MIX):
+    """fit() plus the per-ROI status flags (bsf_fit_host_status): returns
+    (P, CRLB, LL, status) with status (N,) uint8, bits ROI_MAX_ITER,
+    ROI_SOLVE_FAILED, ROI_FISHER_SINGULAR, ROI_NONFINITE."""
+    return _fit(fittype, imstack, iterations, coeff, initP, coeff3, precision, True)
+
+
+def _fit(fittype, imstack, iterations, coeff, initP, coeff3, precision, with_status):
     lib = load()
     imstack = _f32(imstack)
     if imstack.ndim == 2:
@@ -122,8 +140,14 @@ def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=
     req = _request(fittype, S, N, iterations, coeff.ctypes.data, coeff.shape,
                    c3.ctypes.data if c3 is not None else None, ip, precision, imstack.ctypes.data)
     res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
-    _check(lib.bsf_fit_host(C.byref(req), C.byref(res)))
-    return P, CR, LL
+    if not with_status:
+        _check(lib.bsf_fit_host(C.byref(req), C.byref(res)))
+        return P, CR, LL
+    st = np.zeros(N, np.uint8)
+    lib.bsf_fit_host_status.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_void_p]
+    lib.bsf_fit_host_status.restype = C.c_int
+    _check(lib.bsf_fit_host_status(C.byref(req), C.byref(res), st.ctypes.data))
+    return P, CR, LL, st
 
 
 def fit_multistart(fittype, imstack, iterations, coeff, starts, coeff3=None, strict=False, precision=PREC_MIX):
@@ -155,14 +179,21 @@ def fit_multistart(fittype, imstack, iterations, coeff, starts, coeff3=None, str
 
 
 def fit_device(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_ptr, CRLB_ptr, LL_ptr,
-               initP=None, coeff3_ptr=None, precision=PREC_MIX, stream=None):
+               initP=None, coeff3_ptr=None, precision=PREC_MIX, stream=None, status_ptr=None):
     """Queue a fit on device-resident buffers (raw device addresses, e.g.
-    torch tensor .data_ptr()) on a HIP stream handle; returns immediately."""
+    torch tensor .data_ptr()) on a HIP stream handle; returns immediately.
+    status_ptr: a device array of N bytes for the per-ROI status flags
+    (bsf_fit_device_status), or None."""
     lib = load()
     ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
     req = _request(fittype, S, N, iterations, coeff_ptr, coeff_shape, coeff3_ptr, ip, precision, data_ptr)
     res = Result(P_ptr, CRLB_ptr, LL_ptr)
-    _check(lib.bsf_fit_device(C.byref(req), C.byref(res), stream))
+    if status_ptr is None:
+        _check(lib.bsf_fit_device(C.byref(req), C.byref(res), stream))
+        return
+    lib.bsf_fit_device_status.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_void_p, C.c_void_p]
+    lib.bsf_fit_device_status.restype = C.c_int
+    _check(lib.bsf_fit_device_status(C.byref(req), C.byref(res), status_ptr, stream))
 
 
 def GPUmleFit_LM_multiD(imstack, fittype, iterations=50, coeff=None, varmap=None, silent=1,

@@ -103,6 +103,7 @@ struct Dev {
     int cbytes, c3bytes;   // coefficient array sizes (buffer record counts)
     const float *theta_in; // test hook: evaluate once at these parameters
     float *dbg;            // test hook output: err, J[NV], H[NV*NV] per ROI
+    unsigned char *status; // per-ROI status flags (bsf_roi_flag), or NULL
     float initP[4];
     int has_initP;
     int keep;              // 0: write the results; 1 / 2: write them only where the log-likelihood

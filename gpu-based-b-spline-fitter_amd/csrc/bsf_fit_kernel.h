@@ -257,7 +257,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     E newErr = evaluate(true);
     E oldErr = (E)1e13;
     E newLambda = E(1), oldLambda = E(1), sumUpdate = (E)1e10;
-    int turn = 1, kk;
+    int turn = 1, kk, lastinfo = 0;
     for (kk = 0; kk < A.iterations; kk++) {
         if (fabs((newErr - oldErr) / newErr) < (E)1e-6 || (FT == 8 && sumUpdate < (E)1e-5)) break;   // TOLERANCE, TOLERANCE1
         if ((double)newErr > 1.5 * (double)oldErr) {
@@ -291,6 +291,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
                     if (isnan((E)xs[i])) info = 1;
             }
         }
+        lastinfo = info;
         if (info != 0) {
             newLambda = E(10) * newLambda;
             continue;
@@ -453,6 +454,21 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     if (lane == 0) {
         A.P[(size_t)A.N * NV + n] = (float)kk;
         A.LL[n] = llf;
+    }
+    if (A.status) {
+        // per-ROI status flags (bsf_roi_flag in bsfit.h): iteration limit,
+        // failed last solve, singular Fisher information, non-finite output
+        const float cf = (float)crlb, tf = (float)th;
+        const unsigned long long badc = __ballot(lane < NV && !(cf > 0.f && isfinite(cf)));
+        const unsigned long long badp = __ballot(lane < NV && !isfinite(tf));
+        if (lane == 0) {
+            unsigned f = 0;
+            if (kk >= A.iterations) f |= 1u;
+            if (lastinfo != 0) f |= 2u;
+            if (badc) f |= 4u;
+            if (badp || !isfinite(llf)) f |= 8u;
+            A.status[n] = (unsigned char)f;
+        }
     }
 }
 

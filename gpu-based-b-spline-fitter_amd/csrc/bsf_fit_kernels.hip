@@ -62,6 +62,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     A.keep = L.keep;
     A.theta_in = L.theta_in;
     A.dbg = L.dbg;
+    A.status = L.status;
     const int nb = L.count;
     const int sized = !L.rois || L.uniform ? L.S : 0;
     switch (L.fittype * 2 + (L.f64 ? 1 : 0)) {

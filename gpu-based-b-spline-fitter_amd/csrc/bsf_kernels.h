@@ -35,6 +35,7 @@ struct FitLaunch {
     int keep;               // multi-start: 0 write, 1 keep if LL gains >= 1e-4, 2 if > 1e-4
     const float *theta_in;  // test hook (device): NV parameters per ROI, or NULL
     float *dbg;             // test hook (device): err, J, H per ROI
+    unsigned char *status;  // device: per-ROI status flags (bsf_roi_flag), or NULL
 };
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st);

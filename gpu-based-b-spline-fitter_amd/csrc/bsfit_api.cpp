@@ -162,24 +162,38 @@ int bsf_num_params(int fittype) { return nv_of(fittype); }
 
 const char *bsf_last_error(void) { return g_err.c_str(); }
 
-int bsf_fit_device(const bsf_request *r, bsf_result *res, void *stream)
+static int fit_device(const bsf_request *r, bsf_result *res, unsigned char *status, void *stream)
 {
-    g_err.clear();
     int rc = validate(r, res);
     if (rc) return rc;
     if ((rc = check_device(r->sz, r->fittype, r->precision == BSF_PREC_F64))) return rc;
     if (r->nfits == 0) return BSF_OK;
     bsf::FitLaunch L = make_launch(r, r->imstack, r->coeff, r->coeff3, res->P, res->CRLB, res->LL);
+    L.status = status;
     hipError_t e = launch_all(L, (hipStream_t)stream);
     if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
     return BSF_OK;
+}
+
+int bsf_fit_device(const bsf_request *r, bsf_result *res, void *stream)
+{
+    g_err.clear();
+    return fit_device(r, res, nullptr, stream);
+}
+
+int bsf_fit_device_status(const bsf_request *r, bsf_result *res, unsigned char *status, void *stream)
+{
+    g_err.clear();
+    if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
+    return fit_device(r, res, status, stream);
 }
 
 // Host-array fit from nstarts starts (rows of `starts`, n_initP values each;
 // NULL starts: the request's initP, one start): one upload, one launch per
 // start (later starts keep a result only where it beats the kept
 // log-likelihood), one download.
-static int fit_host_starts(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res)
+static int fit_host_starts(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res,
+                           unsigned char *status = nullptr)
 {
     int rc = validate(r, res);
     if (rc) return rc;
@@ -194,7 +208,8 @@ static int fit_host_starts(const bsf_request *r, const float *starts, int nstart
     char *buf = nullptr;
     const size_t a = 256;
     auto up = [&](size_t x) { return (x + a - 1) / a * a; };
-    const size_t total = up(bytes_data) + up(bytes_c) + up(bytes_c3) + up(bytes_out);
+    const size_t bytes_st = status ? N : 0;
+    const size_t total = up(bytes_data) + up(bytes_c) + up(bytes_c3) + up(bytes_out) + up(bytes_st);
     hipError_t e = hipMalloc((void **)&buf, total);
     if (e != hipSuccess) return fail(BSF_ERR_NOMEM, "bsf: could not allocate %zu bytes on the device", total);
     float *d_data = (float *)buf;
@@ -202,6 +217,8 @@ static int fit_host_starts(const bsf_request *r, const float *starts, int nstart
     float *d_c3 = (float *)(buf + up(bytes_data) + up(bytes_c));
     float *d_out = (float *)(buf + up(bytes_data) + up(bytes_c) + up(bytes_c3));
     float *d_P = d_out, *d_CR = d_out + N * (NV + 1), *d_LL = d_CR + N * NV;
+    unsigned char *d_st = status ? (unsigned char *)(buf + up(bytes_data) + up(bytes_c) + up(bytes_c3) + up(bytes_out))
+                                 : nullptr;
     hipStream_t st = hipStreamPerThread;
     rc = BSF_OK;
     e = hipMemcpyAsync(d_data, r->imstack, bytes_data, hipMemcpyHostToDevice, st);
@@ -215,6 +232,7 @@ static int fit_host_starts(const bsf_request *r, const float *starts, int nstart
 #endif
     if (e == hipSuccess) {
         bsf::FitLaunch L = make_launch(r, d_data, d_c, nc3 ? d_c3 : nullptr, d_P, d_CR, d_LL);
+        L.status = d_st;
 #ifdef BSF_TRACE
         L.dbg = d_tr;
 #endif
@@ -241,6 +259,7 @@ static int fit_host_starts(const bsf_request *r, const float *starts, int nstart
     if (e == hipSuccess) e = hipMemcpyAsync(res->P, d_P, N * (NV + 1) * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->CRLB, d_CR, N * NV * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->LL, d_LL, N * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && status) e = hipMemcpyAsync(status, d_st, N, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
     (void)hipFree(buf);
@@ -251,6 +270,13 @@ int bsf_fit_host(const bsf_request *r, bsf_result *res)
 {
     g_err.clear();
     return fit_host_starts(r, nullptr, 1, 0, res);
+}
+
+int bsf_fit_host_status(const bsf_request *r, bsf_result *res, unsigned char *status)
+{
+    g_err.clear();
+    if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
+    return fit_host_starts(r, nullptr, 1, 0, res, status);
 }
 
 // The starts of a multi-start request: fittype 7 or 8, 1..64 starts of

@@ -92,6 +92,33 @@ int bsf_fit_host(const bsf_request *req, bsf_result *res);
  * (GPUmleFit_LM/wrapper.cu:256-279). */
 int bsf_fit_device(const bsf_request *req, bsf_result *res, void *stream);
 
+/* Per-ROI status flags, one byte per ROI (bits of enum bsf_roi_flag; 0: none
+ * of them).  The reference reports failures only for the whole call
+ * (mexErrMsgIdAndTxt + exit, GPUmleFit_LM/mexFunction.cpp:140-176) and
+ * leaves per-ROI trouble silent in its outputs; these flags name it. */
+enum bsf_roi_flag {
+    BSF_ROI_MAX_ITER = 1,        /* the iteration limit ended the fit (the relative
+                                    deviance change, and for fittype 8 the summed
+                                    step, stayed above the tolerance; P[:, NV] ==
+                                    iterations, CPUmleFit_LM.cpp:2555-2557)       */
+    BSF_ROI_SOLVE_FAILED = 2,    /* the damped normal equations of the last
+                                    iteration had no solution (a negative Cholesky
+                                    pivot, or a non-finite step): that step was not
+                                    taken, as the reference's kernel_cholesky
+                                    failure path (CPUmleFit_LM.cpp:2592, 2701-2704) */
+    BSF_ROI_FISHER_SINGULAR = 4, /* a CRLB entry is not a finite positive number:
+                                    the Fisher information at the result is
+                                    singular or indefinite (e.g. a fit ending where
+                                    the spline's z derivative vanishes)           */
+    BSF_ROI_NONFINITE = 8        /* a parameter or the log-likelihood is not finite */
+};
+
+/* bsf_fit_host / bsf_fit_device plus the per-ROI status flags: `status` holds
+ * nfits bytes (host memory for the host entry point, device memory for the
+ * device one).  The fits are the same, bit for bit. */
+int bsf_fit_host_status(const bsf_request *req, bsf_result *res, unsigned char *status);
+int bsf_fit_device_status(const bsf_request *req, bsf_result *res, unsigned char *status, void *stream);
+
 /* Multi-start fit: every ROI is fitted from nstarts starts (rows of
  * `starts`, req->n_initP values each: fittype 7 {z, w}, fittype 8
  * {z, w, u, g2}) and the outputs keep, start by start, a result whose

@@ -49,6 +49,35 @@ def test_errors_without_device_or_bad_args():
         assert e.value.code == 6
 
 
+def test_status_entry_points_without_device_or_bad_args():
+    import torch
+    import bsfit
+    from bsfit import gen
+    from bsfit.api import Request, Result, _request
+    lib = bsfit.load()
+    w = gen.config(0, N=2)
+    im = np.asfortranarray(w.imstack, np.float32)
+    c = np.asfortranarray(w.coeff, np.float32)
+    P = np.zeros((2, 6), np.float32, order="F")
+    CR = np.zeros((2, 5), np.float32, order="F")
+    LL = np.zeros(2, np.float32)
+    req = _request(6, im.shape[0], 2, 50, c.ctypes.data, c.shape, None, None, bsfit.PREC_MIX, im.ctypes.data)
+    res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
+    lib.bsf_fit_host_status.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_void_p]
+    lib.bsf_fit_device_status.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_void_p, C.c_void_p]
+    # a NULL status array is an argument error, reported before any device work
+    assert lib.bsf_fit_host_status(C.byref(req), C.byref(res), None) == 1
+    assert lib.bsf_fit_device_status(C.byref(req), C.byref(res), None, None) == 1
+    assert "status" in lib.bsf_last_error().decode()
+    with pytest.raises(bsfit.BsfError) as e:
+        bsfit.fit_status(9, w.imstack, 50, w.coeff)
+    assert e.value.code == 2
+    if not torch.cuda.is_available():
+        with pytest.raises(bsfit.BsfError) as e:
+            bsfit.fit_status(6, w.imstack, 50, w.coeff)
+        assert e.value.code == 6
+
+
 def test_session_errors_without_device_or_bad_args():
     import torch
     import bsfit
