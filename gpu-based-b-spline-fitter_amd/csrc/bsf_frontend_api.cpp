@@ -17,14 +17,11 @@ using bsf::fail;
 
 int check_gfx950()
 {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: no HIP device (%s)", hipGetErrorString(e));
-    hipDeviceProp_t p;
-    e = hipGetDeviceProperties(&p, dev);
-    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: hipGetDeviceProperties: %s", hipGetErrorString(e));
-    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
-        return fail(BSF_ERR_DEVICE, "bsf: device %d is %s; this build targets gfx950 (MI355X)", dev, p.gcnArchName);
+    bsf::DeviceInfo d;
+    int rc = bsf::device_info(&d);
+    if (rc) return rc;
+    if (!d.gfx950)
+        return fail(BSF_ERR_DEVICE, "bsf: device %d is %s; this build targets gfx950 (MI355X)", d.dev, d.arch);
     return BSF_OK;
 }
 

@@ -110,4 +110,15 @@ hipError_t launch_fit_all(const FitLaunch &L, hipStream_t st);
 int fail(int code, const char *fmt, ...);
 void clear_error();
 
+// The current device: its index, whether it is a gfx950 and its LDS limit per
+// workgroup, queried from HIP once per device (hipGetDeviceProperties takes
+// tens of microseconds, a small launch's worth).  BSF_OK or BSF_ERR_DEVICE.
+struct DeviceInfo {
+    int dev;
+    int gfx950;
+    size_t lds_limit;
+    char arch[64];
+};
+int device_info(DeviceInfo *out);
+
 }  // namespace bsf

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <cmath>
 #include <cstdlib>
@@ -29,6 +30,40 @@ int fail(int code, const char *fmt, ...)
 }
 
 void clear_error() { g_err.clear(); }
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_dev_mu;
+DeviceInfo g_dev[kMaxDevices];
+bool g_dev_known[kMaxDevices];
+}  // namespace
+
+int device_info(DeviceInfo *out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: no HIP device (%s)", hipGetErrorString(e));
+    const bool cacheable = dev >= 0 && dev < kMaxDevices;
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (cacheable && g_dev_known[dev]) {
+        *out = g_dev[dev];
+        return BSF_OK;
+    }
+    hipDeviceProp_t p;
+    e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: hipGetDeviceProperties: %s", hipGetErrorString(e));
+    DeviceInfo d;
+    d.dev = dev;
+    d.gfx950 = std::strncmp(p.gcnArchName, "gfx950", 6) == 0;
+    d.lds_limit = p.sharedMemPerBlock;
+    std::snprintf(d.arch, sizeof d.arch, "%s", p.gcnArchName);
+    if (cacheable) {
+        g_dev[dev] = d;
+        g_dev_known[dev] = true;
+    }
+    *out = d;
+    return BSF_OK;
+}
 
 }  // namespace bsf
 
@@ -85,18 +120,15 @@ int validate(const bsf_request *r, const bsf_result *res)
 // The device is a gfx950 and one workgroup's LDS fits.
 int check_device(int S, int ft, int f64)
 {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: no HIP device (%s)", hipGetErrorString(e));
-    hipDeviceProp_t p;
-    e = hipGetDeviceProperties(&p, dev);
-    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: hipGetDeviceProperties: %s", hipGetErrorString(e));
-    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
-        return fail(BSF_ERR_DEVICE, "bsf: device %d is %s; this build targets gfx950 (MI355X)", dev, p.gcnArchName);
+    bsf::DeviceInfo d;
+    int rc = bsf::device_info(&d);
+    if (rc) return rc;
+    if (!d.gfx950)
+        return fail(BSF_ERR_DEVICE, "bsf: device %d is %s; this build targets gfx950 (MI355X)", d.dev, d.arch);
     size_t lds = bsf::lds_bytes(ft, f64, S);
-    if (lds > p.sharedMemPerBlock)
+    if (lds > d.lds_limit)
         return fail(BSF_ERR_ROI_SIZE, "bsf: ROI size %d with fittype %d needs %zu B of LDS (device limit %zu B)", S,
-                    ft, lds, (size_t)p.sharedMemPerBlock);
+                    ft, lds, d.lds_limit);
     return BSF_OK;
 }
 
@@ -400,15 +432,13 @@ static int check_device_4pi(const bsf_request_4pi *r)
 {
     int rc = check_device(1, 6, 0);
     if (rc) return rc;
-    int dev = 0;
-    hipDeviceProp_t p;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
-        return fail(BSF_ERR_DEVICE, "bsf: device query failed");
+    bsf::DeviceInfo d;
+    if ((rc = bsf::device_info(&d))) return rc;
     const int f64 = r->precision == BSF_PREC_F64, nv = nv_4pi(r->shared, r->nch);
     size_t lds = bsf::lds_bytes_4pi(f64, r->sz, r->nch, nv);
-    if (lds > p.sharedMemPerBlock)
+    if (lds > d.lds_limit)
         return fail(BSF_ERR_ROI_SIZE, "bsf: 4Pi ROI size %d x %d channels needs %zu B of LDS (limit %zu B)", r->sz,
-                    r->nch, lds, (size_t)p.sharedMemPerBlock);
+                    r->nch, lds, d.lds_limit);
     return BSF_OK;
 }
 
@@ -439,10 +469,8 @@ static bsf::Fit4PiLaunch make_launch_4pi(const bsf_request_4pi *r, const float *
     L.dT = dT;
     L.initZ = iz;
     L.initPhi = ip;
-    int dev = 0;
-    hipDeviceProp_t p;
-    const size_t limit = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess
-                             ? (size_t)p.sharedMemPerBlock : 0;
+    bsf::DeviceInfo d;
+    const size_t limit = bsf::device_info(&d) == BSF_OK ? d.lds_limit : 0;
     L.wpc = bsf::use_wpc_4pi(r->nfits < kWpcMaxRois, L.f64, L.S, L.nch, nv_4pi(r->shared, r->nch), limit);
     return L;
 }
