@@ -360,18 +360,6 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             }
         }
         newErr = evaluate(false);
-#ifdef BSF_TRACE
-        if (A.dbg && lane < NV) {
-            float *o = A.dbg + ((size_t)n * 64 + kk) * (3 + 2 * NV);
-            if (lane == 0) {
-                o[0] = (float)kk;
-                o[1] = (float)newErr;
-                o[2] = (float)newLambda;
-            }
-            o[3 + lane] = (float)th;
-            o[3 + NV + lane] = (float)upd;
-        }
-#endif
     }
 
     // ---- Fisher information, CRLB, log-likelihood.  Fittypes 6/7 evaluate

@@ -256,18 +256,9 @@ static int fit_host_starts(const bsf_request *r, const float *starts, int nstart
     e = hipMemcpyAsync(d_data, r->imstack, bytes_data, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_c, r->coeff, bytes_c, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && nc3) e = hipMemcpyAsync(d_c3, r->coeff3, bytes_c3, hipMemcpyHostToDevice, st);
-#ifdef BSF_TRACE
-    float *d_tr = nullptr;
-    const size_t ntr = N * 64 * (3 + 2 * NV);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_tr, ntr * 4);
-    if (e == hipSuccess) e = hipMemset(d_tr, 0, ntr * 4);
-#endif
     if (e == hipSuccess) {
         bsf::FitLaunch L = make_launch(r, d_data, d_c, nc3 ? d_c3 : nullptr, d_P, d_CR, d_LL);
         L.status = d_st;
-#ifdef BSF_TRACE
-        L.dbg = d_tr;
-#endif
         for (int k = 0; k < nstarts && e == hipSuccess; k++) {
             if (starts) {
                 for (int q = 0; q < r->n_initP && q < 4; q++) L.initP[q] = starts[(size_t)k * r->n_initP + q];
@@ -277,17 +268,6 @@ static int fit_host_starts(const bsf_request *r, const float *starts, int nstart
             e = launch_all(L, st);
         }
     }
-#ifdef BSF_TRACE
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess && getenv("BSF_TRACE_FILE")) {
-        float *h = (float *)malloc(ntr * 4);
-        e = hipMemcpy(h, d_tr, ntr * 4, hipMemcpyDeviceToHost);
-        FILE *f = fopen(getenv("BSF_TRACE_FILE"), "wb");
-        if (f) { fwrite(h, 4, ntr, f); fclose(f); }
-        free(h);
-    }
-    (void)hipFree(d_tr);
-#endif
     if (e == hipSuccess) e = hipMemcpyAsync(res->P, d_P, N * (NV + 1) * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->CRLB, d_CR, N * NV * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->LL, d_LL, N * 4, hipMemcpyDeviceToHost, st);
