@@ -480,8 +480,12 @@ inline bool sized_kernel(int, int S) { return S == 11 || S == 13 || S == 15 || S
 // in bsf_fit_sized6.hip / bsf_fit_sized7.hip.
 hipError_t launch_sized6(const Dev &A, int nblocks, hipStream_t st, int S);
 hipError_t launch_sized7(const Dev &A, int nblocks, hipStream_t st, int S);
-// FP64 fittype 7 at S = 11, 13, 15 (sized_kernel_f64)
-inline bool sized_kernel_f64(int ft, int S) { return ft == 7 && (S == 11 || S == 13 || S == 15); }
+// FP64: fittype 6 at every one of these sizes, fittype 7 up to S = 15
+inline bool sized_kernel_f64(int ft, int S)
+{
+    return ft == 6 ? sized_kernel(6, S) : ft == 7 && (S == 11 || S == 13 || S == 15);
+}
+hipError_t launch_sized6d(const Dev &A, int nblocks, hipStream_t st, int S);
 hipError_t launch_sized7d(const Dev &A, int nblocks, hipStream_t st, int S);
 
 }  // namespace bsf

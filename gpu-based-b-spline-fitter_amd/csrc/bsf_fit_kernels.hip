@@ -2,9 +2,8 @@
 // Poisson-MLE Levenberg-Marquardt fitter (fittypes 6: 3D, 7: 4D, 8: 5D+3D):
 // LDS sizing, the generic instantiations of the kernel template in
 // bsf_fit_kernel.h (any ROI size, FP32 and FP64) and the launch dispatch,
-// which sends uniform-size FP32 fittype-6/7 launches (and FP64 fittype 7 at
-// S <= 15) to the size-specialised kernels (bsf_fit_sized6.hip,
-// bsf_fit_sized7.hip).
+// which sends uniform-size fittype-6/7 launches to the size-specialised
+// kernels (bsf_fit_sized6.hip, bsf_fit_sized7.hip).
 #include "bsf_fit_kernel.h"
 
 namespace bsf {
@@ -68,7 +67,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     const int sized = !L.rois || L.uniform ? L.S : 0;
     switch (L.fittype * 2 + (L.f64 ? 1 : 0)) {
     case 12: return sized_kernel(6, sized) ? launch_sized6(A, nb, st, sized) : launch_t<6, float>(A, nb, st);
-    case 13: return launch_t<6, double>(A, nb, st);
+    case 13: return sized_kernel_f64(6, sized) ? launch_sized6d(A, nb, st, sized) : launch_t<6, double>(A, nb, st);
     case 14: return sized_kernel(7, sized) ? launch_sized7(A, nb, st, sized) : launch_t<7, float>(A, nb, st);
     case 15: return sized_kernel_f64(7, sized) ? launch_sized7d(A, nb, st, sized) : launch_t<7, double>(A, nb, st);
     case 16: return launch_t<8, float>(A, nb, st);
