@@ -127,3 +127,28 @@ def test_short_trajectories_match_f64_oracle(orc, ft, cfg, S, iters):
     ok = (rel.max(1) < 1e-5) & (P[:, NV] == Po[:, NV])
     print("fittype", ft, "S", w.S, "iterations", iters, "agree", ok.mean(), flush=True)
     assert ok.mean() >= 0.95
+
+
+@pytest.mark.parametrize("ft,cfg,N,shift,min_f64", [(6, 0, 128, 12.0, 0.9), (7, 1, 96, 12.0, 0.25)])
+def test_negative_pixels_match_oracle(orc, ft, cfg, N, shift, min_f64):
+    """Data shifted below zero, as offset-subtracted camera frames are: the
+    initial values use the raw values (centre of mass, minimum), the
+    likelihood treats d <= 0 as 0 (CPUmleFit_LM.cpp:2530-2549); the GPU
+    against the oracle at the bars of test_fit_matches_oracle."""
+    import bsfit
+    from bsfit import gen
+    w = gen.config(cfg, N=N)
+    im = np.asfortranarray(w.imstack - np.float32(shift), np.float32)
+    frac_neg = float(np.mean(im <= 0))
+    print("case fittype=%d N=%d shift=%g: %.1f%% of pixels <= 0" % (ft, N, shift, 100 * frac_neg), flush=True)
+    assert frac_neg > 0.05
+    gm = bsfit.fit(ft, im, 50, w.coeff, w.initP)
+    gd = bsfit.fit(ft, im, 50, w.coeff, w.initP, precision=bsfit.PREC_F64)
+    of = orc.fit(ft, im, 50, w.coeff, w.initP, nthreads=8)
+    od = orc.fit_f64(ft, im, 50, w.coeff, w.initP, nthreads=8)
+    s64 = compare(ft, *gd, *od)
+    llm = _ll_not_worse(gm[2], of[2])
+    print("gpuF64 vs orcF64", s64, "LL(gpuMIX) >= LL(orcF32) on %.3f" % llm, flush=True)
+    assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0]))
+    assert s64["agree"] >= min_f64
+    assert llm >= 0.75
