@@ -1,0 +1,50 @@
+"""Small ROIs (S = 1, 2, 3, 5, 7): sizes below the 11-17 of the workloads,
+where a ROI fills less than one 64-lane pixel pass, the support window is
+only a few taps wide and, below two pixels per parameter (S <= 2; S = 3 for
+fittypes 7 and 8), rounding decides every step (only the call and the
+finite / non-finite pattern of the outputs are checked there).
+The GPU against the oracle on the same inputs: outputs finite where the
+oracle's are, the FP64 kernel's agreement with the FP64 oracle and the FP32
+kernel's log-likelihood against the oracle's float build."""
+import numpy as np
+import pytest
+
+from parity_util import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _ll_not_worse(LLg, LLo):
+    LLg = np.asarray(LLg, np.float64)
+    LLo = np.asarray(LLo, np.float64)
+    ok = (np.isfinite(LLg) & (LLg >= LLo - 1e-3 * np.abs(LLo) - 1e-3)) | ~np.isfinite(LLo)
+    return float(ok.mean())
+
+
+@pytest.mark.parametrize("S", [1, 2, 3, 5, 7])
+@pytest.mark.parametrize("ft,cfg", [(6, 0), (7, 1), (8, 3)])
+def test_small_roi_matches_oracle(orc, ft, cfg, S):
+    import bsfit
+    from bsfit import gen
+    w = gen.config(cfg, N=24, S=S)
+    w = w[0] if isinstance(w, list) else w
+    print("case fittype=%d S=%d N=%d" % (ft, S, w.N), flush=True)
+    gm = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3)
+    gd = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, precision=bsfit.PREC_F64)
+    of = orc.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
+    od = orc.fit_f64(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
+    s64 = compare(ft, *gd, *od)
+    llm = _ll_not_worse(gm[2], of[2])
+    llo = _ll_not_worse(od[2], of[2])   # the oracle's FP64 build against its float build
+    print("gpuF64 vs orcF64 agree %.3f p_ok %.3f  LL(gpuMIX) >= LL(orcF32) on %.3f (orcF64 >= orcF32 on %.3f)"
+          % (s64["agree"], s64["p_ok"], llm, llo), flush=True)
+    NV = bsfit.num_params(ft)
+    np.testing.assert_array_equal(np.isfinite(gd[0][:, :NV]), np.isfinite(od[0][:, :NV]))
+    if S * S < 2 * NV:
+        # under two pixels per parameter the damped systems are (nearly)
+        # singular and rounding decides the steps: the oracle's own FP64 and
+        # float builds disagree on 0-55 % of these ROIs' log-likelihoods
+        return
+    assert llm >= 0.75
+    if ft == 6:
+        assert s64["agree"] >= 0.8
