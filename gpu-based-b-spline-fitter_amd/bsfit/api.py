@@ -150,12 +150,14 @@ def _fit(fittype, imstack, iterations, coeff, initP, coeff3, precision, with_sta
     return P, CR, LL, st
 
 
-def fit_multistart(fittype, imstack, iterations, coeff, starts, coeff3=None, strict=False, precision=PREC_MIX):
+def fit_multistart(fittype, imstack, iterations, coeff, starts, coeff3=None, strict=False, precision=PREC_MIX,
+                   status=False):
     """Multi-start fit (bsf_fit_multistart_host): starts (K, 2) for fittype 7
     or (K, 4) for fittype 8; a later start's result replaces the kept one
     where its log-likelihood is higher by >= 1e-4 (strict: > 1e-4), as the
     reference's callers combine their per-start MEX calls.  Returns
-    (P, CRLB, LL) as fit()."""
+    (P, CRLB, LL) as fit(); with status=True also the kept results' per-ROI
+    status flags (bsf_fit_multistart_host_status)."""
     lib = load()
     lib.bsf_fit_multistart_host.argtypes = [C.POINTER(Request), C.c_void_p, C.c_int, C.c_int, C.POINTER(Result)]
     lib.bsf_fit_multistart_host.restype = C.c_int
@@ -174,8 +176,17 @@ def fit_multistart(fittype, imstack, iterations, coeff, starts, coeff3=None, str
                    c3.ctypes.data if c3 is not None else None, st[0], precision, imstack.ctypes.data)
     req.n_initP = st.shape[1]
     res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
-    _check(lib.bsf_fit_multistart_host(C.byref(req), st.ctypes.data, st.shape[0], int(bool(strict)), C.byref(res)))
-    return P, CR, LL
+    if not status:
+        _check(lib.bsf_fit_multistart_host(C.byref(req), st.ctypes.data, st.shape[0], int(bool(strict)),
+                                           C.byref(res)))
+        return P, CR, LL
+    flags = np.zeros(N, np.uint8)
+    lib.bsf_fit_multistart_host_status.argtypes = [C.POINTER(Request), C.c_void_p, C.c_int, C.c_int,
+                                                   C.POINTER(Result), C.c_void_p]
+    lib.bsf_fit_multistart_host_status.restype = C.c_int
+    _check(lib.bsf_fit_multistart_host_status(C.byref(req), st.ctypes.data, st.shape[0], int(bool(strict)),
+                                              C.byref(res), flags.ctypes.data))
+    return P, CR, LL, flags
 
 
 def fit_device(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_ptr, CRLB_ptr, LL_ptr,

@@ -307,20 +307,35 @@ static int validate_starts(const bsf_request *r, const float *starts, int nstart
     return BSF_OK;
 }
 
-int bsf_fit_multistart_host(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res)
+static int multistart_host(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res,
+                           unsigned char *status)
 {
-    g_err.clear();
     int rc = validate_starts(r, starts, nstarts);
     if (rc) return rc;
     bsf_request q = *r;
     q.initP = starts;   // row 0 passes the single-start checks
-    return fit_host_starts(&q, starts, nstarts, strict, res);
+    return fit_host_starts(&q, starts, nstarts, strict, res, status);
 }
 
-int bsf_fit_multistart_device(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res,
-                              void *stream)
+int bsf_fit_multistart_host(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res)
 {
     g_err.clear();
+    return multistart_host(r, starts, nstarts, strict, res, nullptr);
+}
+
+int bsf_fit_multistart_host_status(const bsf_request *r, const float *starts, int nstarts, int strict,
+                                   bsf_result *res, unsigned char *status)
+{
+    g_err.clear();
+    if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
+    return multistart_host(r, starts, nstarts, strict, res, status);
+}
+
+// The launches of a device multi-start: the kernel writes a ROI's outputs
+// and status flags only where a later start's result is kept.
+static int multistart_device(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res,
+                             unsigned char *status, void *stream)
+{
     int rc = validate_starts(r, starts, nstarts);
     if (rc) return rc;
     bsf_request q = *r;
@@ -329,6 +344,7 @@ int bsf_fit_multistart_device(const bsf_request *r, const float *starts, int nst
     if ((rc = check_device(q.sz, q.fittype, q.precision == BSF_PREC_F64))) return rc;
     if (q.nfits == 0) return BSF_OK;
     bsf::FitLaunch L = make_launch(&q, q.imstack, q.coeff, q.coeff3, res->P, res->CRLB, res->LL);
+    L.status = status;
     hipError_t e = hipSuccess;
     for (int k = 0; k < nstarts && e == hipSuccess; k++) {
         for (int i = 0; i < q.n_initP && i < 4; i++) L.initP[i] = starts[(size_t)k * q.n_initP + i];
@@ -338,6 +354,21 @@ int bsf_fit_multistart_device(const bsf_request *r, const float *starts, int nst
     }
     if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
     return BSF_OK;
+}
+
+int bsf_fit_multistart_device(const bsf_request *r, const float *starts, int nstarts, int strict, bsf_result *res,
+                              void *stream)
+{
+    g_err.clear();
+    return multistart_device(r, starts, nstarts, strict, res, nullptr, stream);
+}
+
+int bsf_fit_multistart_device_status(const bsf_request *r, const float *starts, int nstarts, int strict,
+                                     bsf_result *res, unsigned char *status, void *stream)
+{
+    g_err.clear();
+    if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
+    return multistart_device(r, starts, nstarts, strict, res, status, stream);
 }
 
 int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_ndim, int fittype, int iterations,

@@ -135,6 +135,13 @@ int bsf_fit_multistart_host(const bsf_request *req, const float *starts, int nst
 int bsf_fit_multistart_device(const bsf_request *req, const float *starts, int nstarts, int strict, bsf_result *res,
                               void *stream);
 
+/* Multi-start fits plus the per-ROI status flags (enum bsf_roi_flag) of the
+ * kept result of every ROI (host / device status array of nfits bytes). */
+int bsf_fit_multistart_host_status(const bsf_request *req, const float *starts, int nstarts, int strict,
+                                   bsf_result *res, unsigned char *status);
+int bsf_fit_multistart_device_status(const bsf_request *req, const float *starts, int nstarts, int strict,
+                                     bsf_result *res, unsigned char *status, void *stream);
+
 /* The MEX argument list as plain C (GPUmleFit_LM/mexFunction.cpp:14-55):
  * imdims = size(imstack) (2 dims: one ROI), coeff_dims = size(coeff).
  * varmap is accepted and ignored as the MEX ignores it for fittypes 6-8

@@ -85,3 +85,22 @@ def test_device_status_equals_host_status():
     Ph, CRh, LLh, sth = bsfit.fit_status(7, w.imstack, 50, w.coeff, w.initP)
     np.testing.assert_array_equal(Pd, Ph)
     np.testing.assert_array_equal(std, sth)
+
+
+@pytest.mark.parametrize("ft,strict", [(7, True), (8, False)])
+def test_multistart_status_follows_the_kept_result(ft, strict):
+    """Multi-start with status: outputs equal bsf_fit_multistart_host's, and
+    the flags describe the kept result (the kernel writes them only where it
+    writes that ROI's outputs)."""
+    if ft == 7:
+        w = gen.config(1, N=96)
+        starts = np.array([[30.5, 1.5], [24.4, 2.5], [36.6, 1.0]], np.float32)
+    else:
+        w = gen.config(3, N=48, S=13)[0]
+        starts = np.array([[10.5, 6.0, 4.5, 0.75], [10.5, 12.0, 4.5, 0.75]], np.float32)
+    ref = bsfit.fit_multistart(ft, w.imstack, 50, w.coeff, starts, coeff3=w.coeff3, strict=strict)
+    P, CR, LL, st = bsfit.fit_multistart(ft, w.imstack, 50, w.coeff, starts, coeff3=w.coeff3, strict=strict,
+                                         status=True)
+    for a, b in zip((P, CR, LL), ref):
+        np.testing.assert_array_equal(a, b)
+    _check_flags(P, CR, LL, st, bsfit.num_params(ft), 50)
