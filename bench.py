@@ -405,8 +405,10 @@ def frames_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    # five untimed launches bring the GPU to its steady clock (the first few
+    # launches of a process run up to 20 % longer); ten timed ones are 80 ms
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--precision", choices=["mix", "f64"], default="mix")
     ap.add_argument("--no-cpu", action="store_true")
