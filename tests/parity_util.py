@@ -1,4 +1,4 @@
-"""Shared comparison of a GPU fit with the oracle (
+"""Shared comparison of a GPU fit with the oracle (per-ROI agreement:
 coordinates within 1e-4 px (spline units for z, w, u; absolute for g2),
 photons / background within 1e-4 relative, CRLB within 1e-4 relative,
 iteration counts equal), and device buffers for tests of the device entry

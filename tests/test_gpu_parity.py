@@ -7,7 +7,7 @@ and wraps) makes iteration counts and late trajectory steps sensitive to
 rounding: the oracle's own float build and its FP64 build disagree on
 15-75 % of ROIs (ft7 / ft8 most), and reordering only its Cholesky flips
 another ~20 % of ft7 / ft8 ROIs.  So per-ROI agreement is measured at the
-north-star tolerances (parity_util.compare) and asserted as rates:
+per-ROI tolerances of parity_util.compare and asserted as rates:
 
 * FP64 kernel vs FP64 oracle: same arithmetic class, different summation
   order; high agreement expected for ft6, lower for the chaotic ft7 / ft8.
