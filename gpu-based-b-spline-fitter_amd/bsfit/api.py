@@ -267,11 +267,12 @@ def num_params_4pi(shared, nch=4):
 
 
 def fit_4pi(imstack, shared, iterations, coeff, phaseShift, dT=None, initZ=None, initPhi=None,
-            precision=PREC_MIX):
+            precision=PREC_MIX, status=False):
     """4Pi multi-channel fit on host arrays: imstack (S, S, N, nch) Fortran
     order, shared (6,), coeff 4D, phaseShift (nch,) in w units, optional
     dT (6, nch, N), initZ (N,), initPhi (N,).  Returns P (N, NV+1),
-    CRLB (N, NV), LL (N,) -- GPUmleFit_LM_4Pi_bspline's outputs."""
+    CRLB (N, NV), LL (N,) -- GPUmleFit_LM_4Pi_bspline's outputs; with
+    status=True also the per-ROI status flags (bsf_fit_4pi_host_status)."""
     lib = load()
     lib.bsf_fit_4pi_host.argtypes = [C.POINTER(Request4Pi), C.POINTER(Result)]
     lib.bsf_fit_4pi_host.restype = C.c_int
@@ -307,8 +308,14 @@ def fit_4pi(imstack, shared, iterations, coeff, phaseShift, dT=None, initZ=None,
     r.phaseShift = ps.ctypes.data
     r.precision = int(precision)
     res = Result(P.ctypes.data, CR.ctypes.data, LL.ctypes.data)
-    _check(lib.bsf_fit_4pi_host(C.byref(r), C.byref(res)))
-    return P, CR, LL
+    if not status:
+        _check(lib.bsf_fit_4pi_host(C.byref(r), C.byref(res)))
+        return P, CR, LL
+    flags = np.zeros(N, np.uint8)
+    lib.bsf_fit_4pi_host_status.argtypes = [C.POINTER(Request4Pi), C.POINTER(Result), C.c_void_p]
+    lib.bsf_fit_4pi_host_status.restype = C.c_int
+    _check(lib.bsf_fit_4pi_host_status(C.byref(r), C.byref(res), flags.ctypes.data))
+    return P, CR, LL, flags
 
 
 def GPUmleFit_LM_4Pi_bspline(imstack, shared, iterations, coeffs, dT=None, initZ=None, initPhi=None,

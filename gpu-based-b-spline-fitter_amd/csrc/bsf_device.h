@@ -879,6 +879,26 @@ __device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV])
     return 0;
 }
 
+// Per-ROI status flags (enum bsf_roi_flag of bsfit.h) of a finished fit:
+// 1 the iteration limit ended it, 2 its last damped solve failed, 4 a CRLB
+// entry is not a finite positive number, 8 a parameter or the log-likelihood
+// is not finite.  Every lane of the wave calls it; `own` marks the lanes that
+// hold a parameter (th) and its CRLB.
+__device__ __forceinline__ void write_roi_status(unsigned char *status, int n, bool own, float th, float crlb,
+                                                 float ll, bool max_iter, bool solve_failed, int lane)
+{
+    const unsigned long long badc = __ballot(own && !(crlb > 0.f && isfinite(crlb)));
+    const unsigned long long badp = __ballot(own && !isfinite(th));
+    if (lane == 0) {
+        unsigned f = 0;
+        if (max_iter) f |= 1u;
+        if (solve_failed) f |= 2u;
+        if (badc) f |= 4u;
+        if (badp || !isfinite(ll)) f |= 8u;
+        status[n] = (unsigned char)f;
+    }
+}
+
 // ROI pixel p (column-major, p = jj*S + ii) to row ii and column jj without
 // an integer divide: exact for p < 2^12, as the window decode of stage_a.
 __device__ __forceinline__ void pixel_rc(int p, float invS, int S, int &ii, int &jj)

@@ -218,7 +218,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 
     E newErr = evaluate(true);
     E oldErr = (E)1e13, newLambda = E(1), oldLambda = E(1);
-    int turn = 1, kk;
+    int turn = 1, kk, lastinfo = 0;
     const int zslot = A4.slot[4][0];
     for (kk = 0; kk < A.iterations; kk++) {
         if (fabs((newErr - oldErr) / newErr) < (E)1e-6) break;
@@ -244,6 +244,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
             upd = (E)xi;
             if (__any(lane < nv && isnan(upd))) info = 1;
         }
+        lastinfo = info;
         if (info != 0) {
             newLambda = E(10) * newLambda;
             continue;
@@ -330,6 +331,9 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
         A.P[(size_t)A.N * nv + n] = (float)kk;
         A.LL[n] = (float)s_err[0];
     }
+    if (A.status)
+        write_roi_status(A.status, n, lane < nv, (float)th, (float)crlb, (float)s_err[0], kk >= A.iterations,
+                         lastinfo != 0, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,7 +563,7 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
 
     E newErr = evaluate(true);
     E oldErr = (E)1e13, newLambda = E(1), oldLambda = E(1);
-    int turn = 1, kk;
+    int turn = 1, kk, lastinfo = 0;
     const int zslot = A4.slot[4][0];
     for (kk = 0; kk < A.iterations; kk++) {
         if (fabs((newErr - oldErr) / newErr) < (E)1e-6) break;
@@ -594,6 +598,7 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
             upd = (E)xi;
             if (__any(lane < nv && isnan(upd))) info = 1;
         }
+        lastinfo = info;
         if (info != 0) {
             newLambda = E(10) * newLambda;
             continue;
@@ -671,6 +676,9 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
         A.P[(size_t)A.N * nv + n] = (float)kk;
         A.LL[n] = (float)s_err[0];
     }
+    if (A.status)
+        write_roi_status(A.status, n, lane < nv, (float)th, (float)crlb, (float)s_err[0], kk >= A.iterations,
+                         lastinfo != 0, lane);
 }
 
 size_t lds_bytes_4pi_wpc(int f64, int S, int nch, int nv) { return lds_4pi_wpc(S, nch, f64 ? 8 : 4, nv).total; }
@@ -730,6 +738,7 @@ hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st)
     A.has_initP = 0;
     A.theta_in = nullptr;
     A.dbg = nullptr;
+    A.status = L.status;
     A4.nch = L.nch;
     int n = 0;
     for (int p = 0; p < 6; p++) {

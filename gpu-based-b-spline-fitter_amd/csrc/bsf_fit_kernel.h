@@ -443,21 +443,9 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         A.P[(size_t)A.N * NV + n] = (float)kk;
         A.LL[n] = llf;
     }
-    if (A.status) {
-        // per-ROI status flags (bsf_roi_flag in bsfit.h): iteration limit,
-        // failed last solve, singular Fisher information, non-finite output
-        const float cf = (float)crlb, tf = (float)th;
-        const unsigned long long badc = __ballot(lane < NV && !(cf > 0.f && isfinite(cf)));
-        const unsigned long long badp = __ballot(lane < NV && !isfinite(tf));
-        if (lane == 0) {
-            unsigned f = 0;
-            if (kk >= A.iterations) f |= 1u;
-            if (lastinfo != 0) f |= 2u;
-            if (badc) f |= 4u;
-            if (badp || !isfinite(llf)) f |= 8u;
-            A.status[n] = (unsigned char)f;
-        }
-    }
+    if (A.status)
+        write_roi_status(A.status, n, lane < NV, (float)th, (float)crlb, llf, kk >= A.iterations, lastinfo != 0,
+                         lane);
 }
 
 // ------------------------------------------------------------ launcher

@@ -56,6 +56,7 @@ struct Fit4PiLaunch {
     const float *dT;          // device [6, nch, N] or NULL
     const float *initZ, *initPhi;  // device [N] or NULL
     int wpc;                  // 1: a workgroup of nch waves per ROI, one per channel; 0: one wave per ROI
+    unsigned char *status;    // device: per-ROI status flags (bsf_roi_flag), or NULL
 };
 
 hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st);

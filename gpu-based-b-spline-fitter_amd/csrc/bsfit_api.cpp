@@ -506,23 +506,35 @@ int bsf_test_set_4pi_kernel(int mode)
     return BSF_OK;
 }
 
-int bsf_fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, void *stream)
+static int fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, unsigned char *status, void *stream)
 {
-    g_err.clear();
     int rc = validate_4pi(r, res);
     if (rc) return rc;
     if ((rc = check_device_4pi(r))) return rc;
     if (r->nfits == 0) return BSF_OK;
     bsf::Fit4PiLaunch L = make_launch_4pi(r, r->imstack, r->coeff, r->dT, r->initZ, r->initPhi, res->P, res->CRLB,
                                           res->LL);
+    L.status = status;
     hipError_t e = launch_all_4pi(L, (hipStream_t)stream);
     if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
     return BSF_OK;
 }
 
-int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
+int bsf_fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, void *stream)
 {
     g_err.clear();
+    return fit_4pi_device(r, res, nullptr, stream);
+}
+
+int bsf_fit_4pi_device_status(const bsf_request_4pi *r, bsf_result *res, unsigned char *status, void *stream)
+{
+    g_err.clear();
+    if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
+    return fit_4pi_device(r, res, status, stream);
+}
+
+static int fit_4pi_host(const bsf_request_4pi *r, bsf_result *res, unsigned char *status)
+{
     int rc = validate_4pi(r, res);
     if (rc) return rc;
     if ((rc = check_device_4pi(r))) return rc;
@@ -533,9 +545,10 @@ int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
     for (int a = 0; a < 4; a++) nc *= (size_t)r->coeff_dims[a];
     const size_t bd = N * S2 * r->nch * 4, bc = nc * 4, bt = r->dT ? N * 6 * r->nch * 4 : 0;
     const size_t bz = r->initZ ? N * 4 : 0, bp = r->initPhi ? N * 4 : 0, bo = N * (2 * nv + 2) * 4;
+    const size_t bs = status ? N : 0;
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     char *buf = nullptr;
-    hipError_t e = hipMalloc((void **)&buf, up(bd) + up(bc) + up(bt) + up(bz) + up(bp) + up(bo));
+    hipError_t e = hipMalloc((void **)&buf, up(bd) + up(bc) + up(bt) + up(bz) + up(bp) + up(bo) + up(bs));
     if (e != hipSuccess) return fail(BSF_ERR_NOMEM, "bsf: device allocation failed");
     char *q = buf;
     float *d_data = (float *)q; q += up(bd);
@@ -544,6 +557,8 @@ int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
     float *d_iz = bz ? (float *)q : nullptr; q += up(bz);
     float *d_ip = bp ? (float *)q : nullptr; q += up(bp);
     float *d_P = (float *)q, *d_CR = d_P + N * (nv + 1), *d_LL = d_CR + N * nv;
+    q += up(bo);
+    unsigned char *d_st = bs ? (unsigned char *)q : nullptr;
     hipStream_t st = hipStreamPerThread;
     e = hipMemcpyAsync(d_data, r->imstack, bd, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_c, r->coeff, bc, hipMemcpyHostToDevice, st);
@@ -552,16 +567,31 @@ int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
     if (e == hipSuccess && bp) e = hipMemcpyAsync(d_ip, r->initPhi, bp, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
         bsf::Fit4PiLaunch L = make_launch_4pi(r, d_data, d_c, d_dT, d_iz, d_ip, d_P, d_CR, d_LL);
-            e = launch_all_4pi(L, st);
+        L.status = d_st;
+        e = launch_all_4pi(L, st);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(res->P, d_P, N * (nv + 1) * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->CRLB, d_CR, N * nv * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->LL, d_LL, N * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && bs) e = hipMemcpyAsync(status, d_st, N, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     rc = BSF_OK;
     if (e != hipSuccess) rc = fail(BSF_ERR_DEVICE, "bsf: %s", hipGetErrorString(e));
     (void)hipFree(buf);
     return rc;
+}
+
+int bsf_fit_4pi_host(const bsf_request_4pi *r, bsf_result *res)
+{
+    g_err.clear();
+    return fit_4pi_host(r, res, nullptr);
+}
+
+int bsf_fit_4pi_host_status(const bsf_request_4pi *r, bsf_result *res, unsigned char *status)
+{
+    g_err.clear();
+    if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
+    return fit_4pi_host(r, res, status);
 }
 
 // ------------------------------------------------------ coefficient builder

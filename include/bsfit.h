@@ -194,6 +194,11 @@ int bsf_fit_4pi_host(const bsf_request_4pi *req, bsf_result *res);
  * and the outputs are device pointers; shared and phaseShift are host). */
 int bsf_fit_4pi_device(const bsf_request_4pi *req, bsf_result *res, void *stream);
 
+/* 4Pi fits plus the per-ROI status flags (enum bsf_roi_flag): host / device
+ * status array of nfits bytes. */
+int bsf_fit_4pi_host_status(const bsf_request_4pi *req, bsf_result *res, unsigned char *status);
+int bsf_fit_4pi_device_status(const bsf_request_4pi *req, bsf_result *res, unsigned char *status, void *stream);
+
 /* Text of the last error on this thread ("" if none).
  * Replaces mexErrMsgIdAndTxt (GPUmleFit_LM/mexFunction.cpp:140-176). */
 const char *bsf_last_error(void);

@@ -104,3 +104,19 @@ def test_multistart_status_follows_the_kept_result(ft, strict):
     for a, b in zip((P, CR, LL), ref):
         np.testing.assert_array_equal(a, b)
     _check_flags(P, CR, LL, st, bsfit.num_params(ft), 50)
+
+
+@pytest.mark.parametrize("N", [200, 1500])   # the wave-per-channel kernel below 1,024 ROIs, one wave per ROI above
+@pytest.mark.parametrize("prec", [bsfit.PREC_MIX, bsfit.PREC_F64])
+def test_4pi_status_flags_match_outputs(N, prec):
+    w = gen.make_rois_4pi(N, 13, 99, gen.model_coeffs("4pi"), shared=(1, 1, 1, 0, 1, 1))
+    NV = bsfit.num_params_4pi(w.shared, w.nch)
+    args = (w.imstack, w.shared, 50, w.coeff, w.phaseShift)
+    kw = dict(initZ=w.initZ, initPhi=w.initPhi, precision=prec)
+    ref = bsfit.fit_4pi(*args, **kw)
+    P, CR, LL, st = bsfit.fit_4pi(*args, status=True, **kw)
+    for a, b in zip((P, CR, LL), ref):
+        np.testing.assert_array_equal(a, b)
+    _check_flags(P, CR, LL, st, NV, 50)
+    print("4Pi N=%d precision=%d flags: max_iter %d solve_failed %d" % (
+        N, prec, int(np.sum(st & bsfit.ROI_MAX_ITER != 0)), int(np.sum(st & bsfit.ROI_SOLVE_FAILED != 0))), flush=True)
