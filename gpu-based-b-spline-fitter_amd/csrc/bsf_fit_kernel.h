@@ -443,9 +443,21 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         A.P[(size_t)A.N * NV + n] = (float)kk;
         A.LL[n] = llf;
     }
-    if (A.status)
-        write_roi_status(A.status, n, lane < NV, (float)th, (float)crlb, llf, kk >= A.iterations, lastinfo != 0,
-                         lane);
+    if (A.status) {
+        // per-ROI status flags, as write_roi_status writes them; kept inline
+        // here, where the call form costs this kernel SGPR spills
+        const float cf = (float)crlb, tf = (float)th;
+        const unsigned long long badc = __ballot(lane < NV && !(cf > 0.f && isfinite(cf)));
+        const unsigned long long badp = __ballot(lane < NV && !isfinite(tf));
+        if (lane == 0) {
+            unsigned f = 0;
+            if (kk >= A.iterations) f |= 1u;
+            if (lastinfo != 0) f |= 2u;
+            if (badc) f |= 4u;
+            if (badp || !isfinite(llf)) f |= 8u;
+            A.status[n] = (unsigned char)f;
+        }
+    }
 }
 
 // ------------------------------------------------------------ launcher
