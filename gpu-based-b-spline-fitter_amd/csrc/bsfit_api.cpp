@@ -371,14 +371,10 @@ int bsf_fit_multistart_device_status(const bsf_request *r, const float *starts, 
     return multistart_device(r, starts, nstarts, strict, res, status, stream);
 }
 
-int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_ndim, int fittype, int iterations,
-                            const float *coeff, const size_t *coeff_dims, int coeff_ndim, const float *varmap,
-                            size_t varmap_numel, int silent, const float *initP, size_t initP_numel,
-                            const float *coeff3, float *P, float *CRLB, float *LL)
+static int mex_fit(const float *imstack, const size_t *imdims, int im_ndim, int fittype, int iterations,
+                   const float *coeff, const size_t *coeff_dims, int coeff_ndim, int silent, const float *initP,
+                   size_t initP_numel, const float *coeff3, float *P, float *CRLB, float *LL, unsigned char *status)
 {
-    (void)varmap;
-    (void)varmap_numel;
-    g_err.clear();
     if (!imdims || im_ndim < 2 || im_ndim > 3) return fail(BSF_ERR_ARG, "bsf: imstack must be 2-D or 3-D");
     if (imdims[0] > BSF_MAX_ROI)
         return fail(BSF_ERR_ROI_SIZE, "bsf: X,Y dimension of data must be smaller than %d", BSF_MAX_ROI);
@@ -400,12 +396,38 @@ int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_n
     r.precision = BSF_PREC_MIX;
     bsf_result res = {P, CRLB, LL};
     auto t0 = std::chrono::steady_clock::now();
-    int rc = bsf_fit_host(&r, &res);
+    int rc = fit_host_starts(&r, nullptr, 1, 0, &res, status);
     double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc == BSF_OK && !silent)
         std::printf("subregion size: %d fittype: %d\nNumber of fits: %d\nfit time %f s, fits per second %f\n", r.sz,
                     fittype, r.nfits, dt, dt > 0 ? r.nfits / dt : 0.0);
     return rc;
+}
+
+int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_ndim, int fittype, int iterations,
+                            const float *coeff, const size_t *coeff_dims, int coeff_ndim, const float *varmap,
+                            size_t varmap_numel, int silent, const float *initP, size_t initP_numel,
+                            const float *coeff3, float *P, float *CRLB, float *LL)
+{
+    (void)varmap;
+    (void)varmap_numel;
+    g_err.clear();
+    return mex_fit(imstack, imdims, im_ndim, fittype, iterations, coeff, coeff_dims, coeff_ndim, silent, initP,
+                   initP_numel, coeff3, P, CRLB, LL, nullptr);
+}
+
+int bsf_GPUmleFit_LM_multiD_status(const float *imstack, const size_t *imdims, int im_ndim, int fittype,
+                                   int iterations, const float *coeff, const size_t *coeff_dims, int coeff_ndim,
+                                   const float *varmap, size_t varmap_numel, int silent, const float *initP,
+                                   size_t initP_numel, const float *coeff3, float *P, float *CRLB, float *LL,
+                                   unsigned char *status)
+{
+    (void)varmap;
+    (void)varmap_numel;
+    g_err.clear();
+    if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
+    return mex_fit(imstack, imdims, im_ndim, fittype, iterations, coeff, coeff_dims, coeff_ndim, silent, initP,
+                   initP_numel, coeff3, P, CRLB, LL, status);
 }
 
 // ------------------------------------------------------------------ 4Pi

@@ -6,7 +6,9 @@
  *   [P, CRLB, LogL] = GPUmleFit_LM_multiD(imstack, fittype, iterations, coeff,
  *                                         varmap, silent, initP, coeff3)
  *
- * for the B-spline fittypes 6, 7 and 8.  Argument checks follow the
+ * for the B-spline fittypes 6, 7 and 8.  An optional fourth output, which the
+ * reference MEX does not have, returns the per-ROI status flags (uint8, N x 1,
+ * bits of enum bsf_roi_flag).  Argument checks follow the
  * reference MEX (:247-385): single-precision square data, ROI size <= 51,
  * iterations default 50, varmap ignored for these fittypes, initP and coeff3
  * read when given.  Everything else is bsf_GPUmleFit_LM_multiD.
@@ -30,7 +32,8 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     int imnd, cnd, fittype, iterations, silent, NV, rc, a;
     const mwSize *d;
     const float *data, *coeff, *initP = NULL, *coeff3 = NULL;
-    mxArray *out[3];
+    mxArray *out[4];
+    if (nlhs > 4) mexErrMsgIdAndTxt("GPUmleFit_LM:WrongNoArgs", "Outputs: P, CRLB, LogL[, status]");
     if (nrhs < 4)
         mexErrMsgIdAndTxt("GPUmleFit_LM:WrongNoArgs",
                           "Inputs: imstack, fittype, iterations, coeff[, varmap, silent, initP, coeff3]");
@@ -61,15 +64,24 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     out[0] = mxCreateNumericMatrix(N, (mwSize)NV + 1, mxSINGLE_CLASS, mxREAL);
     out[1] = mxCreateNumericMatrix(N, (mwSize)NV, mxSINGLE_CLASS, mxREAL);
     out[2] = mxCreateNumericMatrix(N, 1, mxSINGLE_CLASS, mxREAL);
-    rc = bsf_GPUmleFit_LM_multiD(data, imdims, imnd > 2 ? 3 : 2, fittype, iterations, coeff, cdims, cnd, NULL, 0,
-                                 silent, initP, ninit, coeff3,
-                                 (float *)mxGetData(out[0]), (float *)mxGetData(out[1]), (float *)mxGetData(out[2]));
+    out[3] = nlhs > 3 ? mxCreateNumericMatrix(N, 1, mxUINT8_CLASS, mxREAL) : NULL;
+    if (out[3])
+        rc = bsf_GPUmleFit_LM_multiD_status(data, imdims, imnd > 2 ? 3 : 2, fittype, iterations, coeff, cdims, cnd,
+                                            NULL, 0, silent, initP, ninit, coeff3, (float *)mxGetData(out[0]),
+                                            (float *)mxGetData(out[1]), (float *)mxGetData(out[2]),
+                                            (unsigned char *)mxGetData(out[3]));
+    else
+        rc = bsf_GPUmleFit_LM_multiD(data, imdims, imnd > 2 ? 3 : 2, fittype, iterations, coeff, cdims, cnd, NULL, 0,
+                                     silent, initP, ninit, coeff3, (float *)mxGetData(out[0]),
+                                     (float *)mxGetData(out[1]), (float *)mxGetData(out[2]));
     if (rc) {
-        for (a = 0; a < 3; a++) mxDestroyArray(out[a]);
+        for (a = 0; a < 4; a++)
+            if (out[a]) mxDestroyArray(out[a]);
         mexErrMsgIdAndTxt("GPUmleFit_LM:Fit", "%s", bsf_last_error());
     }
     /* plhs holds max(nlhs, 1) outputs */
-    for (a = 0; a < 3; a++) {
+    for (a = 0; a < 4; a++) {
+        if (!out[a]) continue;
         if (a < (nlhs > 0 ? nlhs : 1))
             plhs[a] = out[a];
         else

@@ -152,6 +152,15 @@ int bsf_GPUmleFit_LM_multiD(const float *imstack, const size_t *imdims, int im_n
                             const float *varmap, size_t varmap_numel, int silent, const float *initP,
                             size_t initP_numel, const float *coeff3, float *P, float *CRLB, float *LL);
 
+/* The same plus the per-ROI status flags (enum bsf_roi_flag, one byte per
+ * ROI): the gateway's optional fourth output.  The reference MEX has three
+ * outputs and no such flags. */
+int bsf_GPUmleFit_LM_multiD_status(const float *imstack, const size_t *imdims, int im_ndim, int fittype,
+                                   int iterations, const float *coeff, const size_t *coeff_dims, int coeff_ndim,
+                                   const float *varmap, size_t varmap_numel, int silent, const float *initP,
+                                   size_t initP_numel, const float *coeff3, float *P, float *CRLB, float *LL,
+                                   unsigned char *status);
+
 /* ------------------------------------------------------------------ 4Pi
  * Multi-channel 4Pi fit.  Replaces the MEX the reference's 4Pi demo calls,
  *   [P, CRLB, LogL] = GPUmleFit_LM_4Pi_bspline(imstack, uint32(shared), iterations,

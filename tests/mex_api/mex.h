@@ -17,6 +17,7 @@ typedef enum {
     mxUNKNOWN_CLASS = 0,
     mxDOUBLE_CLASS = 6,
     mxSINGLE_CLASS = 7,
+    mxUINT8_CLASS = 9,
     mxUINT16_CLASS = 11,
     mxUINT32_CLASS = 13
 } mxClassID;

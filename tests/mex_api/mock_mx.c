@@ -88,6 +88,7 @@ double mxGetScalar(const mxArray *a)
     switch (a->cls) {
     case mxDOUBLE_CLASS: return ((double *)a->data)[0];
     case mxSINGLE_CLASS: return ((float *)a->data)[0];
+    case mxUINT8_CLASS: return ((uint8_t *)a->data)[0];
     case mxUINT16_CLASS: return ((uint16_t *)a->data)[0];
     case mxUINT32_CLASS: return ((uint32_t *)a->data)[0];
     default: return 0.0;

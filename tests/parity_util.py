@@ -72,7 +72,7 @@ class Mex:
     MATLAB would call it: numpy arrays in, numpy arrays out; a
     mexErrMsgIdAndTxt comes back as MexError(id, message)."""
 
-    _CLS = {np.dtype(np.float64): 6, np.dtype(np.float32): 7, np.dtype(np.uint16): 11, np.dtype(np.uint32): 13}
+    _CLS = {np.dtype(np.float64): 6, np.dtype(np.float32): 7, np.dtype(np.uint8): 9, np.dtype(np.uint16): 11, np.dtype(np.uint32): 13}
     _DT = {v: k for k, v in _CLS.items()}
 
     def __init__(self, name, outdir):

@@ -41,6 +41,20 @@ def test_fitter_gateway_equals_library(mexdir):
     np.testing.assert_array_equal(got[0], bsfit.fit(6, one, 50, w6.coeff)[0])
 
 
+def test_fitter_gateway_status_output(mexdir):
+    """The optional fourth output: the per-ROI status flags (uint8, N x 1),
+    equal to bsfit.fit_status; the first three outputs are unchanged by it."""
+    m = Mex("GPUmleFit_LM_multiD", mexdir)
+    w7 = gen.config(1, N=48)
+    for iters in (50.0, 3.0):
+        got = m(4, w7.imstack, 7.0, iters, w7.coeff, 0.0, 1.0, w7.initP)
+        ref = bsfit.fit_status(7, w7.imstack, int(iters), w7.coeff, w7.initP)
+        assert got[3].dtype == np.uint8 and got[3].shape == (48, 1)
+        for a, b in zip(got, ref):
+            np.testing.assert_array_equal(a.reshape(b.shape), b)
+    assert np.all(got[3].ravel() & bsfit.ROI_MAX_ITER)                  # 3 iterations: every ROI hit the limit
+
+
 def test_roi_gateway_equals_library(mexdir):
     m = Mex("bsf_find_rois", mexdir)
     frames, _ = gen.make_frames(3, 120, 110, 10, 12)

@@ -31,7 +31,13 @@ def test_fitter_gateway_argument_errors(mexdir):
         with pytest.raises(MexError) as e:
             m(3, *args)
         assert e.value.ident == ident
+    with pytest.raises(MexError) as e:                                   # P, CRLB, LogL and status at most
+        m(5, w.imstack, 6.0, 50.0, w.coeff)
+    assert e.value.ident == "GPUmleFit_LM:WrongNoArgs"
     if not torch.cuda.is_available():
+        with pytest.raises(MexError) as e:
+            m(4, w.imstack, 6.0, 50.0, w.coeff)
+        assert e.value.ident == "GPUmleFit_LM:Fit" and "device" in str(e.value)
         with pytest.raises(MexError) as e:
             m(3, w.imstack, 6.0, 50.0, w.coeff)
         assert e.value.ident == "GPUmleFit_LM:Fit" and "device" in str(e.value)
