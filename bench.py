@@ -430,8 +430,9 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(ROIs shard across GPUs with no exchange, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: this fitter is measured on one GPU; --gpus %d is not supported "
+              "(ROIs are independent: N GPUs would each fit their own share of the ROIs with no exchange, "
+              "one process per GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "coeff":
         coeff_bench(args)
