@@ -6,6 +6,9 @@ HBM): 100,000 seeded 15x15 ROIs, fittype 7 (x, y, N, bg, z, colour), a
 FP32 evaluation + FP64 solve.  Synthetic data: ROIs rendered from the spline
 model itself plus Poisson noise (gen.config(1)).
 
+--model 3d / vortex5d (and --roi-size) run the 3D astigmatic (fittype 6,
+gen.config(0)) and vortex-5D (fittype 8, gen.config(3)) fits the same way.
+
 Prints one JSON line: fits/s (whole job), the roofline of the fit kernel
 (algorithmic FP32 flops per ROI-evaluation x evaluations / kernel time) and
 the CPU oracle timed on a bounded sample on the host cores.
@@ -38,6 +41,14 @@ def f_alg(ft, S, nch=1, NV=None):
     if ft == 8:
         f += 2 * P * P * 4 * 2 + 2 * S * S * 16 * 4 + 20 * S * S
     return f
+
+
+# bench.py --model: configuration of bsfit.gen.config, fittype, label, grid
+FIT_MODELS = {
+    "3d": (0, 6, "3D astigmatic", "27x27x61 B-spline"),
+    "colour4d": (1, 7, "colour-4D", "27x27x61x3 B-spline"),
+    "vortex5d": (3, 8, "vortex-5D", "21x21x21x17x9 B-spline + 21^3 averaged model"),
+}
 
 
 def cpu_baseline(w, budget_s=15.0):
@@ -411,6 +422,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--precision", choices=["mix", "f64"], default="mix")
+    ap.add_argument("--model", choices=sorted(FIT_MODELS), default="colour4d",
+                    help="fit: PSF model (colour4d, fittype 7, is the headline workload)")
+    ap.add_argument("--roi-size", type=int, default=None,
+                    help="fit: ROI size (default 13 for 3d and vortex5d, 15 for colour4d)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy (for scaling rocprofv3 byte counters)")
@@ -455,14 +470,18 @@ def main():
     import bsfit
     from bsfit import gen
 
+    cfg, ft, label, grid = FIT_MODELS[args.model]
     t0 = time.perf_counter()
-    w = gen.config(1, N=args.n)
+    w = gen.config(cfg, N=args.n, S=args.roi_size or (13 if ft == 8 else None))
+    w = w[0] if isinstance(w, list) else w
     tgen = time.perf_counter() - t0
-    print("bench: generated %d ROIs (S=%d, fittype 7) in %.1f s" % (w.N, w.S, tgen), file=sys.stderr, flush=True)
+    print("bench: generated %d ROIs (S=%d, fittype %d) in %.1f s" % (w.N, w.S, ft, tgen), file=sys.stderr,
+          flush=True)
     dev = torch.device("cuda:0")
     d_data = torch.from_numpy(np.ascontiguousarray(w.imstack.ravel(order="F"))).to(dev)
     d_coeff = torch.from_numpy(np.ascontiguousarray(w.coeff.ravel(order="F"))).to(dev)
-    NV = 6
+    d_c3 = None if w.coeff3 is None else torch.from_numpy(np.ascontiguousarray(w.coeff3.ravel(order="F"))).to(dev)
+    NV = {6: 5, 7: 6, 8: 8}[ft]
     d_P = torch.empty((NV + 1) * w.N, dtype=torch.float32, device=dev)
     d_CR = torch.empty(NV * w.N, dtype=torch.float32, device=dev)
     d_LL = torch.empty(w.N, dtype=torch.float32, device=dev)
@@ -470,8 +489,9 @@ def main():
     stream = torch.cuda.current_stream()
 
     def step():
-        bsfit.fit_device(7, d_data.data_ptr(), w.S, w.N, w.iterations, d_coeff.data_ptr(), w.coeff.shape,
+        bsfit.fit_device(ft, d_data.data_ptr(), w.S, w.N, w.iterations, d_coeff.data_ptr(), w.coeff.shape,
                          d_P.data_ptr(), d_CR.data_ptr(), d_LL.data_ptr(), initP=w.initP,
+                         coeff3_ptr=None if d_c3 is None else d_c3.data_ptr(),
                          precision=prec, stream=stream.cuda_stream)
 
     for _ in range(args.warmup):
@@ -491,13 +511,13 @@ def main():
     P = d_P.cpu().numpy().reshape(NV + 1, w.N)
     kk = P[NV]
     evals = float(np.sum(kk + 2))
-    flops = evals * f_alg(7, w.S)
+    flops = evals * f_alg(ft, w.S)
     achieved = flops / (kern_ms / 1000.0)
     print("bench: %.2f ms/step (kernel %.2f ms), mean iterations %.2f" % (ms_per_step, kern_ms, kk.mean()),
           file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(w)
     out = {
-        "metric": "ROI fits/sec (colour-4D, fittype 7, FP32 eval + FP64 solve)",
+        "metric": "ROI fits/sec (%s, fittype %d, FP32 eval + FP64 solve)" % (label, ft),
         "value": w.N / (ms_per_step / 1000.0),
         "unit": "fits/s",
         "n_gpus": 1,
@@ -509,15 +529,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "mix" else "f64",
         "data": "synthetic (seeded spline-rendered ROIs + Poisson noise)",
-        "config": {"workload": "colour-4D fit: 100k 15x15 ROIs, 27x27x61x3 B-spline, 50 LM iterations",
-                   "fittype": 7, "rois": w.N, "roi_size": w.S, "iterations": w.iterations,
+        "config": {"workload": "%s fit: %s %dx%d ROIs, %s, %d LM iterations"
+                               % (label, "100k" if w.N == 100000 else str(w.N), w.S, w.S, grid, w.iterations),
+                   "fittype": ft, "rois": w.N, "roi_size": w.S, "iterations": w.iterations,
                    "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": traffic_bytes(),
+                     "frac": achieved / FP32_PEAK,
+                     "traffic": traffic_bytes() if args.model == "colour4d" and w.S == 15 else None,
                      "note": "FP32 vector-ALU bound (no MFMA on this path); peak = MI355X FP32 157.3 TFLOP/s; "
-                             "achieved = sum(iterations+2) x F_alg(S=15) = %d flop per ROI-evaluation / kernel time; "
+                             "achieved = sum(iterations+2) x F_alg(S=%d) = %d flop per ROI-evaluation / kernel time; "
                              "traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)"
-                             % f_alg(7, w.S)},
+                             % (w.S, f_alg(ft, w.S))},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
