@@ -77,6 +77,18 @@ typedef float real;
 #define TOL R(1e-6)            /* definitions.h:27 */
 #define TOL1 R(1e-5)           /* definitions.h:28 */
 
+/* Planted mistakes, for the tests that show the whole-fit pins catch a wrong
+ * LM control (0 = none, the restatement as written).  Set only by tests,
+ * never while another fit runs:
+ *   1 limiter sign flipped (theta += step)     2 clamp fade dropped
+ *   3 damping never raised on a rejected step  4 step limiter dropped
+ *   5 lambda decay on accepted steps dropped   6 Hessian weight d/mu instead of d/mu^2
+ *   7 x-derivative sign flipped in the model's dudt
+ *   8 (not a mistake) the 5D contraction's two irregular derivative terms
+ *     replaced by the regular ones, so that the fittype-8 LM control can be
+ *     checked as an optimiser apart from the reference's derivative defects */
+int SYM(orc_plant) = 0;
+
 /* ------------------------------------------------------------------ A6 */
 /* Centred cubic (deg 3) and quadratic (deg 2) B-spline basis,
  * CPUsplineLib.cpp:1289-1326.  The left quadratic branch has a double
@@ -215,8 +227,8 @@ void SYM(orc_deriv_eval)(int D, const real *coord, const int *nd, const real *ta
                  *     order (the w table), which is how tab[] is laid out.
                  *  u: term 16 differences the u tap k-idx instead of
                  *     k+idx-1 (CPUsplineLib.cpp:420-421, GPUsplineLib.cuh:681). */
-                if (D == 5 && a == 2 && t == 27) w = wd[271];
-                if (D == 5 && a == 4 && t == 16) {
+                if (D == 5 && a == 2 && t == 27 && SYM(orc_plant) != 8) w = wd[271];
+                if (D == 5 && a == 4 && t == 16 && SYM(orc_plant) != 8) {
                     int idx = ((blk >> 4) & 1) + 1;
                     a0 -= (long)(2 * idx - 1) * stride[4];
                 }
@@ -387,6 +399,7 @@ static void pixel_model(const fitctx *x, const real *th, int ii, int jj, real *d
     for (int a = 2; a < x->D; a++) cc[a] = x->coord[a];
     if (x->ft != 8) {
         SYM(orc_deriv_eval)(x->D, cc, x->nd, x->tab, x->c, th, dudt, model);
+        if (SYM(orc_plant) == 7) dudt[0] = -dudt[0];
         return;
     }
     /* fittype 8: mu = (1-g2) mu3 + g2 mu5; only dudt[0..4] blended;
@@ -398,6 +411,7 @@ static void pixel_model(const fitctx *x, const real *th, int ii, int jj, real *d
     for (int l = 0; l < 5; l++) dudt[l] = dudt[l] * g2 + (R(1.0) - g2) * d3[l];
     *model = m3 * (R(1.0) - g2) + m5 * g2;
     dudt[7] = m5 - m3;
+    if (SYM(orc_plant) == 7) dudt[0] = -dudt[0];
 }
 
 /* One pass over the ROI: Poisson deviance, gradient and expected-information
@@ -427,6 +441,7 @@ static real eval_pass(const fitctx *x, const real *th, int first, real *J, real 
                 t2 = data / (model * model);
             else
                 t2 = (real)((double)data / ((double)model * (double)model));
+            if (SYM(orc_plant) == 6) t2 = data / model;
             for (int l = 0; l < NV; l++)
                 for (int m = l; m < NV; m++) {
                     H[l * NV + m] += t2 * dudt[l] * dudt[m];
@@ -442,8 +457,9 @@ static real dmin(real a, double b) { return ((double)a < b) ? a : (real)b; }
 /* Whole fit of one ROI; writes column-major outputs with stride Nfits. */
 static void fit_one(int ft, int n, const float *data, const float *coeff, const float *coeff3,
                     const int *nd, int sz, int iterations, const float *initP,
-                    real *P, real *CRLB, real *LL, int Nfits)
+                    real *P, real *CRLB, real *LL, int Nfits, unsigned char *status)
 {
+    const int plant = SYM(orc_plant);
     static const float clamp6[5] = {2.0f, 2.0f, 10000, 20, 20};
     static const float clamp7[6] = {1.0f, 1.0f, 10000, 20, 2, 0.00001f};
     static const float fade7[6] = {0.5f, 0.5f, 0.4f, 0.5f, 0.7f, 0.8f};
@@ -453,7 +469,7 @@ static void fit_one(int ft, int n, const float *data, const float *coeff, const 
     real th[9], oth[9], sgn[8] = {0}, osgn[8] = {0}, clampv[8], oclamp[8], fade[8], upd[8] = {0};
     real J[8], H[64], L[64], U[64], M[64], Minv[64], Diag[8];
     real newErr, oldErr = (real)1e13, newLambda = 1, oldLambda = 1, sumUpdate = (real)1e10;
-    int kk, turn = 1;
+    int kk, turn = 1, info = 0, firstinfo = 0;
 
     x->ft = ft;
     x->NV = NV;
@@ -518,21 +534,22 @@ static void fit_one(int ft, int n, const float *data, const float *coeff, const 
             }
             newLambda = oldLambda;
             newErr = oldErr;
-            newLambda = 10 * newLambda;
-        } else if (newErr < oldErr) {
+            if (plant != 3) newLambda = 10 * newLambda;
+        } else if (newErr < oldErr && plant != 5) {
             if (newLambda > 1)
                 newLambda = (real)((double)newLambda * 0.8);
             else if (newLambda < 1)
                 newLambda = 1;
         }
         for (int i = 0; i < NV; i++) H[i * NV + i] = H[i * NV + i] * newLambda;
-        int info = SYM(orc_cholesky)(H, NV, L, U);
+        info = SYM(orc_cholesky)(H, NV, L, U);
         if (info == 0) {
             SYM(orc_lu_eval)(L, U, J, NV, upd);
             if (ft != 6)
                 for (int l = 0; l < NV; l++)
                     if (isnan(upd[l])) info = 1;
         }
+        if (kk == 0) firstinfo = info;
         if (info != 0) {
             newLambda = 10 * newLambda;
             continue;
@@ -554,12 +571,16 @@ static void fit_one(int ft, int n, const float *data, const float *coeff, const 
         }
         if (ft == 8) sumUpdate = 0;
         for (int l = 0; l < NV; l++) {
-            if (sgn[l] != 0) {
+            if (sgn[l] != 0 && plant != 2) {
                 if ((sgn[l] == 1 && upd[l] < 0) || (sgn[l] == -1 && upd[l] > 0))
                     clampv[l] = (ft == 6) ? (real)((double)clampv[l] * 0.5) : clampv[l] * fade[l];
             }
             sgn[l] = (upd[l] > 0) ? 1 : -1;
-            if (ft == 8) {
+            if (plant == 1 || plant == 4) {
+                real st = (plant == 4) ? upd[l] : upd[l] / (1 + FABSR(upd[l] / clampv[l]));
+                th[l] = (plant == 1) ? th[l] + st : th[l] - st;
+                if (ft == 8) sumUpdate += FABSR(st);
+            } else if (ft == 8) {
                 upd[l] = upd[l] / (1 + FABSR(upd[l] / clampv[l]));
                 th[l] = th[l] - upd[l];
                 sumUpdate += FABSR(upd[l]);
@@ -629,6 +650,21 @@ static void fit_one(int ft, int n, const float *data, const float *coeff, const 
     P[(long)Nfits * NV + n] = (real)kk;
     for (int a = 0; a < NV; a++) CRLB[(long)Nfits * a + n] = Diag[a];
     LL[n] = Div;
+    if (status) {
+        /* the library's per-ROI flags (1 iteration limit, 2 last solve failed,
+         * 4 a CRLB entry not finite positive, 8 a non-finite output, 16 the
+         * first damped solve failed) */
+        unsigned f = 0;
+        if (kk >= iterations) f |= 1u;
+        if (info != 0) f |= 2u;
+        for (int a = 0; a < NV; a++) {
+            if (!(Diag[a] > 0 && isfinite(Diag[a]))) f |= 4u;
+            if (!isfinite(th[a])) f |= 8u;
+        }
+        if (!isfinite(Div)) f |= 8u;
+        if (iterations > 0 && firstinfo != 0) f |= 16u;
+        status[n] = (unsigned char)f;
+    }
     free(x);
 }
 
@@ -740,7 +776,8 @@ static void fit_one_4pi(int n, const float *data, int sz, int Nfits, int nch, co
     real th[24], oth[24], sgn[24] = {0}, osgn[24] = {0}, clampv[24], oclamp[24], fade[24], upd[24] = {0};
     real J[24], H[576], L[576], U[576], M[576], Minv[576], Diag[24];
     real newErr, oldErr = (real)1e13, newLambda = 1, oldLambda = 1;
-    int kk, turn = 1, base_of[24];
+    int kk, turn = 1, base_of[24], info = 0;
+    const int plant = SYM(orc_plant);
     c->nch = nch;
     c->NV = slot_map(shared, nch, c->slot);
     c->dT = dT;
@@ -806,15 +843,15 @@ static void fit_one_4pi(int n, const float *data, int sz, int Nfits, int nch, co
             }
             newLambda = oldLambda;
             newErr = oldErr;
-            newLambda = 10 * newLambda;
-        } else if (newErr < oldErr) {
+            if (plant != 3) newLambda = 10 * newLambda;
+        } else if (newErr < oldErr && plant != 5) {
             if (newLambda > 1)
                 newLambda = (real)((double)newLambda * 0.8);
             else if (newLambda < 1)
                 newLambda = 1;
         }
         for (int i = 0; i < NV; i++) H[i * NV + i] = H[i * NV + i] * newLambda;
-        int info = SYM(orc_cholesky)(H, NV, L, U);
+        info = SYM(orc_cholesky)(H, NV, L, U);
         if (info == 0) {
             SYM(orc_lu_eval)(L, U, J, NV, upd);
             for (int l = 0; l < NV; l++)
@@ -847,11 +884,12 @@ static void fit_one_4pi(int n, const float *data, int sz, int Nfits, int nch, co
             turn = 0;
         }
         for (int l = 0; l < NV; l++) {
-            if (sgn[l] != 0) {
+            if (sgn[l] != 0 && plant != 2) {
                 if ((sgn[l] == 1 && upd[l] < 0) || (sgn[l] == -1 && upd[l] > 0)) clampv[l] = clampv[l] * fade[l];
             }
             sgn[l] = (upd[l] > 0) ? 1 : -1;
-            th[l] = th[l] - upd[l] / (1 + FABSR(upd[l] / clampv[l]));
+            const real st = (plant == 4) ? upd[l] : upd[l] / (1 + FABSR(upd[l] / clampv[l]));
+            th[l] = (plant == 1) ? th[l] + st : th[l] - st;
         }
         for (int l = 0; l < NV; l++) {
             switch (base_of[l]) {
@@ -949,6 +987,20 @@ real SYM(orc_eval_pass)(int fittype, const float *roi, int sz, const float *coef
  *   cdims[]: coefficient array dims as stored (data grid + 3 per axis).
  *   nthreads > 1 fits ROIs in parallel (OpenMP); 1 = the reference's
  *   sequential loop.  Returns 0, or -1 for an unsupported fittype. */
+int SYM(orc_fit_status)(int fittype, const float *data, int sz, int Nfits, const float *coeff,
+                        const int *cdims, const float *coeff3, int iterations, const float *initP,
+                        real *P, real *CRLB, real *LL, unsigned char *status, int nthreads)
+{
+    int nd[5] = {0, 0, 0, 0, 0};
+    int D = (fittype == 6) ? 3 : (fittype == 7) ? 4 : (fittype == 8) ? 5 : 0;
+    if (!D) return -1;
+    for (int a = 0; a < D; a++) nd[a] = cdims[a] - 3;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int n = 0; n < Nfits; n++)
+        fit_one(fittype, n, data, coeff, coeff3, nd, sz, iterations, initP, P, CRLB, LL, Nfits, status);
+    return 0;
+}
+
 int SYM(orc_fit)(int fittype, const float *data, int sz, int Nfits, const float *coeff,
                  const int *cdims, const float *coeff3, int iterations, const float *initP,
                  real *P, real *CRLB, real *LL, int nthreads)
@@ -959,6 +1011,6 @@ int SYM(orc_fit)(int fittype, const float *data, int sz, int Nfits, const float 
     for (int a = 0; a < D; a++) nd[a] = cdims[a] - 3;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
     for (int n = 0; n < Nfits; n++)
-        fit_one(fittype, n, data, coeff, coeff3, nd, sz, iterations, initP, P, CRLB, LL, Nfits);
+        fit_one(fittype, n, data, coeff, coeff3, nd, sz, iterations, initP, P, CRLB, LL, Nfits, NULL);
     return 0;
 }

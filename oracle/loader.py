@@ -55,6 +55,9 @@ def _L():
             f = getattr(_lib, "orc_fit" + sfx)
             f.restype = C.c_int
             f.argtypes = [C.c_int, fp, C.c_int, C.c_int, fp, ip, fp, C.c_int, fp, rp, rp, rp, C.c_int]
+            f = getattr(_lib, "orc_fit_status" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.c_int, fp, C.c_int, C.c_int, fp, ip, fp, C.c_int, fp, rp, rp, rp, C.c_void_p, C.c_int]
             g = getattr(_lib, "orc_bspline" + sfx)
             g.restype = C.c_float if sfx == "" else C.c_double
             g.argtypes = [C.c_float if sfx == "" else C.c_double, C.c_int]
@@ -69,7 +72,7 @@ def _ptr(a, t=fp):
     return a.ctypes.data_as(t) if a is not None else None
 
 
-def _fit(sfx, rdt, fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1):
+def _fit(sfx, rdt, fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1, status=False):
     lib = _L()
     imstack = _f32(imstack)
     if imstack.ndim == 2:
@@ -87,22 +90,50 @@ def _fit(sfx, rdt, fittype, imstack, iterations, coeff, initP=None, coeff3=None,
     CR = np.zeros((N, NV), rdt, order="F")
     LL = np.zeros(N, rdt)
     t = dp if rdt == np.float64 else fp
-    rc = getattr(lib, "orc_fit" + sfx)(
-        fittype, _ptr(imstack), S, N, _ptr(coeff), cd.ctypes.data_as(ip), _ptr(c3),
-        int(iterations), _ptr(ip_), _ptr(P, t), _ptr(CR, t), _ptr(LL, t), int(nthreads))
+    if status:
+        st = np.zeros(N, np.uint8)
+        rc = getattr(lib, "orc_fit_status" + sfx)(
+            fittype, _ptr(imstack), S, N, _ptr(coeff), cd.ctypes.data_as(ip), _ptr(c3),
+            int(iterations), _ptr(ip_), _ptr(P, t), _ptr(CR, t), _ptr(LL, t), st.ctypes.data, int(nthreads))
+    else:
+        rc = getattr(lib, "orc_fit" + sfx)(
+            fittype, _ptr(imstack), S, N, _ptr(coeff), cd.ctypes.data_as(ip), _ptr(c3),
+            int(iterations), _ptr(ip_), _ptr(P, t), _ptr(CR, t), _ptr(LL, t), int(nthreads))
     if rc != 0:
         raise ValueError("oracle: unsupported fittype %r" % fittype)
-    return P, CR, LL
+    return (P, CR, LL, st) if status else (P, CR, LL)
 
 
-def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1):
-    """Parity (float) restatement of the reference CPU fitter."""
-    return _fit("", np.float32, fittype, imstack, iterations, coeff, initP, coeff3, nthreads)
+def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1, status=False):
+    """Parity (float) restatement of the reference CPU fitter.  status=True
+    also returns the per-ROI flags of the library's bsf_roi_flag (uint8),
+    with bit 16 = the first damped solve failed."""
+    return _fit("", np.float32, fittype, imstack, iterations, coeff, initP, coeff3, nthreads, status)
 
 
-def fit_f64(fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1):
+def fit_f64(fittype, imstack, iterations, coeff, initP=None, coeff3=None, nthreads=1, status=False):
     """The same fit with every quantity in double (FP64 oracle)."""
-    return _fit("_f64", np.float64, fittype, imstack, iterations, coeff, initP, coeff3, nthreads)
+    return _fit("_f64", np.float64, fittype, imstack, iterations, coeff, initP, coeff3, nthreads, status)
+
+
+class planted:
+    """Context manager that plants mistake `k` in both oracle builds (see
+    orc_plant in bsf_oracle.c) for tests showing that a pin catches it."""
+
+    def __init__(self, k):
+        self.k = int(k)
+
+    def __enter__(self):
+        lib = _L()
+        for name in ("orc_plant", "orc_plant_f64"):
+            C.c_int.in_dll(lib, name).value = self.k
+        return self
+
+    def __exit__(self, *exc):
+        lib = _L()
+        for name in ("orc_plant", "orc_plant_f64"):
+            C.c_int.in_dll(lib, name).value = 0
+        return False
 
 
 # ---------------------------------------------------------------- primitives
