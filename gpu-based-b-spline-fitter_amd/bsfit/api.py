@@ -105,6 +105,7 @@ ROI_MAX_ITER = 1
 ROI_SOLVE_FAILED = 2
 ROI_FISHER_SINGULAR = 4
 ROI_NONFINITE = 8
+ROI_FIRST_SOLVE_FAILED = 16
 
 
 def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=PREC_MIX):
@@ -116,7 +117,7 @@ def fit(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=
 def fit_status(fittype, imstack, iterations, coeff, initP=None, coeff3=None, precision=PREC_MIX):
     """fit() plus the per-ROI status flags (bsf_fit_host_status): returns
     (P, CRLB, LL, status) with status (N,) uint8, bits ROI_MAX_ITER,
-    ROI_SOLVE_FAILED, ROI_FISHER_SINGULAR, ROI_NONFINITE."""
+    ROI_SOLVE_FAILED, ROI_FISHER_SINGULAR, ROI_NONFINITE, ROI_FIRST_SOLVE_FAILED."""
     return _fit(fittype, imstack, iterations, coeff, initP, coeff3, precision, True)
 
 
@@ -189,16 +190,49 @@ def fit_multistart(fittype, imstack, iterations, coeff, starts, coeff3=None, str
     return P, CR, LL, flags
 
 
+class RoiDesc(C.Structure):
+    """bsf_roi: one ROI of a launch that mixes ROI sizes."""
+    _fields_ = [("off", C.c_longlong), ("cstride", C.c_int), ("sz", C.c_int)]
+
+
+class Extras(C.Structure):
+    """bsf_fit_extras: optional device-side inputs and outputs."""
+    _fields_ = [("rois", C.c_void_p), ("status", C.c_void_p), ("evals", C.c_void_p)]
+
+
+def roi_table(sizes, offsets=None, cstride=0):
+    """Host array of bsf_roi records (numpy structured, 16 bytes each) for
+    ROI sizes `sizes` packed back to back (or at `offsets`, in floats)."""
+    sizes = np.asarray(sizes, np.int64)
+    if offsets is None:
+        offsets = np.concatenate([[0], np.cumsum(sizes * sizes)[:-1]]) if len(sizes) else np.zeros(0, np.int64)
+    t = np.zeros(len(sizes), dtype=np.dtype([("off", "<i8"), ("cstride", "<i4"), ("sz", "<i4")]))
+    t["off"] = offsets
+    t["cstride"] = cstride
+    t["sz"] = sizes
+    return t
+
+
 def fit_device(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_ptr, CRLB_ptr, LL_ptr,
-               initP=None, coeff3_ptr=None, precision=PREC_MIX, stream=None, status_ptr=None):
+               initP=None, coeff3_ptr=None, precision=PREC_MIX, stream=None, status_ptr=None, rois_ptr=None,
+               evals_ptr=None):
     """Queue a fit on device-resident buffers (raw device addresses, e.g.
     torch tensor .data_ptr()) on a HIP stream handle; returns immediately.
     status_ptr: a device array of N bytes for the per-ROI status flags
-    (bsf_fit_device_status), or None."""
+    (bsf_fit_device_status), or None.  rois_ptr: a device array of N bsf_roi
+    records (roi_table) for ROIs of mixed sizes, S then being the largest;
+    evals_ptr: a device int32 array of N for the evaluations each fit ran
+    (both through bsf_fit_device_ex)."""
     lib = load()
     ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
     req = _request(fittype, S, N, iterations, coeff_ptr, coeff_shape, coeff3_ptr, ip, precision, data_ptr)
     res = Result(P_ptr, CRLB_ptr, LL_ptr)
+    if rois_ptr is not None or evals_ptr is not None:
+        x = Extras(rois_ptr, status_ptr, evals_ptr)
+        lib.bsf_fit_device_ex.argtypes = [C.POINTER(Request), C.POINTER(Result), C.POINTER(Extras), C.c_void_p]
+        lib.bsf_fit_device_ex.restype = C.c_int
+        _check(lib.bsf_fit_device_ex(C.byref(req), C.byref(res), C.byref(x), stream))
+        return
     if status_ptr is None:
         _check(lib.bsf_fit_device(C.byref(req), C.byref(res), stream))
         return
@@ -264,6 +298,34 @@ def num_params_4pi(shared, nch=4):
     sh = np.ascontiguousarray(shared, np.uint32)
     lib.bsf_num_params_4pi.argtypes = [C.c_void_p, C.c_int]
     return lib.bsf_num_params_4pi(sh.ctypes.data, int(nch))
+
+
+def fit_4pi_device(data_ptr, S, N, nch, shared, iterations, coeff_ptr, coeff_shape, phaseShift, P_ptr, CRLB_ptr,
+                   LL_ptr, initZ_ptr=None, initPhi_ptr=None, dT_ptr=None, precision=PREC_MIX, stream=None,
+                   status_ptr=None, rois_ptr=None, evals_ptr=None):
+    """Queue a 4Pi fit on device-resident buffers (bsf_fit_4pi_device_ex):
+    data [S, S, N, nch], initZ / initPhi [N] or None, dT [6, nch, N] or None
+    (device addresses); shared (6,) and phaseShift (nch,) are host arrays.
+    Returns immediately; status / rois / evals as in fit_device."""
+    lib = load()
+    sh = np.ascontiguousarray(shared, np.uint32)
+    ps = np.ascontiguousarray(phaseShift, np.float32)
+    r = Request4Pi()
+    r.imstack = data_ptr
+    r.sz, r.nfits, r.nch = int(S), int(N), int(nch)
+    r.shared = sh.ctypes.data
+    r.iterations = int(iterations)
+    r.coeff = coeff_ptr
+    for a in range(4):
+        r.coeff_dims[a] = int(coeff_shape[a])
+    r.dT, r.initZ, r.initPhi = dT_ptr, initZ_ptr, initPhi_ptr
+    r.phaseShift = ps.ctypes.data
+    r.precision = int(precision)
+    res = Result(P_ptr, CRLB_ptr, LL_ptr)
+    x = Extras(rois_ptr, status_ptr, evals_ptr)
+    lib.bsf_fit_4pi_device_ex.argtypes = [C.POINTER(Request4Pi), C.POINTER(Result), C.POINTER(Extras), C.c_void_p]
+    lib.bsf_fit_4pi_device_ex.restype = C.c_int
+    _check(lib.bsf_fit_4pi_device_ex(C.byref(r), C.byref(res), C.byref(x), stream))
 
 
 def fit_4pi(imstack, shared, iterations, coeff, phaseShift, dT=None, initZ=None, initPhi=None,

@@ -104,6 +104,7 @@ struct Dev {
     const float *theta_in; // test hook: evaluate once at these parameters
     float *dbg;            // test hook output: err, J[NV], H[NV*NV] per ROI
     unsigned char *status; // per-ROI status flags (bsf_roi_flag), or NULL
+    int *evals;            // per-ROI count of evaluations run (the final Fisher pass included), or NULL
     float initP[4];
     int has_initP;
     int keep;              // 0: write the results; 1 / 2: write them only where the log-likelihood
@@ -787,28 +788,37 @@ __device__ __forceinline__ int chol_solve_rows(double (&a)[NV], double b, int la
 // Diagonal of the inverse Fisher matrix by unpivoted Gauss-Jordan elimination
 // over rows held one per lane, in FP64: the reference's kernel_MatInvN
 // (MatInvLib.h:17-77) is the same unpivoted elimination written as LU.
+// A pivot that is exactly zero belongs to a parameter without information
+// (every dudt of it is exactly 0: a coordinate on a mirror-padded bound of
+// the spline, where the derivative vanishes by symmetry): its row and column
+// are zero, so it decouples; its CRLB is +inf and the others come from the
+// rest of the matrix.  (The reference divides by its rounding residue there
+// and reports 1e7-1e31 for that parameter, the others as here.)
 template <int NV>
 __device__ __forceinline__ double inverse_diag_rows(double (&m)[NV], int lane)
 {
     double v[NV];
+    bool none = false;   // this lane's parameter has no information
 #pragma unroll
     for (int j = 0; j < NV; j++) v[j] = (j == lane) ? 1.0 : 0.0;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         const double mkk = readlane_d(m[k], k);
-        const double r = 1.0 / mkk;
-        const double fac = m[k] * r;
+        const bool zp = mkk == 0.0;
+        none = none || (zp && lane == k);
+        const double r = zp ? 0.0 : 1.0 / mkk;
+        const double fac = zp ? 0.0 : m[k] * r;
 #pragma unroll
         for (int j = 0; j < NV; j++) {
             const double mk = readlane_d(m[j], k), vk = readlane_d(v[j], k);
-            m[j] = (lane == k) ? mk * r : m[j] - fac * mk;
-            v[j] = (lane == k) ? vk * r : v[j] - fac * vk;
+            m[j] = (lane == k) ? (zp ? m[j] : mk * r) : m[j] - fac * mk;
+            v[j] = (lane == k) ? (zp ? v[j] : vk * r) : v[j] - fac * vk;
         }
     }
     double d = 0.0;
 #pragma unroll
     for (int j = 0; j < NV; j++) d = (j == lane) ? v[j] : d;
-    return d;
+    return none ? __builtin_inf() : d;
 }
 
 // Index of (row i, column j) in the packed upper triangle used by the sums.
@@ -837,6 +847,19 @@ __device__ __forceinline__ double rcp_f64(double d)
     return d == 0.0 ? __builtin_inf() : r;
 }
 
+// A parameter whose row of the system (and gradient entry) is exactly zero
+// has no information in this evaluation: every dudt of it is 0, which
+// happens exactly on the mirror-padded first sample of a spline axis, where
+// the fits' lower bounds clamp z (0), the colour w (0) and the polar u (1).
+// The reference's arithmetic leaves rounding residue there (|J| ~ 1e-16,
+// H ~ 1e-30), so its Newton step for that parameter is a ratio of residues
+// that the limiter caps at the parameter's clamp, and the bound keeps only
+// the steps leaving it: its fits walk off the bound.  Here such a row is
+// solved out (the rest of the system as it is) and the parameter gets
+// kEscapeStep, which the limiter turns into a step of its clamp upward, off
+// the bound; without it every later solve would fail and the fit freeze.
+constexpr double kEscapeStep = -1e30;
+
 template <int NV>
 __device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV])
 {
@@ -845,12 +868,20 @@ __device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV])
     for (int i = 0; i < NV; i++)
 #pragma unroll
         for (int j = 0; j <= i; j++) a[i][j] = jh[NV + packed_index<NV>(j, i)];
+    bool zr[NV];   // rows without information (see kEscapeStep)
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        bool z = jh[k] == 0.0;
+#pragma unroll
+        for (int j = 0; j < NV; j++) z = z && a[k > j ? k : j][k > j ? j : k] == 0.0;
+        zr[k] = z;
+    }
     double r[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         const double d = a[k][k];
         if (!(d >= 0.0)) return 1;
-        r[k] = rcp_f64(d);
+        r[k] = zr[k] ? 0.0 : rcp_f64(d);
         double t[NV];
 #pragma unroll
         for (int i = k + 1; i < NV; i++) {
@@ -876,16 +907,19 @@ __device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV])
 #pragma unroll
         for (int k = i + 1; k < NV; k++) x[i] -= a[k][i] * x[k];
     }
+#pragma unroll
+    for (int k = 0; k < NV; k++) x[k] = zr[k] ? kEscapeStep : x[k];
     return 0;
 }
 
 // Per-ROI status flags (enum bsf_roi_flag of bsfit.h) of a finished fit:
 // 1 the iteration limit ended it, 2 its last damped solve failed, 4 a CRLB
 // entry is not a finite positive number, 8 a parameter or the log-likelihood
-// is not finite.  Every lane of the wave calls it; `own` marks the lanes that
+// is not finite, 16 the damped solve of its first iteration failed.  Every lane of the wave calls it; `own` marks the lanes that
 // hold a parameter (th) and its CRLB.
 __device__ __forceinline__ void write_roi_status(unsigned char *status, int n, bool own, float th, float crlb,
-                                                 float ll, bool max_iter, bool solve_failed, int lane)
+                                                 float ll, bool max_iter, bool solve_failed, bool first_failed,
+                                                 int lane)
 {
     const unsigned long long badc = __ballot(own && !(crlb > 0.f && isfinite(crlb)));
     const unsigned long long badp = __ballot(own && !isfinite(th));
@@ -893,6 +927,7 @@ __device__ __forceinline__ void write_roi_status(unsigned char *status, int n, b
         unsigned f = 0;
         if (max_iter) f |= 1u;
         if (solve_failed) f |= 2u;
+        if (first_failed) f |= 16u;
         if (badc) f |= 4u;
         if (badp || !isfinite(ll)) f |= 8u;
         status[n] = (unsigned char)f;
@@ -980,6 +1015,11 @@ template <int NVMAX>
 __device__ __forceinline__ int chol_solve_rows_rt(double (&a)[NVMAX], double b, int lane, int nv, double &xi)
 {
     double colL[NVMAX], rinv[NVMAX];
+    // rows (and gradient entries) that are exactly zero: see kEscapeStep
+    bool mine = lane < nv && b == 0.0;
+#pragma unroll
+    for (int j = 0; j < NVMAX; j++) mine = mine && (j >= nv || a[j] == 0.0);
+    const unsigned long long zrows = __ballot(mine);
 #pragma unroll
     for (int j = 0; j < NVMAX; j++) colL[j] = 0.0, rinv[j] = 0.0;
 #pragma unroll
@@ -988,7 +1028,7 @@ __device__ __forceinline__ int chol_solve_rows_rt(double (&a)[NVMAX], double b, 
             const double akk = readlane_d(a[k], k);
             if (!(akk >= 0.0)) return 1;
             const double lkk = sqrt(akk);
-            rinv[k] = 1.0 / lkk;
+            rinv[k] = ((zrows >> k) & 1ull) ? 0.0 : 1.0 / lkk;
             const double lik = lane > k ? a[k] * rinv[k] : (lane == k ? lkk : 0.0);
             a[k] = lik;
 #pragma unroll
@@ -1020,6 +1060,7 @@ __device__ __forceinline__ int chol_solve_rows_rt(double (&a)[NVMAX], double b, 
             acc2 += colL[k] * xk;
         }
     }
+    if ((zrows >> lane) & 1ull) xi = kEscapeStep;
     return 0;
 }
 
@@ -1027,20 +1068,23 @@ template <int NVMAX>
 __device__ __forceinline__ double inverse_diag_rows_rt(double (&m)[NVMAX], int lane, int nv)
 {
     double v[NVMAX];
+    bool none = false;   // a zero pivot: as in inverse_diag_rows
 #pragma unroll
     for (int j = 0; j < NVMAX; j++) v[j] = (j == lane) ? 1.0 : 0.0;
 #pragma unroll
     for (int k = 0; k < NVMAX; k++) {
         if (k < nv) {
             const double mkk = readlane_d(m[k], k);
-            const double r = 1.0 / mkk;
-            const double fac = m[k] * r;
+            const bool zp = mkk == 0.0;
+            none = none || (zp && lane == k);
+            const double r = zp ? 0.0 : 1.0 / mkk;
+            const double fac = zp ? 0.0 : m[k] * r;
 #pragma unroll
             for (int j = 0; j < NVMAX; j++) {
                 if (j < nv) {
                     const double mk = readlane_d(m[j], k), vk = readlane_d(v[j], k);
-                    m[j] = (lane == k) ? mk * r : m[j] - fac * mk;
-                    v[j] = (lane == k) ? vk * r : v[j] - fac * vk;
+                    m[j] = (lane == k) ? (zp ? m[j] : mk * r) : m[j] - fac * mk;
+                    v[j] = (lane == k) ? (zp ? v[j] : vk * r) : v[j] - fac * vk;
                 }
             }
         }
@@ -1048,7 +1092,7 @@ __device__ __forceinline__ double inverse_diag_rows_rt(double (&m)[NVMAX], int l
     double d = 0.0;
 #pragma unroll
     for (int j = 0; j < NVMAX; j++) d = (j == lane) ? v[j] : d;
-    return d;
+    return none ? __builtin_inf() : d;
 }
 
 }  // namespace bsf

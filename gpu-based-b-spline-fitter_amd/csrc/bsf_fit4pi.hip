@@ -61,7 +61,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     const int Smax = A.S;
     if (A.rois) {
         const RoiDesc d = A.rois[n];
-        A.S = d.S;
+        A.S = d.S < 1 ? 1 : d.S > A.S ? A.S : d.S;   // within the LDS sized for the largest
         dbase = (size_t)d.off;
         cstride = (size_t)d.cstride;
     }
@@ -217,8 +217,9 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     };
 
     E newErr = evaluate(true);
+    int nev = 1;   // evaluations run (for the evals output)
     E oldErr = (E)1e13, newLambda = E(1), oldLambda = E(1);
-    int turn = 1, kk, lastinfo = 0;
+    int turn = 1, kk, lastinfo = 0, firstinfo = 0;
     const int zslot = A4.slot[4][0];
     for (kk = 0; kk < A.iterations; kk++) {
         if (fabs((newErr - oldErr) / newErr) < (E)1e-6) break;
@@ -245,6 +246,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
             if (__any(lane < nv && isnan(upd))) info = 1;
         }
         lastinfo = info;
+        if (kk == 0) firstinfo = info;
         if (info != 0) {
             newLambda = E(10) * newLambda;
             continue;
@@ -271,6 +273,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
         else if (myb == 4) th = dminr<E>(dmaxr<E>(th, 0.0), (double)(E)A.nd[2] - 0.5);
         else if (myb == 5) th = wrap_phase<E>(th, A.nd[3]);
         newErr = evaluate(false);
+        nev++;
     }
 
     // ---- Fisher information over all channels, CRLB, log-likelihood
@@ -330,10 +333,11 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     if (lane == 0) {
         A.P[(size_t)A.N * nv + n] = (float)kk;
         A.LL[n] = (float)s_err[0];
+        if (A.evals) A.evals[n] = nev + 1;
     }
     if (A.status)
         write_roi_status(A.status, n, lane < nv, (float)th, (float)crlb, (float)s_err[0], kk >= A.iterations,
-                         lastinfo != 0, lane);
+                         lastinfo != 0, firstinfo != 0, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,7 +404,7 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
     size_t dbase = (size_t)n * A.S * A.S, cstride = (size_t)A.N * A.S * A.S;
     if (A.rois) {
         const RoiDesc d = A.rois[n];
-        A.S = d.S;
+        A.S = d.S < 1 ? 1 : d.S > A.S ? A.S : d.S;   // within the LDS sized for the largest
         dbase = (size_t)d.off;
         cstride = (size_t)d.cstride;
     }
@@ -562,8 +566,9 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
     };
 
     E newErr = evaluate(true);
+    int nev = 1;   // evaluations run (for the evals output)
     E oldErr = (E)1e13, newLambda = E(1), oldLambda = E(1);
-    int turn = 1, kk, lastinfo = 0;
+    int turn = 1, kk, lastinfo = 0, firstinfo = 0;
     const int zslot = A4.slot[4][0];
     for (kk = 0; kk < A.iterations; kk++) {
         if (fabs((newErr - oldErr) / newErr) < (E)1e-6) break;
@@ -599,6 +604,7 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
             if (__any(lane < nv && isnan(upd))) info = 1;
         }
         lastinfo = info;
+        if (kk == 0) firstinfo = info;
         if (info != 0) {
             newLambda = E(10) * newLambda;
             continue;
@@ -625,6 +631,7 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
         else if (myb == 4) th = dminr<E>(dmaxr<E>(th, 0.0), (double)(E)A.nd[2] - 0.5);
         else if (myb == 5) th = wrap_phase<E>(th, A.nd[3]);
         newErr = evaluate(false);
+        nev++;
     }
 
     // ---- Fisher information, CRLB, log-likelihood: channel k on wave k
@@ -675,10 +682,11 @@ __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
     if (lane == 0) {
         A.P[(size_t)A.N * nv + n] = (float)kk;
         A.LL[n] = (float)s_err[0];
+        if (A.evals) A.evals[n] = nev + 1;
     }
     if (A.status)
         write_roi_status(A.status, n, lane < nv, (float)th, (float)crlb, (float)s_err[0], kk >= A.iterations,
-                         lastinfo != 0, lane);
+                         lastinfo != 0, firstinfo != 0, lane);
 }
 
 size_t lds_bytes_4pi_wpc(int f64, int S, int nch, int nv) { return lds_4pi_wpc(S, nch, f64 ? 8 : 4, nv).total; }
@@ -739,6 +747,7 @@ hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st)
     A.theta_in = nullptr;
     A.dbg = nullptr;
     A.status = L.status;
+    A.evals = L.evals;
     A4.nch = L.nch;
     int n = 0;
     for (int p = 0; p < 6; p++) {

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     const float *dptr = A.data + (size_t)n * Smax * Smax;
     if (A.rois) {
         const RoiDesc d = A.rois[n];
-        A.S = d.S;
+        A.S = d.S < 1 ? 1 : d.S > Smax ? Smax : d.S;   // within the LDS sized for the largest
         dptr = A.data + d.off;
     }
     if (SC) A.S = SC;
@@ -255,9 +255,10 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         return;
     }
     E newErr = evaluate(true);
+    int nev = 1;   // evaluations run (for the evals output)
     E oldErr = (E)1e13;
     E newLambda = E(1), oldLambda = E(1), sumUpdate = (E)1e10;
-    int turn = 1, kk, lastinfo = 0;
+    int turn = 1, kk, lastinfo = 0, firstinfo = 0;
     for (kk = 0; kk < A.iterations; kk++) {
         if (fabs((newErr - oldErr) / newErr) < (E)1e-6 || (FT == 8 && sumUpdate < (E)1e-5)) break;   // TOLERANCE, TOLERANCE1
         if ((double)newErr > 1.5 * (double)oldErr) {
@@ -292,6 +293,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             }
         }
         lastinfo = info;
+        if (kk == 0) firstinfo = info;
         if (info != 0) {
             newLambda = E(10) * newLambda;
             continue;
@@ -360,6 +362,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             }
         }
         newErr = evaluate(false);
+        nev++;
     }
 
     // ---- Fisher information, CRLB, log-likelihood.  Fittypes 6/7 evaluate
@@ -427,6 +430,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     for (int j = 0; j < NV; j++) mrow[j] = (double)tot[1 + packed_index<NV>(myrow, j)];
     const double crlb = inverse_diag_rows<NV>(mrow, lane);
     const float llf = (float)tot[0];
+    if (A.evals && lane == 0) A.evals[n] = nev + 1;
     // A later start of a multi-start fit replaces the kept result only where
     // its log-likelihood beats it by 1e-4, compared in FP32 as the
     // reference's callers compare the MEX outputs
@@ -453,6 +457,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             unsigned f = 0;
             if (kk >= A.iterations) f |= 1u;
             if (lastinfo != 0) f |= 2u;
+            if (firstinfo != 0) f |= 16u;
             if (badc) f |= 4u;
             if (badp || !isfinite(llf)) f |= 8u;
             A.status[n] = (unsigned char)f;

@@ -63,6 +63,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     A.theta_in = L.theta_in;
     A.dbg = L.dbg;
     A.status = L.status;
+    A.evals = L.evals;
     const int nb = L.count;
     const int sized = !L.rois || L.uniform ? L.S : 0;
     switch (L.fittype * 2 + (L.f64 ? 1 : 0)) {

@@ -36,6 +36,7 @@ struct FitLaunch {
     const float *theta_in;  // test hook (device): NV parameters per ROI, or NULL
     float *dbg;             // test hook (device): err, J, H per ROI
     unsigned char *status;  // device: per-ROI status flags (bsf_roi_flag), or NULL
+    int *evals;             // device: per-ROI evaluations run, or NULL
 };
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st);
@@ -57,6 +58,7 @@ struct Fit4PiLaunch {
     const float *initZ, *initPhi;  // device [N] or NULL
     int wpc;                  // 1: a workgroup of nch waves per ROI, one per channel; 0: one wave per ROI
     unsigned char *status;    // device: per-ROI status flags (bsf_roi_flag), or NULL
+    int *evals;               // device: per-ROI evaluations run, or NULL
 };
 
 hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st);

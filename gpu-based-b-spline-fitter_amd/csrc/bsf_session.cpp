@@ -350,7 +350,7 @@ int bsf_session_destroy(bsf_session *s)
         for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
         if (s->s_in) (void)hipStreamDestroy(s->s_in);
         if (s->s_out) (void)hipStreamDestroy(s->s_out);
-        for (int j = 0; j < 8; j++)
+        for (int j = 0; j < kMaxKernelStreams; j++)   // entries past nks are null
             if (s->ks[j]) (void)hipStreamDestroy(s->ks[j]);
         if (s->d_arena) (void)hipFree(s->d_arena);
         if (s->h_arena) (void)hipHostFree(s->h_arena);

@@ -10,6 +10,7 @@
 #include <string>
 #include <cmath>
 #include <cstdlib>
+#include <cstddef>
 
 #include "../../include/bsfit.h"
 #include "bsf_kernels.h"
@@ -104,6 +105,10 @@ int validate(const bsf_request *r, const bsf_result *res)
         if (r->coeff_dims[a] < 5)
             return fail(BSF_ERR_SHAPE, "bsf: coefficient dim %d is %d; each axis needs >= 2 data samples (+3)", a,
                         r->coeff_dims[a]);
+    // the kernels address the arrays with 32-bit byte offsets (buffer records)
+    if (coeff_numel(r, D) * 4 >= ((size_t)1 << 31))
+        return fail(BSF_ERR_SHAPE, "bsf: the coefficient array has %zu elements; at most 2^29 - 1 are supported",
+                    coeff_numel(r, D));
     if (r->fittype == 8 && !r->coeff3)
         return fail(BSF_ERR_ARG, "bsf: fittype 8 needs coeff3 (the 3D coefficient array)");
     if (r->initP) {
@@ -194,7 +199,8 @@ int bsf_num_params(int fittype) { return nv_of(fittype); }
 
 const char *bsf_last_error(void) { return g_err.c_str(); }
 
-static int fit_device(const bsf_request *r, bsf_result *res, unsigned char *status, void *stream)
+static int fit_device(const bsf_request *r, bsf_result *res, unsigned char *status, void *stream,
+                      const bsf_roi *rois = nullptr, int *evals = nullptr)
 {
     int rc = validate(r, res);
     if (rc) return rc;
@@ -202,6 +208,8 @@ static int fit_device(const bsf_request *r, bsf_result *res, unsigned char *stat
     if (r->nfits == 0) return BSF_OK;
     bsf::FitLaunch L = make_launch(r, r->imstack, r->coeff, r->coeff3, res->P, res->CRLB, res->LL);
     L.status = status;
+    L.evals = evals;
+    L.rois = (const bsf::RoiDesc *)rois;
     hipError_t e = launch_all(L, (hipStream_t)stream);
     if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
     return BSF_OK;
@@ -218,6 +226,18 @@ int bsf_fit_device_status(const bsf_request *r, bsf_result *res, unsigned char *
     g_err.clear();
     if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
     return fit_device(r, res, status, stream);
+}
+
+static_assert(sizeof(bsf_roi) == sizeof(bsf::RoiDesc) && offsetof(bsf_roi, off) == offsetof(bsf::RoiDesc, off) &&
+                  offsetof(bsf_roi, cstride) == offsetof(bsf::RoiDesc, cstride) &&
+                  offsetof(bsf_roi, sz) == offsetof(bsf::RoiDesc, S),
+              "bsf_roi and the kernels' ROI descriptor share one layout");
+
+int bsf_fit_device_ex(const bsf_request *r, bsf_result *res, const bsf_fit_extras *x, void *stream)
+{
+    g_err.clear();
+    if (!x) return fit_device(r, res, nullptr, stream);
+    return fit_device(r, res, x->status, stream, x->rois, x->evals);
 }
 
 // Host-array fit from nstarts starts (rows of `starts`, n_initP values each;
@@ -456,6 +476,9 @@ static int validate_4pi(const bsf_request_4pi *r, const bsf_result *res)
     if (!r->coeff || !r->phaseShift) return fail(BSF_ERR_ARG, "bsf: coeff and phaseShift must not be NULL");
     for (int a = 0; a < 4; a++)
         if (r->coeff_dims[a] < 5) return fail(BSF_ERR_SHAPE, "bsf: coefficient dim %d is %d (needs >= 5)", a, r->coeff_dims[a]);
+    const size_t nc = (size_t)r->coeff_dims[0] * r->coeff_dims[1] * r->coeff_dims[2] * r->coeff_dims[3];
+    if (nc * 4 >= ((size_t)1 << 31))
+        return fail(BSF_ERR_SHAPE, "bsf: the coefficient array has %zu elements; at most 2^29 - 1 are supported", nc);
     if (r->precision != BSF_PREC_MIX && r->precision != BSF_PREC_F64)
         return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", r->precision);
     return BSF_OK;
@@ -528,7 +551,8 @@ int bsf_test_set_4pi_kernel(int mode)
     return BSF_OK;
 }
 
-static int fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, unsigned char *status, void *stream)
+static int fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, unsigned char *status, void *stream,
+                          const bsf_roi *rois = nullptr, int *evals = nullptr)
 {
     int rc = validate_4pi(r, res);
     if (rc) return rc;
@@ -537,6 +561,8 @@ static int fit_4pi_device(const bsf_request_4pi *r, bsf_result *res, unsigned ch
     bsf::Fit4PiLaunch L = make_launch_4pi(r, r->imstack, r->coeff, r->dT, r->initZ, r->initPhi, res->P, res->CRLB,
                                           res->LL);
     L.status = status;
+    L.evals = evals;
+    L.rois = (const bsf::RoiDesc *)rois;
     hipError_t e = launch_all_4pi(L, (hipStream_t)stream);
     if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
     return BSF_OK;
@@ -553,6 +579,13 @@ int bsf_fit_4pi_device_status(const bsf_request_4pi *r, bsf_result *res, unsigne
     g_err.clear();
     if (!status) return fail(BSF_ERR_ARG, "bsf: NULL status array");
     return fit_4pi_device(r, res, status, stream);
+}
+
+int bsf_fit_4pi_device_ex(const bsf_request_4pi *r, bsf_result *res, const bsf_fit_extras *x, void *stream)
+{
+    g_err.clear();
+    if (!x) return fit_4pi_device(r, res, nullptr, stream);
+    return fit_4pi_device(r, res, x->status, stream, x->rois, x->evals);
 }
 
 static int fit_4pi_host(const bsf_request_4pi *r, bsf_result *res, unsigned char *status)

@@ -110,7 +110,12 @@ enum bsf_roi_flag {
                                     the Fisher information at the result is
                                     singular or indefinite (e.g. a fit ending where
                                     the spline's z derivative vanishes)           */
-    BSF_ROI_NONFINITE = 8        /* a parameter or the log-likelihood is not finite */
+    BSF_ROI_NONFINITE = 8,       /* a parameter or the log-likelihood is not finite */
+    BSF_ROI_FIRST_SOLVE_FAILED = 16 /* the damped normal equations of the first
+                                    iteration had no solution (fittype 7's first
+                                    Hessian is singular by construction,
+                                    CPUmleFit_LM.cpp:2896, so rounding decides
+                                    whether its first step is taken)             */
 };
 
 /* bsf_fit_host / bsf_fit_device plus the per-ROI status flags: `status` holds
@@ -118,6 +123,39 @@ enum bsf_roi_flag {
  * device one).  The fits are the same, bit for bit. */
 int bsf_fit_host_status(const bsf_request *req, bsf_result *res, unsigned char *status);
 int bsf_fit_device_status(const bsf_request *req, bsf_result *res, unsigned char *status, void *stream);
+
+/* Placement and size of one ROI in a launch that mixes ROI sizes: its
+ * column-major sz x sz image is at imstack + off (floats), 4Pi channel k at
+ * imstack + off + k * cstride.  No counterpart in the reference, whose MEX
+ * takes one ROI size per call (GPUmleFit_LM/mexFunction.cpp:251-263); for
+ * callers that fit ROIs of several sizes (a frame's ROIs cut at the size each
+ * emitter needs) in one launch. */
+typedef struct bsf_roi {
+    long long off;
+    int cstride;
+    int sz;
+} bsf_roi;
+
+/* Optional device-side inputs and outputs of the extended entry points
+ * (each may be NULL):
+ *   rois    [nfits] per-ROI placement and size; req->sz is then the largest
+ *           size of the launch (every rois[n].sz in 1..req->sz: the LDS of a
+ *           workgroup is sized for it), imstack the base `off` is counted from
+ *   status  [nfits] per-ROI flags (enum bsf_roi_flag)
+ *   evals   [nfits] ROI evaluations each fit ran (model, derivatives and
+ *           pixel sums over the whole ROI; the initial one, one per accepted or
+ *           rejected step, and the final Fisher pass) */
+typedef struct bsf_fit_extras {
+    const bsf_roi *rois;
+    unsigned char *status;
+    int *evals;
+} bsf_fit_extras;
+
+/* bsf_fit_device (the launch wrappers, GPUmleFit_LM/wrapper.cu:256-279) with
+ * the extras above; x == NULL is bsf_fit_device.  With rois every ROI is
+ * fitted by the same kernel and code as a uniform launch of its size; a
+ * launch of mixed sizes gives each ROI the results it has alone, bit for bit. */
+int bsf_fit_device_ex(const bsf_request *req, bsf_result *res, const bsf_fit_extras *x, void *stream);
 
 /* Multi-start fit: every ROI is fitted from nstarts starts (rows of
  * `starts`, req->n_initP values each: fittype 7 {z, w}, fittype 8
@@ -207,6 +245,9 @@ int bsf_fit_4pi_device(const bsf_request_4pi *req, bsf_result *res, void *stream
  * status array of nfits bytes. */
 int bsf_fit_4pi_host_status(const bsf_request_4pi *req, bsf_result *res, unsigned char *status);
 int bsf_fit_4pi_device_status(const bsf_request_4pi *req, bsf_result *res, unsigned char *status, void *stream);
+/* bsf_fit_4pi_device with the extras of bsf_fit_device_ex (rois: per-ROI
+ * placement of channel 0 and the channel stride). */
+int bsf_fit_4pi_device_ex(const bsf_request_4pi *req, bsf_result *res, const bsf_fit_extras *x, void *stream);
 
 /* Text of the last error on this thread ("" if none).
  * Replaces mexErrMsgIdAndTxt (GPUmleFit_LM/mexFunction.cpp:140-176). */

@@ -9,6 +9,21 @@ import numpy as np
 
 COORD = {6: [0, 1, 4], 7: [0, 1, 4, 5], 8: [0, 1, 4, 5, 6, 7]}
 REL = {6: [2, 3], 7: [2, 3], 8: [2, 3]}
+# A CRLB (variance) above this, or infinite, means the parameter has no
+# information (e.g. a coordinate on a mirror-padded bound of the spline, where
+# its derivative vanishes): the kernel reports +inf there, the reference's
+# arithmetic its rounding residue's 1e7-1e31.  Two such entries agree.
+NO_INFO = 1e12
+
+
+def _crlb_rel(CR, CRo):
+    CR = np.asarray(CR, np.float64)
+    CRo = np.asarray(CRo, np.float64)
+    big, bigo = ~(CR <= NO_INFO), ~(CRo <= NO_INFO)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        d = np.abs(CR - CRo) / np.maximum(np.abs(CRo), 1e-30)
+    d = np.where(big & bigo, 0.0, d)
+    return np.where(np.isnan(d), np.inf, d)
 
 
 def compare(ft, P, CR, LL, Po, CRo, LLo, atol=1e-4, rtol=1e-4):
@@ -17,7 +32,7 @@ def compare(ft, P, CR, LL, Po, CRo, LLo, atol=1e-4, rtol=1e-4):
     Po = np.asarray(Po, np.float64)
     dcoord = np.abs(P[:, COORD[ft]] - Po[:, COORD[ft]]).max(1)
     drel = (np.abs(P[:, REL[ft]] - Po[:, REL[ft]]) / np.maximum(np.abs(Po[:, REL[ft]]), 1e-12)).max(1)
-    dcr = (np.abs(CR - CRo) / np.maximum(np.abs(CRo), 1e-30)).max(1)
+    dcr = _crlb_rel(CR, CRo).max(1)
     same_kk = P[:, NV] == Po[:, NV]
     ok = (dcoord <= atol) & (drel <= rtol) & (dcr <= rtol) & same_kk
     return dict(n=len(P), agree=float(ok.mean()), same_kk=float(same_kk.mean()),
@@ -26,6 +41,43 @@ def compare(ft, P, CR, LL, Po, CRo, LLo, atol=1e-4, rtol=1e-4):
                 p99_dcoord=float(np.percentile(dcoord, 99)), max_dcoord=float(dcoord.max()),
                 p99_drel=float(np.percentile(drel, 99)), p99_dcrlb=float(np.percentile(dcr, 99)),
                 finite=bool(np.all(np.isfinite(P)) and np.all(np.isfinite(CR))))
+
+
+def errors(ft, P, CR, Po, CRo, same_tol=1e-3):
+    """Error of a fit against a reference fit of the same ROIs.
+
+    coord: quantiles (50, 90, 99, max) over all ROIs of the largest
+    coordinate error (px; spline units for z, w, u; absolute for g2), which
+    includes ROIs whose LM trajectories ended at different points.
+    same: the fraction of ROIs that reached the same optimum (coordinates
+    within same_tol); rel / crlb: over those ROIs, quantiles of the largest
+    relative N / bg error and of the largest relative CRLB error (entries
+    without information on both sides count as equal) -- the precision of
+    the fit at a given optimum."""
+    P = np.asarray(P, np.float64)
+    Po = np.asarray(Po, np.float64)
+    dcoord = np.abs(P[:, COORD[ft]] - Po[:, COORD[ft]]).max(1)
+    drel = (np.abs(P[:, REL[ft]] - Po[:, REL[ft]]) / np.maximum(np.abs(Po[:, REL[ft]]), 1e-12)).max(1)
+    dcr = _crlb_rel(CR, CRo).max(1)
+    same = dcoord <= same_tol
+
+    def q(a):
+        if len(a) == 0:
+            return [np.nan] * 4
+        return [float(np.percentile(a, 50)), float(np.percentile(a, 90)), float(np.percentile(a, 99)),
+                float(a.max())]
+    return dict(coord=q(dcoord), same=float(same.mean()), rel=q(drel[same]), crlb=q(dcr[same]))
+
+
+def fmt_errors(e):
+    return ("coord p50 %.1e p90 %.1e p99 %.1e max %.1e | same optimum %.3f: N,bg rel p50 %.1e p90 %.1e p99 %.1e"
+            " | CRLB rel p50 %.1e p90 %.1e p99 %.1e" % (*e["coord"], e["same"], *e["rel"][:3], *e["crlb"][:3]))
+
+
+def crlb_nan_free(CR):
+    """No CRLB entry is NaN (an exactly singular Fisher matrix gives +inf for
+    the parameter without information, not NaN everywhere)."""
+    return not np.any(np.isnan(np.asarray(CR)))
 
 
 class Hip:

@@ -153,3 +153,53 @@ def test_front_end_argument_errors_and_host_kernels():
         with pytest.raises(bsfit.BsfError) as e:
             FR.find_rois(fr, 7, h, 1.0, capacity=4)
         assert e.value.code == 6
+
+
+def test_oversized_coefficient_arrays_are_refused():
+    """The kernels address coefficients with 32-bit byte offsets: arrays of
+    2^31 bytes or more are refused (BSF_ERR_SHAPE) before any device work,
+    by the 6/7/8 fit and by the 4Pi fit (no array that large is allocated:
+    the checks read only the dimensions)."""
+    import bsfit
+    from bsfit.api import Request, Result, Request4Pi, Extras, _request
+    lib = bsfit.load()
+    buf = np.zeros(16, np.float32)
+    P = np.zeros(16, np.float32)
+    res = Result(P.ctypes.data, P.ctypes.data, P.ctypes.data)
+    req = _request(6, 13, 1, 50, buf.ctypes.data, (1024, 1024, 600), None, None, bsfit.PREC_MIX, buf.ctypes.data)
+    lib.bsf_fit_device.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_void_p]
+    assert lib.bsf_fit_device(C.byref(req), C.byref(res), None) == 4
+    assert "2^29" in lib.bsf_last_error().decode()
+    lib.bsf_fit_device_ex.argtypes = [C.POINTER(Request), C.POINTER(Result), C.POINTER(Extras), C.c_void_p]
+    x = Extras(None, None, None)
+    assert lib.bsf_fit_device_ex(C.byref(req), C.byref(res), C.byref(x), None) == 4
+    # just below the limit: the same check passes (then no device: BSF_ERR_DEVICE, or BSF_OK on a GPU box
+    # would need real memory, so only the error code path is exercised where no GPU is present)
+    import torch
+    if not torch.cuda.is_available():
+        req = _request(6, 13, 1, 50, buf.ctypes.data, (1024, 1024, 511), None, None, bsfit.PREC_MIX,
+                       buf.ctypes.data)
+        assert lib.bsf_fit_device(C.byref(req), C.byref(res), None) == 6
+    sh = np.ones(6, np.uint32)
+    ps = np.zeros(4, np.float32)
+    r = Request4Pi()
+    r.imstack, r.sz, r.nfits, r.nch = buf.ctypes.data, 13, 1, 4
+    r.shared, r.iterations, r.coeff = sh.ctypes.data, 50, buf.ctypes.data
+    for a, d in enumerate((1024, 1024, 64, 20)):
+        r.coeff_dims[a] = d
+    r.phaseShift, r.precision = ps.ctypes.data, bsfit.PREC_MIX
+    lib.bsf_fit_4pi_device.argtypes = [C.POINTER(Request4Pi), C.POINTER(Result), C.c_void_p]
+    assert lib.bsf_fit_4pi_device(C.byref(r), C.byref(res), None) == 4
+    lib.bsf_fit_4pi_device_ex.argtypes = [C.POINTER(Request4Pi), C.POINTER(Result), C.POINTER(Extras), C.c_void_p]
+    assert lib.bsf_fit_4pi_device_ex(C.byref(r), C.byref(res), C.byref(x), None) == 4
+
+
+def test_roi_table_layout():
+    """bsfit.roi_table packs bsf_roi records (16 bytes: off, cstride, sz)
+    back to back by default."""
+    import bsfit
+    from bsfit.api import RoiDesc
+    t = bsfit.roi_table([11, 13, 17])
+    assert t.dtype.itemsize == C.sizeof(RoiDesc) == 16
+    assert list(t["off"]) == [0, 121, 121 + 169]
+    assert list(t["sz"]) == [11, 13, 17]

@@ -46,8 +46,15 @@ def test_large_roi_matches_oracle(orc, ft, cfg, S, N, min_f64):
     w = w[0] if isinstance(w, list) else w
     print("case fittype=%d S=%d N=%d" % (ft, w.S, w.N), flush=True)
     gm = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3)
-    of = orc.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
+    of = orc.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=16)
+    od = orc.fit_f64(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=16)
     smx = compare(ft, *gm, *of)
+    sref = compare(ft, *of, *od)
+    print("orcF32 vs orcF64", sref, flush=True)
+    # the FP32 kernel no further from the reference's float arithmetic than
+    # that is from FP64 (a wide margin: 12-16 ROIs per case)
+    assert smx["agree"] >= sref["agree"] - 0.25
+    assert not np.any(np.isnan(gm[1]))
     llm = _ll_not_worse(gm[2], of[2])
     assert np.all(np.isfinite(gm[0])), "non-finite parameters"
     eg = np.median(np.abs(gm[0][:, :2] - w.truth[:, :2]), 0)
@@ -60,7 +67,6 @@ def test_large_roi_matches_oracle(orc, ft, cfg, S, N, min_f64):
         assert e.value.code == 3
     else:
         gd = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, precision=bsfit.PREC_F64)
-        od = orc.fit_f64(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
         s64 = compare(ft, *gd, *od)
         print("gpuF64 vs orcF64", s64, flush=True)
         assert np.all(np.isfinite(gd[0]))

@@ -4,36 +4,49 @@ oracle on the same seeded inputs.
 The reference's LM control (accept / reject on |d err / err| < 1e-6 and
 err > 1.5 old err, multiplicative damping, sign-flip clamp fading, bounds
 and wraps) makes iteration counts and late trajectory steps sensitive to
-rounding: the oracle's own float build and its FP64 build disagree on
-15-75 % of ROIs (ft7 / ft8 most), and reordering only its Cholesky flips
-another ~20 % of ft7 / ft8 ROIs.  So per-ROI agreement is measured at the
-per-ROI tolerances of parity_util.compare and asserted as rates:
+rounding: the oracle's own float build and its FP64 build agree (P within
+1e-4 px / 1e-4 relative, CRLB within 1e-4 relative, same iteration count) on
+only 87 % (fittype 6), 28 % (7) and 20-27 % (8) of ROIs.  So per-ROI
+agreement is asserted as rates, each against the reference's own spread:
 
-* FP64 kernel vs FP64 oracle: same arithmetic class, different summation
-  order; high agreement expected for ft6, lower for the chaotic ft7 / ft8.
-* FP32 kernel vs the oracle's float (reference-semantics) build: the rate
-  is printed beside the oracle's own float-vs-FP64 rate; asserted are the
-  objective (the GPU's final log-likelihood is not worse than the oracle's on
-  the large majority of ROIs) and matched accuracy (median position error
-  against the simulation truth no worse than the oracle's, within 20 %).
+* FP64 kernel vs FP64 oracle (same arithmetic class, different summation
+  order): floors just under the measured 0.99 / 0.53-0.55 / 0.86-0.90;
+* FP32 kernel (MIX: FP32 evaluation, FP64 solve) vs the oracle's float
+  build (the reference's arithmetic): at least the float build's own
+  agreement with the FP64 build, less 0.10;
+* MIX vs the FP64 oracle: at least that same self-agreement, less 0.05;
+  coordinate errors (median, 90th percentile) no larger than the float
+  build's own against the FP64 oracle (x 1.5), the median within 1e-4; at
+  the same optimum (coordinates within 1e-3, reached as often as by the
+  float build) N, bg and CRLB within 1e-4 relative at the median, 1e-3 at
+  the 90th percentile and no worse than the float build's (x 1.5);
+* the objective: the GPU's final log-likelihood not worse than the
+  oracle's on the large majority of ROIs; accuracy against the simulation
+  truth no worse than the oracle's;
+* CRLB: no NaN; an infinite entry only for a parameter on a mirror-padded
+  bound (no information there).
 
 The model, derivatives and pixel sums themselves are pinned strictly by
-test_gpu_eval.py.
+test_gpu_eval.py; the first LM steps by the short trajectories below.
 """
 import numpy as np
 import pytest
 
-from parity_util import compare
+from parity_util import compare, errors, fmt_errors, crlb_nan_free
 
 pytestmark = pytest.mark.gpu
 
-# (fittype, cfg, S, N, min FP64 agreement)
+# (fittype, cfg, S, N, min FP64-vs-FP64 agreement)
 CASES = [
-    (6, 0, None, 256, 0.95),
-    (7, 1, None, 128, 0.30),
-    (8, 3, 11, 48, 0.65),
-    (8, 3, 17, 32, 0.65),
+    (6, 0, None, 512, 0.97),
+    (7, 1, None, 384, 0.45),
+    (8, 3, 11, 256, 0.78),
+    (8, 3, 17, 192, 0.78),
 ]
+
+# parameter index and bound value at which a parameter has no information
+# (the spline's derivative vanishes on its mirror-padded first sample)
+NO_INFO_BOUNDS = {6: [(4, 0.0)], 7: [(4, 0.0), (5, 0.0)], 8: [(4, 0.0), (6, 1.0)]}
 
 
 def _ll_not_worse(LLg, LLo):
@@ -41,6 +54,19 @@ def _ll_not_worse(LLg, LLo):
     LLo = np.asarray(LLo, np.float64)
     ok = np.isfinite(LLg) & (LLg >= LLo - 1e-3 * np.abs(LLo) - 1e-3)
     return float(ok.mean())
+
+
+def _check_crlb(ft, P, CR):
+    assert crlb_nan_free(CR), "NaN CRLB"
+    inf = ~np.isfinite(CR)
+    on_bound = np.zeros_like(inf)
+    for a, v in NO_INFO_BOUNDS[ft]:
+        on_bound[:, a] = P[:, a] == v
+    # fittype 8 at u = 1: the polar angle is 0, the azimuth has no information either
+    if ft == 8:
+        on_bound[:, 5] |= P[:, 6] == 1.0
+    assert np.all(on_bound[inf]), "infinite CRLB off the bounds"
+    return float(np.mean(inf.any(1)))
 
 
 @pytest.mark.parametrize("ft,cfg,S,N,min_f64", CASES)
@@ -52,16 +78,40 @@ def test_fit_matches_oracle(orc, ft, cfg, S, N, min_f64):
     print("case fittype=%d cfg=%d S=%d N=%d" % (ft, cfg, w.S, w.N), flush=True)
     gm = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3)
     gd = bsfit.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, precision=bsfit.PREC_F64)
-    of = orc.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
-    od = orc.fit_f64(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=8)
+    of = orc.fit(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=16)
+    od = orc.fit_f64(ft, w.imstack, 50, w.coeff, w.initP, w.coeff3, nthreads=16)
     s64 = compare(ft, *gd, *od)
     smx = compare(ft, *gm, *of)
+    smx64 = compare(ft, *gm, *od)
     sref = compare(ft, *of, *od)
     llm = _ll_not_worse(gm[2], of[2])
-    print("gpuF64 vs orcF64", s64, "\ngpuMIX vs orcF32", smx, "\norcF32 vs orcF64", sref,
-          "\nLL(gpuMIX) >= LL(orcF32) on %.3f" % llm, flush=True)
+    e_mix = errors(ft, gm[0], gm[1], od[0], od[1])
+    e_ref = errors(ft, of[0], of[1], od[0], od[1])
+    print("agreement: gpuF64~orcF64 %.3f  gpuMIX~orcF32 %.3f  gpuMIX~orcF64 %.3f  orcF32~orcF64 %.3f"
+          % (s64["agree"], smx["agree"], smx64["agree"], sref["agree"]))
+    print("gpuMIX - orcF64:", fmt_errors(e_mix))
+    print("orcF32 - orcF64:", fmt_errors(e_ref))
+    print("LL(gpuMIX) >= LL(orcF32) on %.3f" % llm, flush=True)
     assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0])), "non-finite parameters"
+    inf_mix = _check_crlb(ft, gm[0], gm[1])
+    inf_f64 = _check_crlb(ft, gd[0], gd[1])
+    print("ROIs with a parameter without information (CRLB inf): MIX %.3f, F64 %.3f" % (inf_mix, inf_f64))
     assert s64["agree"] >= min_f64
+    assert smx["agree"] >= sref["agree"] - 0.10
+    assert smx64["agree"] >= sref["agree"] - 0.05
+    # coordinates over all ROIs: median within the tolerance, median and 90th
+    # percentile no larger than the reference arithmetic's own (x 1.5)
+    assert e_mix["coord"][0] <= 1e-4, e_mix["coord"]
+    assert e_mix["coord"][0] <= 1.5 * e_ref["coord"][0] + 1e-6, (e_mix["coord"], e_ref["coord"])
+    assert e_mix["coord"][1] <= 1.5 * e_ref["coord"][1] + 1e-5, (e_mix["coord"], e_ref["coord"])
+    # at the same optimum (as often as the reference's arithmetic reaches it):
+    # N, bg and CRLB within 1e-4 relative at the median and 1e-3 at the 90th
+    # percentile, and no worse than the reference's own (x 1.5)
+    assert e_mix["same"] >= e_ref["same"] - 0.05, (e_mix["same"], e_ref["same"])
+    for k in ("rel", "crlb"):
+        assert e_mix[k][0] <= 1e-4, (k, e_mix[k])
+        assert e_mix[k][1] <= 1e-3, (k, e_mix[k])
+        assert e_mix[k][1] <= 1.5 * e_ref[k][1] + 1e-5, (k, e_mix[k], e_ref[k])
     assert llm >= 0.75
     eg = np.median(np.abs(gm[0][:, :2] - w.truth[:, :2]), 0)
     eo = np.median(np.abs(of[0][:, :2] - w.truth[:, :2]), 0)
@@ -106,27 +156,41 @@ def test_single_roi_and_iterations_zero(orc):
     assert np.allclose(LL0, LLo, rtol=1e-5, atol=1e-3)
 
 
-@pytest.mark.parametrize("ft,cfg,S", [(6, 0, None), (8, 3, 11), (8, 3, 17)])
+@pytest.mark.parametrize("ft,cfg,S", [(6, 0, None), (7, 1, None), (8, 3, 11), (8, 3, 17)])
 @pytest.mark.parametrize("iters", [1, 2, 4])
 def test_short_trajectories_match_f64_oracle(orc, ft, cfg, S, iters):
     """The first LM steps (initial estimates, accept / reject, damping,
     solve, limiter, clamp fading, bounds, wraps and the fittype-8 extra
     convergence test) before rounding differences grow: the FP64 kernel
-    against the FP64 oracle to 1e-5.  Fittype 7 is absent on purpose: its
-    first pass copies dudt[4] into dudt[5] (CPUmleFit_LM.cpp:2896), so the
-    first Hessian has two equal rows and whether its Cholesky passes is
-    decided by rounding (test_oracle_pins.py::test_ft7_first_hessian_singular)."""
+    against the FP64 oracle to 1e-5.  Fittype 7's first pass copies dudt[4]
+    into dudt[5] (CPUmleFit_LM.cpp:2896), so the first Hessian has two equal
+    rows and whether its Cholesky passes is decided by rounding
+    (test_oracle_pins.py::test_ft7_first_hessian_singular): both sides report
+    that outcome (status bit 16, a rounding coin flip on each side).  Where
+    both failed it, both continue from the same state with the damping
+    raised tenfold, where the system is no longer singular: the trajectories
+    are compared there."""
     import bsfit
     from bsfit import gen
-    w = gen.config(cfg, N=48, S=S)
+    w = gen.config(cfg, N=96 if ft == 7 else 48, S=S)
     w = w[0] if isinstance(w, list) else w
-    P, CR, LL = bsfit.fit(ft, w.imstack, iters, w.coeff, w.initP, w.coeff3, precision=bsfit.PREC_F64)
-    Po, CRo, LLo = orc.fit_f64(ft, w.imstack, iters, w.coeff, w.initP, w.coeff3, nthreads=8)
+    P, CR, LL, st = bsfit.fit_status(ft, w.imstack, iters, w.coeff, w.initP, w.coeff3, precision=bsfit.PREC_F64)
+    Po, CRo, LLo, sto = orc.fit_f64(ft, w.imstack, iters, w.coeff, w.initP, w.coeff3, nthreads=16, status=True)
     NV = CR.shape[1]
     rel = np.abs(P[:, :NV] - Po[:, :NV]) / np.maximum(np.abs(Po[:, :NV]), 1.0)
     ok = (rel.max(1) < 1e-5) & (P[:, NV] == Po[:, NV])
-    print("fittype", ft, "S", w.S, "iterations", iters, "agree", ok.mean(), flush=True)
-    assert ok.mean() >= 0.95
+    first = (st & bsfit.ROI_FIRST_SOLVE_FAILED) != 0
+    firsto = (sto & bsfit.ROI_FIRST_SOLVE_FAILED) != 0
+    both = first & firsto
+    print("fittype %d S %d iterations %d: first solve failed GPU %.3f oracle %.3f, both %.3f; "
+          "agree %.3f (where both failed it %.3f)" % (ft, w.S, iters, first.mean(), firsto.mean(), both.mean(),
+                                                     ok.mean(), ok[both].mean() if both.any() else -1), flush=True)
+    if ft == 7:
+        assert both.mean() >= 0.4
+        assert ok[both].mean() >= 0.95
+    else:
+        assert np.all(first == firsto)
+        assert ok.mean() >= 0.95
 
 
 @pytest.mark.parametrize("ft,cfg,N,shift,min_f64", [(6, 0, 128, 12.0, 0.9), (7, 1, 96, 12.0, 0.25)])
@@ -144,8 +208,8 @@ def test_negative_pixels_match_oracle(orc, ft, cfg, N, shift, min_f64):
     assert frac_neg > 0.05
     gm = bsfit.fit(ft, im, 50, w.coeff, w.initP)
     gd = bsfit.fit(ft, im, 50, w.coeff, w.initP, precision=bsfit.PREC_F64)
-    of = orc.fit(ft, im, 50, w.coeff, w.initP, nthreads=8)
-    od = orc.fit_f64(ft, im, 50, w.coeff, w.initP, nthreads=8)
+    of = orc.fit(ft, im, 50, w.coeff, w.initP, nthreads=16)
+    od = orc.fit_f64(ft, im, 50, w.coeff, w.initP, nthreads=16)
     s64 = compare(ft, *gd, *od)
     llm = _ll_not_worse(gm[2], of[2])
     print("gpuF64 vs orcF64", s64, "LL(gpuMIX) >= LL(orcF32) on %.3f" % llm, flush=True)

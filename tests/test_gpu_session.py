@@ -228,3 +228,23 @@ def test_multistart_groups_equal_standalone_multistart():
         for a, b in zip(got, exp):
             np.testing.assert_array_equal(a, b)
     assert np.any(r[0][2] != r[1][2])          # the later starts changed some ROIs
+
+
+def test_sixteen_stream_sessions_create_and_destroy():
+    """Sessions with the largest number of kernel streams (16) release every
+    stream: many create / destroy cycles, then a session still fits."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    for _ in range(24):
+        with bsfit.Session(arena_bytes=8 << 20, kernel_streams=16) as s:
+            pass
+    c3d = gen.model_coeffs("astig3d")
+    w = gen.make_rois(6, 32, 13, 99, c3d)
+    ref = bsfit.fit(6, w.imstack, 50, c3d)
+    with bsfit.Session(arena_bytes=8 << 20, kernel_streams=16) as s:
+        m = s.register_model(6, c3d)
+        t = s.submit([bsfit.Group(m, w.imstack, 50)])
+        res = s.poll(t, wait=True)
+    for a, b in zip(res[0], ref):
+        _eq(a, b)
+    assert hip.hipDeviceSynchronize() == 0
