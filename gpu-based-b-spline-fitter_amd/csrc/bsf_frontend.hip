@@ -31,6 +31,8 @@
 //                      the window re-read from the raw frame and converted,
 //                      the filtered value recomputed in the same order.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <stdint.h>
 
 #include <cmath>
@@ -526,6 +528,36 @@ size_t front_detect_lds_bytes(int mr, int mc)
 }
 
 size_t front_work_bytes(int nr, int nc, int nframes) { return work_layout(nr, nc, nframes).total; }
+
+// Workspace of the allocation-free entry point (bsf_find_rois_device_noalloc):
+// one array per GPU for the life of the process.  Nothing in it needs to
+// start at zero (the column minima, every mask word and every block sum are
+// written before they are read), and calls that overlap on one GPU share it
+// and race.  In host code `scratch` only stands in for the symbol: its device
+// address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[kFrontScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kFrontScratchDevices];
+}  // namespace
+
+hipError_t front_scratch(void **ptr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kFrontScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 hipError_t launch_find_rois(const FrontLaunch &L, hipStream_t st)
 {

@@ -218,6 +218,28 @@ int bsf_find_rois_device(const bsf_roi_request *r, bsf_roi_result *res, int *cou
     return BSF_OK;
 }
 
+int bsf_find_rois_device_noalloc(const bsf_roi_request *r, bsf_roi_result *res, int *count, void *stream)
+{
+    bsf::clear_error();
+    int rc = validate_rois(r, res);
+    if (rc) return rc;
+    if (!count) return fail(BSF_ERR_ARG, "bsf: NULL count");
+    const size_t need = bsf::front_work_bytes(r->nrows, r->ncols, r->nframes);
+    if (need > bsf::kFrontScratchBytes)
+        return fail(BSF_ERR_NOMEM,
+                    "bsf: %d frames of %d x %d need %zu B of workspace; the allocation-free entry point has %zu B "
+                    "(use bsf_find_rois_device with a work buffer)",
+                    r->nframes, r->nrows, r->ncols, need, bsf::kFrontScratchBytes);
+    if ((rc = check_gfx950())) return rc;
+    void *work = nullptr;
+    hipError_t e = bsf::front_scratch(&work);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: workspace lookup failed: %s", hipGetErrorString(e));
+    bsf::FrontLaunch L = make_front(r, r->frames, r->filter, res->imstack, res->peaks, res->value, count, work);
+    e = bsf::launch_find_rois(L, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: launch failed: %s", hipGetErrorString(e));
+    return BSF_OK;
+}
+
 int bsf_find_rois_host(const bsf_roi_request *r, bsf_roi_result *res)
 {
     bsf::clear_error();

@@ -103,6 +103,12 @@ struct FrontLaunch {
 };
 hipError_t launch_find_rois(const FrontLaunch &L, hipStream_t st);
 size_t front_work_bytes(int nr, int nc, int nframes);
+// The allocation-free front-end's workspace: a file-scope device array of
+// kFrontScratchBytes per GPU; front_scratch returns the current device's
+// address of it (looked up once per device).
+constexpr size_t kFrontScratchBytes = (size_t)16 << 20;
+constexpr int kFrontScratchDevices = 64;
+hipError_t front_scratch(void **ptr);
 size_t front_detect_lds_bytes(int mr, int mc);
 
 // Checks a fit request as bsf_fit_device does and builds its launch (the

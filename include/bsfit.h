@@ -460,6 +460,16 @@ size_t bsf_find_rois_work_bytes(const bsf_roi_request *req);
 int bsf_find_rois_device(const bsf_roi_request *req, bsf_roi_result *res, int *count, void *work, size_t work_bytes,
                          void *stream);
 
+/* bsf_find_rois_device without the work buffer, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid allocation).  No counterpart in the reference.  The workspace is a
+ * device array inside the library, 16 MiB per GPU, held until the process
+ * ends: requests needing more than that (bsf_find_rois_work_bytes; about
+ * 8 B per 64 pixels of all frames, e.g. up to 120 frames of 1024 x 1024)
+ * return BSF_ERR_NOMEM.  The call allocates nothing.  Calls that overlap on
+ * one GPU share that array and race: serialise them (one stream, or events). */
+int bsf_find_rois_device_noalloc(const bsf_roi_request *req, bsf_roi_result *res, int *count, void *stream);
+
 /* Frames -> ROIs -> fits in one call: the GUI's per-frame loop followed by
  * its fitspline call on the ROI stack (simplefitter_spline.m:197-316, 479-546)
  * without a round trip to the host.  fit gives fittype, iterations, coeff
