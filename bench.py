@@ -1,17 +1,25 @@
-"""Throughput of the MI355X B-spline PSF fitter on the colour-4D workload.
+"""Throughput of the MI355X B-spline PSF fitter (one JSON line).
 
-Workload (one "step" = one fit call over the whole batch, inputs resident in
-HBM): 100,000 seeded 15x15 ROIs, fittype 7 (x, y, N, bg, z, colour), a
-27x27x61x3 cubic B-spline model (coefficients 30x30x64x6), 50 LM iterations,
-FP32 evaluation + FP64 solve.  Synthetic data: ROIs rendered from the spline
-model itself plus Poisson noise (gen.config(1)).
+Headline workload (one "step" = one fit launch over the whole batch, inputs
+resident in HBM): the vortex-5D orientation fit, fittype 8, 100,000 seeded
+ROIs with sizes mixed uniformly in {11, 13, 15, 17} fitted in ONE launch
+(bsf_fit_device_ex with a per-ROI size table), a 21x21x21x17x9 cubic
+B-spline (coefficients 24x24x24x20x12) blended with a 21^3 averaged model,
+50 LM iterations, FP32 evaluation + FP64 solve (MIX).  Synthetic data: ROIs
+rendered from the spline models themselves plus Poisson noise
+(gen.config(3)).
 
---model 3d / vortex5d (and --roi-size) run the 3D astigmatic (fittype 6,
-gen.config(0)) and vortex-5D (fittype 8, gen.config(3)) fits the same way.
+Beside it, in the same line: the fit kernel's roofline (algorithmic FP32
+flops = sum over ROIs of the evaluations each fit ran x F_alg(S), over the
+kernel time), per ROI size frac, the same workload host -> host (a serving
+session: pack, copy in, one launch, copy out), per-fittype lines in both
+precision modes (3D 10k x 13^2, colour-4D 100k x 15^2, 4Pi 200k x 4 x 13^2,
+vortex-5D 100k mixed) and the CPU oracle on the host cores (1 core and all
+allowed cores, up to 16) on a bounded sample of the same ROIs.
 
-Prints one JSON line: fits/s (whole job), the roofline of the fit kernel
-(algorithmic FP32 flops per ROI-evaluation x evaluations / kernel time) and
-the CPU oracle timed on a bounded sample on the host cores.
+--model 3d / colour4d / 4pi / vortex5d (and --roi-size) run one of them as
+the headline; --workload serve / coeff / frames run the streaming mix, the
+coefficient builder and the camera-frame front-end.
 """
 import argparse
 import json
@@ -26,7 +34,10 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 FP32_PEAK = 157.3e12     # MI355X FP32 vector, FLOP/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fit_kernel_colour4d_traffic.md")
+METRIC = "ROI fits/sec per fittype (FP32+FP64-solve vs FP64); p50/p99 frame latency"
+TRAFFIC_FILES = {"vortex5d": os.path.join(ROOT, "profiles", "fit_kernel_vortex5d_mixed_traffic.md"),
+                 "colour4d": os.path.join(ROOT, "profiles", "fit_kernel_colour4d_traffic.md")}
+TRAFFIC_FILE = TRAFFIC_FILES["colour4d"]
 
 
 def f_alg(ft, S, nch=1, NV=None):
@@ -47,29 +58,41 @@ def f_alg(ft, S, nch=1, NV=None):
 FIT_MODELS = {
     "3d": (0, 6, "3D astigmatic", "27x27x61 B-spline"),
     "colour4d": (1, 7, "colour-4D", "27x27x61x3 B-spline"),
+    "4pi": (2, 104, "4Pi-4D (4 channels)", "27x27x61x18 B-spline, phase offsets 0, pi/2, pi, 3pi/2"),
     "vortex5d": (3, 8, "vortex-5D", "21x21x21x17x9 B-spline + 21^3 averaged model"),
 }
+DEFAULT_N = {"3d": 10000, "colour4d": 100000, "4pi": 200000, "vortex5d": 100000}
 
 
-def cpu_baseline(w, budget_s=15.0):
+def cpu_threads():
+    """Host cores this process may use, capped at 16 (the CPU share of one
+    GPU on the boxes this runs on)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(groups, threads, budget_s=10.0):
     """The oracle (C restatement of the reference CPU fitter) on a bounded
-    sample of the same workload, ROI-parallel over the host cores."""
+    sample of the same ROIs: an equal share of every size group (one oracle
+    call per size, as the reference's one-size-per-call MEX), ROI-parallel
+    over `threads` host threads.  Returns (fits/s, sample text)."""
     import numpy as np
     import oracle
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    n = 64
-    t = 0.0
+    n = 8
     while True:
-        sl = np.asfortranarray(w.imstack[:, :, :n])
         t0 = time.perf_counter()
-        oracle.fit(w.fittype, sl, w.iterations, w.coeff, w.initP, w.coeff3, nthreads=threads)
+        done = 0
+        for w in groups:
+            k = min(n, w.N)
+            sl = np.asfortranarray(w.imstack[:, :, :k])
+            oracle.fit(w.fittype, sl, w.iterations, w.coeff, w.initP, w.coeff3, nthreads=threads)
+            done += k
         t = time.perf_counter() - t0
-        if t > budget_s / 4 or n >= w.N:
+        if t > budget_s / 4 or all(n >= w.N for w in groups):
             break
-        n = min(w.N, n * 4)
-    return {"value": n / t, "unit": "fits/s", "cores": threads, "kind": "port",
-            "sample": "%d of the %d ROIs (same seeded workload), %.1f s, oracle/bsf_oracle.c parity build"
-                      % (n, w.N, t)}
+        n *= 4
+    total = sum(w.N for w in groups)
+    return done / t, "%d of the %d ROIs (%s), %.1f s, oracle/bsf_oracle.c parity build" % (
+        done, total, ", ".join("%d x %d^2" % (min(n, w.N), w.S) for w in groups), t)
 
 
 def traffic_bytes(path=None):
@@ -420,17 +443,20 @@ def main():
     # launches of a process run up to 20 % longer); ten timed ones are 80 ms
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--n", type=int, default=None, help="fit: ROIs (default: the workload's own count)")
     ap.add_argument("--precision", choices=["mix", "f64"], default="mix")
-    ap.add_argument("--model", choices=sorted(FIT_MODELS), default="colour4d",
-                    help="fit: PSF model (colour4d, fittype 7, is the headline workload)")
+    ap.add_argument("--model", choices=sorted(FIT_MODELS), default="vortex5d",
+                    help="fit: PSF model of the headline line (vortex5d, sizes mixed in one launch, by default)")
     ap.add_argument("--roi-size", type=int, default=None,
-                    help="fit: ROI size (default 13 for 3d and vortex5d, 15 for colour4d)")
+                    help="fit: one ROI size (default: the workload's own: 13 for 3d and 4pi, 15 for colour4d, "
+                         "11-17 mixed for vortex5d)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="fit: only the headline (no per-size, host-to-host and per-fittype lines)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--workload", choices=["fit", "serve", "coeff", "frames"], default="fit",
-                    help="fit: the colour-4D batch fit (default); serve: the streamed mix through a session; "
+                    help="fit: batch fits (default: vortex-5D, mixed sizes); serve: the streamed mix through a session; "
                          "coeff: the B-spline coefficient build of the 5D model; frames: camera frames -> ROIs "
                          "(-> fits)")
     ap.add_argument("--nframes", type=int, default=32, help="frames: camera frames per step")
@@ -465,83 +491,240 @@ def main():
         serve(args)
         return
 
-    import numpy as np
+    fit_bench(args)
+
+
+# ------------------------------------------------------------------ fit bench
+class FitCase:
+    """One fit workload resident on the device, timed launch by launch."""
+
+    def __init__(self, model, N=None, S=None, precision="mix"):
+        import numpy as np
+        import torch
+        import bsfit
+        from bsfit import gen
+        self.model = model
+        cfg, ft, label, grid = FIT_MODELS[model]
+        self.ft, self.label, self.grid = ft, label, grid
+        N = N or DEFAULT_N[model]
+        dev = torch.device("cuda:0")
+        up = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32).ravel(order="F"))).to(dev)
+        self.prec = bsfit.PREC_F64 if precision == "f64" else bsfit.PREC_MIX
+        self.precision = precision
+        if ft == 104:
+            # a seeded pool of 25k linked ROIs repeated to N (fits are independent per ROI)
+            pool = min(N, 25000)
+            w = gen.config(2, N=pool, S=S)
+            reps = (N + pool - 1) // pool
+            im = np.concatenate([w.imstack] * reps, axis=2)[:, :, :N, :]
+            self.groups = [w]
+            self.S_list = np.full(N, w.S)
+            self.N = N
+            self.nch = w.nch
+            self.shared = w.shared
+            self.NV = bsfit.num_params_4pi(w.shared, w.nch)
+            self.d_data = up(im)
+            self.d_iz = up(np.tile(w.initZ, reps)[:N])
+            self.d_ip = up(np.tile(w.initPhi, reps)[:N])
+            self.coeff, self.phaseShift, self.iterations = w.coeff, w.phaseShift, w.iterations
+            self.d_coeff = up(w.coeff)
+            self.sample = "25,000 seeded linked ROIs repeated to %d" % N if reps > 1 else "%d seeded ROIs" % N
+        else:
+            ws = gen.config(cfg, N=N, S=S)
+            ws = ws if isinstance(ws, list) else [ws]
+            ws = sorted(ws, key=lambda w: -w.S)           # largest ROIs first in the launch
+            self.groups = ws
+            self.S_list = np.concatenate([np.full(w.N, w.S) for w in ws])
+            self.N = len(self.S_list)
+            self.NV = {6: 5, 7: 6, 8: 8}[ft]
+            self.d_data = up(np.concatenate([np.asarray(w.imstack).ravel(order="F") for w in ws]))
+            w0 = ws[0]
+            self.coeff, self.coeff3, self.initP, self.iterations = w0.coeff, w0.coeff3, w0.initP, w0.iterations
+            self.d_coeff = up(w0.coeff)
+            self.d_c3 = None if w0.coeff3 is None else up(w0.coeff3)
+            self.mixed = len(ws) > 1
+            self.d_rois = None
+            if self.mixed:
+                t = bsfit.roi_table(self.S_list)
+                self.d_rois = torch.from_numpy(t.view(np.uint8).copy()).to(dev)
+            self.sample = "%d seeded ROIs" % self.N
+        self.Smax = int(self.S_list.max())
+        self.d_P = torch.empty((self.NV + 1) * self.N, dtype=torch.float32, device=dev)
+        self.d_CR = torch.empty(self.NV * self.N, dtype=torch.float32, device=dev)
+        self.d_LL = torch.empty(self.N, dtype=torch.float32, device=dev)
+        self.d_ev = torch.zeros(self.N, dtype=torch.int32, device=dev)
+        self.stream = torch.cuda.current_stream()
+
+    def step(self):
+        import bsfit
+        if self.ft == 104:
+            bsfit.fit_4pi_device(self.d_data.data_ptr(), self.Smax, self.N, self.nch, self.shared, self.iterations,
+                                 self.d_coeff.data_ptr(), self.coeff.shape, self.phaseShift, self.d_P.data_ptr(),
+                                 self.d_CR.data_ptr(), self.d_LL.data_ptr(), initZ_ptr=self.d_iz.data_ptr(),
+                                 initPhi_ptr=self.d_ip.data_ptr(), precision=self.prec,
+                                 stream=self.stream.cuda_stream, evals_ptr=self.d_ev.data_ptr())
+            return
+        bsfit.fit_device(self.ft, self.d_data.data_ptr(), self.Smax, self.N, self.iterations,
+                         self.d_coeff.data_ptr(), self.coeff.shape, self.d_P.data_ptr(), self.d_CR.data_ptr(),
+                         self.d_LL.data_ptr(), initP=self.initP,
+                         coeff3_ptr=None if self.d_c3 is None else self.d_c3.data_ptr(), precision=self.prec,
+                         stream=self.stream.cuda_stream,
+                         rois_ptr=None if self.d_rois is None else self.d_rois.data_ptr(),
+                         evals_ptr=self.d_ev.data_ptr())
+
+    def run(self, steps, warmup):
+        """Time `steps` launches after `warmup` untimed ones: per-launch HIP
+        events on the launch stream and the wall clock around all of them."""
+        import numpy as np
+        import torch
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record(self.stream)
+            self.step()
+            b.record(self.stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        evals = self.d_ev.cpu().numpy().astype(np.float64)
+        P = self.d_P.cpu().numpy().reshape(self.NV + 1, self.N)
+        nch = self.nch if self.ft == 104 else 1
+        ftb = 7 if self.ft == 104 else self.ft
+        fl = {int(S): f_alg(ftb, int(S), nch=nch, NV=self.NV) for S in np.unique(self.S_list)}
+        flops = float(sum(evals[self.S_list == S].sum() * f for S, f in fl.items()))
+        return dict(ms_per_step=wall * 1e3 / steps, kernel_ms=kern_ms, fits_per_s=self.N / (wall / steps),
+                    kernel_fits_per_s=self.N / (kern_ms * 1e-3), achieved=flops / (kern_ms * 1e-3),
+                    frac=flops / (kern_ms * 1e-3) / FP32_PEAK, mean_evals=float(evals.mean()),
+                    mean_iterations=float(P[self.NV].mean()), f_alg=fl)
+
+    def per_size(self, steps, warmup):
+        """frac of uniform launches over each size group of this workload."""
+        out = {}
+        if self.ft == 104 or len(self.groups) < 2:
+            return out
+        import numpy as np
+        import torch
+        import bsfit
+        for w in self.groups:
+            sel = np.nonzero(self.S_list == w.S)[0]
+            a, b = int(sel[0]), int(sel[-1]) + 1
+            off = int(sum(int(S) * int(S) for S in self.S_list[:a]))
+            sub = FitCase.__new__(FitCase)
+            sub.__dict__.update(self.__dict__)
+            sub.N, sub.S_list, sub.Smax, sub.d_rois, sub.groups = b - a, self.S_list[a:b], int(w.S), None, [w]
+            sub.d_data = self.d_data[off:off + (b - a) * w.S * w.S]
+            sub.d_P = torch.empty((self.NV + 1) * sub.N, dtype=torch.float32, device=self.d_P.device)
+            sub.d_CR = torch.empty(self.NV * sub.N, dtype=torch.float32, device=self.d_P.device)
+            sub.d_LL = torch.empty(sub.N, dtype=torch.float32, device=self.d_P.device)
+            sub.d_ev = torch.zeros(sub.N, dtype=torch.int32, device=self.d_P.device)
+            r = sub.run(steps, warmup)
+            out[str(w.S)] = {"rois": sub.N, "kernel_fits_per_s": r["kernel_fits_per_s"], "frac": r["frac"],
+                             "f_alg": r["f_alg"][int(w.S)], "mean_evals": r["mean_evals"]}
+        return out
+
+    def host_to_host(self, steps):
+        """The same ROIs host -> host through a serving session: one frame of
+        one group per ROI size (pack into pinned memory, one copy in, one
+        launch of mixed sizes, one copy out, results scattered to the
+        caller's arrays)."""
+        import bsfit
+        if self.ft == 104:
+            return None
+        need = sum(w.imstack.nbytes for w in self.groups) + self.N * (2 * self.NV + 2) * 4 + self.N * 16
+        with bsfit.Session(arena_bytes=int(need * 1.25) + (16 << 20), kernel_streams=1,
+                           precision=self.prec) as s:
+            m = s.register_model(self.ft, self.coeff, self.coeff3)
+            groups = [bsfit.Group(m, w.imstack, w.iterations, initP=w.initP) for w in self.groups]
+            s.poll(s.submit(groups), wait=True)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s.poll(s.submit(groups), wait=True)
+            dt = (time.perf_counter() - t0) / steps
+        return {"fits_per_s": self.N / dt, "ms_per_step": dt * 1e3, "steps": steps,
+                "path": "bsf_submit + bsf_poll: host arrays -> pinned arena -> H2D -> one launch -> D2H -> host arrays"}
+
+
+def fit_bench(args):
     import torch
-    import bsfit
-    from bsfit import gen
-
-    cfg, ft, label, grid = FIT_MODELS[args.model]
+    model = args.model
     t0 = time.perf_counter()
-    w = gen.config(cfg, N=args.n, S=args.roi_size or (13 if ft == 8 else None))
-    w = w[0] if isinstance(w, list) else w
-    tgen = time.perf_counter() - t0
-    print("bench: generated %d ROIs (S=%d, fittype %d) in %.1f s" % (w.N, w.S, ft, tgen), file=sys.stderr,
-          flush=True)
-    dev = torch.device("cuda:0")
-    d_data = torch.from_numpy(np.ascontiguousarray(w.imstack.ravel(order="F"))).to(dev)
-    d_coeff = torch.from_numpy(np.ascontiguousarray(w.coeff.ravel(order="F"))).to(dev)
-    d_c3 = None if w.coeff3 is None else torch.from_numpy(np.ascontiguousarray(w.coeff3.ravel(order="F"))).to(dev)
-    NV = {6: 5, 7: 6, 8: 8}[ft]
-    d_P = torch.empty((NV + 1) * w.N, dtype=torch.float32, device=dev)
-    d_CR = torch.empty(NV * w.N, dtype=torch.float32, device=dev)
-    d_LL = torch.empty(w.N, dtype=torch.float32, device=dev)
-    prec = bsfit.PREC_F64 if args.precision == "f64" else bsfit.PREC_MIX
-    stream = torch.cuda.current_stream()
-
-    def step():
-        bsfit.fit_device(ft, d_data.data_ptr(), w.S, w.N, w.iterations, d_coeff.data_ptr(), w.coeff.shape,
-                         d_P.data_ptr(), d_CR.data_ptr(), d_LL.data_ptr(), initP=w.initP,
-                         coeff3_ptr=None if d_c3 is None else d_c3.data_ptr(),
-                         precision=prec, stream=stream.cuda_stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for a, b in ev:
-        a.record(stream)
-        step()
-        b.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    ms_per_step = wall * 1000.0 / args.steps
-    P = d_P.cpu().numpy().reshape(NV + 1, w.N)
-    kk = P[NV]
-    evals = float(np.sum(kk + 2))
-    flops = evals * f_alg(ft, w.S)
-    achieved = flops / (kern_ms / 1000.0)
-    print("bench: %.2f ms/step (kernel %.2f ms), mean iterations %.2f" % (ms_per_step, kern_ms, kk.mean()),
+    head = FitCase(model, N=args.n, S=args.roi_size, precision=args.precision)
+    print("bench: %s workload ready (%d ROIs, sizes %s) in %.1f s" % (
+        model, head.N, sorted(set(int(S) for S in head.S_list)), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    r = head.run(args.steps, args.warmup)
+    print("bench: %.2f ms/step (kernel %.2f ms), mean iterations %.2f, mean evaluations %.2f, frac %.3f"
+          % (r["ms_per_step"], r["kernel_ms"], r["mean_iterations"], r["mean_evals"], r["frac"]),
           file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(w)
+    extra = {}
+    if not args.no_extras:
+        extra["per_size"] = head.per_size(3, 2)
+        h2h = head.host_to_host(3)
+        if h2h:
+            extra["host_to_host"] = h2h
+        per = {}
+        for m in ("3d", "colour4d", "4pi", "vortex5d"):
+            per[m] = {}
+            for prec in ("mix", "f64"):
+                if m == model and prec == args.precision and args.n is None and args.roi_size is None:
+                    rr = r
+                else:
+                    c = FitCase(m, precision=prec)
+                    rr = c.run(3, 2)
+                    del c
+                    torch.cuda.empty_cache()
+                per[m][prec] = {"fits_per_s": rr["fits_per_s"], "kernel_fits_per_s": rr["kernel_fits_per_s"],
+                                "frac_fp32": rr["frac"], "mean_evals": rr["mean_evals"]}
+            print("bench: %s MIX %.3g fits/s (frac %.3f), F64 %.3g fits/s" % (
+                m, per[m]["mix"]["fits_per_s"], per[m]["mix"]["frac_fp32"], per[m]["f64"]["fits_per_s"]),
+                file=sys.stderr, flush=True)
+        extra["per_fittype"] = per
+    cpu = None
+    if not args.no_cpu and head.ft != 104:
+        th = cpu_threads()
+        v_all, samp_all = cpu_baseline(head.groups, th)
+        v_one, samp_one = cpu_baseline(head.groups, 1)
+        cpu = {"value": v_all, "unit": "fits/s", "cores": th, "kind": "port", "sample": samp_all,
+               "value_1core": v_one, "sample_1core": samp_one}
+    sizes = sorted(set(int(S) for S in head.S_list))
+    size_txt = ("sizes mixed uniformly in {%s}, one launch" % ",".join(map(str, sizes))) if len(sizes) > 1 else \
+        "%dx%d" % (sizes[0], sizes[0])
+    nk = "%gk" % (head.N / 1000) if head.N % 1000 == 0 else str(head.N)
+    traffic = traffic_bytes(TRAFFIC_FILES.get(model)) if (args.n is None and args.roi_size is None
+                                                          and args.precision == "mix") else None
     out = {
-        "metric": "ROI fits/sec (%s, fittype %d, FP32 eval + FP64 solve)" % (label, ft),
-        "value": w.N / (ms_per_step / 1000.0),
+        "metric": METRIC,
+        "value": r["fits_per_s"],
         "unit": "fits/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "mix" else "f64",
-        "data": "synthetic (seeded spline-rendered ROIs + Poisson noise)",
-        "config": {"workload": "%s fit: %s %dx%d ROIs, %s, %d LM iterations"
-                               % (label, "100k" if w.N == 100000 else str(w.N), w.S, w.S, grid, w.iterations),
-                   "fittype": ft, "rois": w.N, "roi_size": w.S, "iterations": w.iterations,
-                   "precision": args.precision},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK,
-                     "traffic": traffic_bytes() if args.model == "colour4d" and w.S == 15 else None,
-                     "note": "FP32 vector-ALU bound (no MFMA on this path); peak = MI355X FP32 157.3 TFLOP/s; "
-                             "achieved = sum(iterations+2) x F_alg(S=%d) = %d flop per ROI-evaluation / kernel time; "
-                             "traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)"
-                             % (w.S, f_alg(ft, w.S))},
+        "data": "synthetic (seeded spline-rendered ROIs + Poisson noise; %s)" % head.sample,
+        "config": {"workload": "%s fit (fittype %s): %s ROIs, %s, %s, %d LM iterations, %s"
+                               % (head.label, "4Pi" if head.ft == 104 else head.ft, nk, size_txt, head.grid,
+                                  head.iterations,
+                                  "FP32 evaluation + FP64 solve" if args.precision == "mix" else "all FP64"),
+                   "fittype": head.ft, "rois": head.N, "roi_sizes": sizes, "iterations": head.iterations,
+                   "precision": args.precision, "kernel_fits_per_s": r["kernel_fits_per_s"]},
+        "roofline": {"bound": "valu", "achieved": r["achieved"] / 1e12, "peak": FP32_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": r["frac"], "traffic": traffic,
+                     "note": "FP32 vector-ALU bound (no MFMA: not a dense contraction); peak = MI355X FP32 "
+                             "157.3 TFLOP/s (counts packed FP32, which this build does not issue); achieved = "
+                             "sum over ROIs of the evaluations each fit ran (kernel counter, mean %.2f) x "
+                             "F_alg(S) (%s flop per ROI-evaluation) / kernel time (HIP events on the launch "
+                             "stream); traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                             "copy-calibrated)" % (r["mean_evals"], ", ".join(
+                                 "S=%d: %d" % (S, f) for S, f in sorted(r["f_alg"].items())))},
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

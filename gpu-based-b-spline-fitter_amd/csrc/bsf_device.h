@@ -860,28 +860,39 @@ __device__ __forceinline__ double rcp_f64(double d)
 // the bound; without it every later solve would fail and the fit freeze.
 constexpr double kEscapeStep = -1e30;
 
+// Rows of the damped system without information, as a bit mask the same in
+// every lane.  H_kk is a sum of non-negative terms t2 dudt_k^2, so H_kk == 0
+// means dudt_k vanishes at every pixel with weight and the whole row is zero;
+// J_k == 0 as well (otherwise the pixels without weight still see the
+// parameter).  jh: J[NV] then the packed H, as ldlt_solve takes them.
 template <int NV>
-__device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV])
+__device__ __forceinline__ unsigned zero_rows(const double *jh)
+{
+    unsigned zr = 0;
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        zr |= (jh[NV + packed_index<NV>(k, k)] == 0.0 && jh[k] == 0.0) ? 1u << k : 0u;
+    return __builtin_amdgcn_readfirstlane(zr);
+}
+
+// zr: rows without information (zero_rows) -- their diagonal is taken as 1,
+// so that they decouple and their unknown comes out 0.
+template <int NV>
+__device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV], unsigned zr = 0)
 {
     double a[NV][NV];
 #pragma unroll
     for (int i = 0; i < NV; i++)
 #pragma unroll
         for (int j = 0; j <= i; j++) a[i][j] = jh[NV + packed_index<NV>(j, i)];
-    bool zr[NV];   // rows without information (see kEscapeStep)
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-        bool z = jh[k] == 0.0;
-#pragma unroll
-        for (int j = 0; j < NV; j++) z = z && a[k > j ? k : j][k > j ? j : k] == 0.0;
-        zr[k] = z;
-    }
+    for (int i = 0; i < NV; i++) a[i][i] = ((zr >> i) & 1u) ? 1.0 : a[i][i];
     double r[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         const double d = a[k][k];
         if (!(d >= 0.0)) return 1;
-        r[k] = zr[k] ? 0.0 : rcp_f64(d);
+        r[k] = rcp_f64(d);
         double t[NV];
 #pragma unroll
         for (int i = k + 1; i < NV; i++) {
@@ -907,8 +918,6 @@ __device__ __forceinline__ int ldlt_solve(const double *jh, double (&x)[NV])
 #pragma unroll
         for (int k = i + 1; k < NV; k++) x[i] -= a[k][i] * x[k];
     }
-#pragma unroll
-    for (int k = 0; k < NV; k++) x[k] = zr[k] ? kEscapeStep : x[k];
     return 0;
 }
 

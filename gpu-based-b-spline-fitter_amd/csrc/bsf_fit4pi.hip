@@ -53,7 +53,8 @@ template <typename E, int NVMAX>
 __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 {
     Dev A = A4.b;
-    const int n = xcd_block(blockIdx.x, gridDim.x) + A.roi_offset;
+    // mixed sizes: block b takes ROI b (every XCD the same size mix, as fit_kernel)
+    const int n = (A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + A.roi_offset;
     if (n >= A.N) return;
     const int lane = threadIdx.x;
     size_t dbase = (size_t)n * A.S * A.S, cstride = (size_t)A.N * A.S * A.S;
@@ -398,7 +399,8 @@ template <typename E, int NVMAX>
 __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
 {
     Dev A = A4.b;
-    const int n = xcd_block(blockIdx.x, gridDim.x) + A.roi_offset;
+    // mixed sizes: block b takes ROI b (every XCD the same size mix, as fit_kernel)
+    const int n = (A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + A.roi_offset;
     if (n >= A.N) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;      // wave = channel
     size_t dbase = (size_t)n * A.S * A.S, cstride = (size_t)A.N * A.S * A.S;

@@ -79,7 +79,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         G = c < 0 ? 0 : c < G ? c : G;
         if ((int)blockIdx.x >= G) return;
     }
-    const int n = xcd_block(blockIdx.x, G) + A.roi_offset;
+    // uniform launches: each XCD takes a contiguous range of ROIs (their
+    // column-major output stores meet in one L2); launches of mixed sizes,
+    // whose tables list the largest ROIs first: block b takes ROI b, so that
+    // every XCD (blocks are dealt round-robin over the 8) gets the same mix
+    // of sizes instead of a range of one size
+    const int n = (A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, G)) + A.roi_offset;
     if (n >= A.N) return;
     const int lane = threadIdx.x;
     // the launch's largest ROI size, which the LDS allocation is sized for
@@ -279,13 +284,16 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
             *dg = (double)((E)*dg * newLambda);
         }
         __syncthreads();
+        // rows without information (see kEscapeStep): solved with a 1 on their
+        // diagonal, which decouples them, and given the escape step
+        const unsigned zr = zero_rows<NV>(s_jh);
         double xs[NV];
-        int info = ldlt_solve<NV>(s_jh, xs);
+        int info = ldlt_solve<NV>(s_jh, xs, zr);
         if (info == 0) {
             E u = (E)xs[0];
 #pragma unroll
             for (int i = 1; i < NV; i++) u = (myl == i) ? (E)xs[i] : u;
-            upd = u;
+            upd = ((zr >> myl) & 1u) ? (E)kEscapeStep : u;
             if constexpr (FT != 6) {
 #pragma unroll
                 for (int i = 0; i < NV; i++)

@@ -25,9 +25,24 @@ def test_fit_models_generate_their_fittype(model):
     cfg, ft, label, grid = bench.FIT_MODELS[model]
     w = gen.config(cfg, N=4, S=13 if ft == 8 else None)
     w = w[0] if isinstance(w, list) else w
-    assert w.fittype == ft and w.N == 4
+    assert w.N == 4
+    if ft == 104:                       # 4Pi: four linked channels, the colour-4D machinery per channel
+        assert w.imstack.ndim == 4 and w.nch == 4
+        assert bench.f_alg(7, w.S, nch=4, NV=6) == 4 * bench.f_alg(7, w.S)
+        return
+    assert w.fittype == ft
     assert (w.coeff3 is not None) == (ft == 8)
     assert bench.f_alg(ft, w.S) > 0
+
+
+def test_default_headline_is_the_mixed_size_vortex_fit():
+    """With no flags the headline is fittype 8 over sizes 11-17 in one launch;
+    the metric is the workload's own."""
+    import argparse
+    assert bench.DEFAULT_N["vortex5d"] == 100000
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert 'default="vortex5d"' in src
+    assert bench.METRIC.startswith("ROI fits/sec per fittype")
 
 
 def test_f_alg_values():
