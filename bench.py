@@ -455,10 +455,11 @@ def main():
                     help="fit: only the headline (no per-size, host-to-host and per-fittype lines)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy (for scaling rocprofv3 byte counters)")
-    ap.add_argument("--workload", choices=["fit", "serve", "coeff", "frames"], default="fit",
+    ap.add_argument("--workload", choices=["fit", "serve", "coeff", "frames", "tiles"], default="fit",
                     help="fit: batch fits (default: vortex-5D, mixed sizes); serve: the streamed mix through a session; "
                          "coeff: the B-spline coefficient build of the 5D model; frames: camera frames -> ROIs "
-                         "(-> fits)")
+                         "(-> fits); tiles: one vs two ROIs per wavefront (bsf_tune's timings) per fittype 6 / 7, "
+                         "precision and ROI size")
     ap.add_argument("--nframes", type=int, default=32, help="frames: camera frames per step")
     ap.add_argument("--cpu-frames", type=int, default=4, help="frames: frames in the CPU baseline sample")
     ap.add_argument("--frames", type=int, default=3000, help="serve: frames")
@@ -481,6 +482,9 @@ def main():
     if args.workload == "frames":
         frames_bench(args)
         return
+    if args.workload == "tiles":
+        tiles_bench(args)
+        return
     if args.workload == "serve":
         # the session's kernel streams plus its two copy streams each need a
         # hardware queue of their own to run concurrently; HIP reads this
@@ -492,6 +496,41 @@ def main():
         return
 
     fit_bench(args)
+
+
+# ------------------------------------------------------------------ tile sweep
+def tiles_bench(args):
+    """One vs two ROIs per wavefront: bsf_tune on 8,192 seeded ROIs repeated
+    16 times (131,072 per launch), 50 iterations, `--steps` timed launches per
+    tile; one JSON line per (fittype, precision, ROI size).  The tuned choice
+    is reset afterwards, so the process keeps the measured defaults."""
+    import numpy as np
+    import torch
+    import bsfit
+    from bsfit import gen
+    nu, rep = 8192, 16
+    N = nu * rep
+    for ft in (6, 7):
+        c = gen.model_coeffs("astig3d" if ft == 6 else "color4d")
+        NV = bsfit.num_params(ft)
+        dc = torch.from_numpy(np.asfortranarray(c, np.float32).ravel(order="F").copy()).cuda()
+        sizes = (args.roi_size,) if args.roi_size else (5, 7, 9, 11, 13, 15, 17, 21)
+        for S in sizes:
+            w = gen.make_rois(ft, nu, S, 5000 + S, c)
+            im = np.asfortranarray(w.imstack, np.float32).ravel(order="F")
+            data = torch.from_numpy(np.tile(im, rep)).cuda()
+            P = torch.empty((NV + 1) * N, device="cuda")
+            CR = torch.empty(NV * N, device="cuda")
+            LL = torch.empty(N, device="cuda")
+            for prec in (bsfit.PREC_MIX, bsfit.PREC_F64):
+                default = bsfit.fit_tile(ft, prec, S)
+                t, ms = bsfit.tune(ft, data.data_ptr(), S, N, 50, dc.data_ptr(), c.shape, P.data_ptr(),
+                                   CR.data_ptr(), LL.data_ptr(), initP=w.initP, reps=args.steps)
+                bsfit.set_fit_tile(ft, prec, S, 0)
+                print(json.dumps({"workload": "tiles", "fittype": ft, "precision": "f64" if prec else "mix",
+                                  "roi_size": S, "rois": N, "ms_tile1": round(ms[0], 4), "ms_tile2": round(ms[1], 4),
+                                  "speedup_tile2": round(ms[0] / ms[1], 3) if ms[1] else None, "tuned": t,
+                                  "default": default}), flush=True)
 
 
 # ------------------------------------------------------------------ fit bench

@@ -12,7 +12,7 @@ library or a gfx950 device is missing, calls raise ``BsfError``.
 """
 from .api import (  # noqa: F401
     BsfError, GPUmleFit_LM_multiD, GPUmleFit_LM_4Pi_bspline, coeff_build, fit, fit_4pi, fit_4pi_device,
-    fit_device, fit_multistart, roi_table,
+    fit_device, fit_multistart, fit_tile, roi_table, set_fit_tile, tune,
     fit_status, ROI_MAX_ITER, ROI_SOLVE_FAILED, ROI_FISHER_SINGULAR, ROI_NONFINITE,
     ROI_FIRST_SOLVE_FAILED,
     lib_path, load, num_params, num_params_4pi, PREC_MIX, PREC_F64,

@@ -197,7 +197,7 @@ class RoiDesc(C.Structure):
 
 class Extras(C.Structure):
     """bsf_fit_extras: optional device-side inputs and outputs."""
-    _fields_ = [("rois", C.c_void_p), ("status", C.c_void_p), ("evals", C.c_void_p)]
+    _fields_ = [("rois", C.c_void_p), ("status", C.c_void_p), ("evals", C.c_void_p), ("tile", C.c_int)]
 
 
 def roi_table(sizes, offsets=None, cstride=0):
@@ -215,20 +215,21 @@ def roi_table(sizes, offsets=None, cstride=0):
 
 def fit_device(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_ptr, CRLB_ptr, LL_ptr,
                initP=None, coeff3_ptr=None, precision=PREC_MIX, stream=None, status_ptr=None, rois_ptr=None,
-               evals_ptr=None):
+               evals_ptr=None, tile=0):
     """Queue a fit on device-resident buffers (raw device addresses, e.g.
     torch tensor .data_ptr()) on a HIP stream handle; returns immediately.
     status_ptr: a device array of N bytes for the per-ROI status flags
     (bsf_fit_device_status), or None.  rois_ptr: a device array of N bsf_roi
     records (roi_table) for ROIs of mixed sizes, S then being the largest;
-    evals_ptr: a device int32 array of N for the evaluations each fit ran
-    (both through bsf_fit_device_ex)."""
+    evals_ptr: a device int32 array of N for the evaluations each fit ran;
+    tile: ROIs per wavefront (0: the recorded choice, 1 or 2) (the last three
+    through bsf_fit_device_ex)."""
     lib = load()
     ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
     req = _request(fittype, S, N, iterations, coeff_ptr, coeff_shape, coeff3_ptr, ip, precision, data_ptr)
     res = Result(P_ptr, CRLB_ptr, LL_ptr)
-    if rois_ptr is not None or evals_ptr is not None:
-        x = Extras(rois_ptr, status_ptr, evals_ptr)
+    if rois_ptr is not None or evals_ptr is not None or tile:
+        x = Extras(rois_ptr, status_ptr, evals_ptr, int(tile))
         lib.bsf_fit_device_ex.argtypes = [C.POINTER(Request), C.POINTER(Result), C.POINTER(Extras), C.c_void_p]
         lib.bsf_fit_device_ex.restype = C.c_int
         _check(lib.bsf_fit_device_ex(C.byref(req), C.byref(res), C.byref(x), stream))
@@ -239,6 +240,40 @@ def fit_device(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_pt
     lib.bsf_fit_device_status.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_void_p, C.c_void_p]
     lib.bsf_fit_device_status.restype = C.c_int
     _check(lib.bsf_fit_device_status(C.byref(req), C.byref(res), status_ptr, stream))
+
+
+def tune(fittype, data_ptr, S, N, iterations, coeff_ptr, coeff_shape, P_ptr, CRLB_ptr, LL_ptr, initP=None,
+         coeff3_ptr=None, precision=PREC_MIX, stream=None, reps=5):
+    """bsf_tune: time each tile (ROIs per wavefront) on this fit (device
+    buffers, as fit_device), record the fastest for (fittype, precision, S)
+    and return (tile, [ms of tile 1, ms of tile 2]) (0 ms: not possible)."""
+    lib = load()
+    ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
+    req = _request(fittype, S, N, iterations, coeff_ptr, coeff_shape, coeff3_ptr, ip, precision, data_ptr)
+    res = Result(P_ptr, CRLB_ptr, LL_ptr)
+    t = C.c_int(0)
+    ms = (C.c_float * 2)()
+    lib.bsf_tune.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_int, C.POINTER(C.c_int),
+                             C.POINTER(C.c_float), C.c_void_p]
+    lib.bsf_tune.restype = C.c_int
+    _check(lib.bsf_tune(C.byref(req), C.byref(res), int(reps), C.byref(t), ms, stream))
+    return t.value, [ms[0], ms[1]]
+
+
+def fit_tile(fittype, precision=PREC_MIX, S=13):
+    """The recorded tile (ROIs per wavefront) for (fittype, precision, S)."""
+    lib = load()
+    lib.bsf_fit_tile.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.bsf_fit_tile.restype = C.c_int
+    return lib.bsf_fit_tile(int(fittype), int(precision), int(S))
+
+
+def set_fit_tile(fittype, precision, S, tile):
+    """Record a tile by hand (0: back to the measured default)."""
+    lib = load()
+    lib.bsf_set_fit_tile.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.bsf_set_fit_tile.restype = C.c_int
+    _check(lib.bsf_set_fit_tile(int(fittype), int(precision), int(S), int(tile)))
 
 
 def GPUmleFit_LM_multiD(imstack, fittype, iterations=50, coeff=None, varmap=None, silent=1,

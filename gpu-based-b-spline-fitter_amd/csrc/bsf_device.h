@@ -20,6 +20,60 @@ struct Ft<7> { static constexpr int D = 4, K = 2, NV = 6, NPL = 3; };
 template <>
 struct Ft<8> { static constexpr int D = 5, K = 3, NV = 8, NPL = 8; };
 
+// ----------------------------------------------------------- lane reads
+// The NV x NV systems are spread over the wave: lane i holds row i, pivots
+// and columns travel by v_readlane (uniform lane index), so the solve costs a
+// few registers per lane instead of NV^2 doubles.
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <typename E>
+__device__ __forceinline__ E readlane_e(E v, int l)
+{
+    if constexpr (sizeof(E) == 8) return readlane_d(v, l);
+    else return readlane_f(v, l);
+}
+
+// Waves that fit R ROIs, one per 64 / R lanes: a value of lane hb + i of
+// the caller's ROI (hb its first lane, i the same for all of its lanes).
+// R = 1 is v_readlane (a scalar); otherwise ds_bpermute (per lane).
+template <int R>
+__device__ __forceinline__ int roi_read_i(int v, int i, int hb)
+{
+    if constexpr (R == 1) return __builtin_amdgcn_readlane(v, i);
+    else return __builtin_amdgcn_ds_bpermute((hb + i) << 2, v);
+}
+
+template <int R, typename E>
+__device__ __forceinline__ E roi_read(E v, int i, int hb)
+{
+    if constexpr (R == 1) return readlane_e<E>(v, i);
+    else if constexpr (sizeof(E) == 8) {
+        const long long b = __double_as_longlong(v);
+        const int lo = roi_read_i<R>((int)(b & 0xffffffffLL), i, hb), hi = roi_read_i<R>((int)(b >> 32), i, hb);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    } else return __int_as_float(roi_read_i<R>(__float_as_int(v), i, hb));
+}
+
+// Ballot over the caller's ROI's lanes: bit i = lane hb + i.
+template <int R>
+__device__ __forceinline__ unsigned long long roi_ballot(bool c, int hb)
+{
+    const unsigned long long b = __ballot(c);
+    if constexpr (R == 1) return b;
+    else return (b >> hb) & ((1ull << (64 / R)) - 1);
+}
+
 // ---------------------------------------------------------------- basis
 // Centred cubic / quadratic B-splines (the reference's kernel_evalBSpline,
 // CPUsplineLib.cpp:1289-1326, writes them as four / three polynomial pieces
@@ -185,9 +239,9 @@ struct GeoLds {
 // operation order), lane i evaluates the clamped taps of ROI row / column i
 // (CPUsplineLib.cpp:1203-1225, per pixel exactly as the reference), and the
 // window spans the taps of all rows / columns.  Every lane must call it.
-template <typename E, int FT>
+template <typename E, int FT, int R = 1>
 __device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> &g, int lane, const GeoLds<E> &L,
-                                         bool fixed_cs = false)
+                                         bool fixed_cs = false, int hb = 0)
 {
     constexpr int D = Ft<FT>::D;
     const int S = A.S;
@@ -213,10 +267,11 @@ __device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> 
     const E rc = rowc + (E)li + off, cc = colc + (E)li + off;
     const int n1 = A.nd[1], n0 = A.nd[0];
     const int kr = base_tap<E>(rc, n1), kq = base_tap<E>(cc, n0);
-    const unsigned long long sr = __ballot(lane < S && rc == (E)n1);
-    const unsigned long long sq = __ballot(lane < S && cc == (E)n0);
-    const int kr0 = __builtin_amdgcn_readlane(kr, 0), krl = __builtin_amdgcn_readlane(kr, S - 1);
-    const int kq0 = __builtin_amdgcn_readlane(kq, 0), kql = __builtin_amdgcn_readlane(kq, S - 1);
+    // (lane is the lane within the ROI's lanes, hb their first lane in the wave)
+    const unsigned long long sr = roi_ballot<R>(lane < S && rc == (E)n1, hb);
+    const unsigned long long sq = roi_ballot<R>(lane < S && cc == (E)n0, hb);
+    const int kr0 = roi_read_i<R>(kr, 0, hb), krl = roi_read_i<R>(kr, S - 1, hb);
+    const int kq0 = roi_read_i<R>(kq, 0, hb), kql = roi_read_i<R>(kq, S - 1, hb);
     int rmin = kr0, rmax = krl, cmin = kq0, cmax = kql;
     if (sr) {
         rmax = n1 + 1;
@@ -276,7 +331,10 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int s
 }
 
 // Contract the non-xy axes over the window; planes[p][b*CS + a].
-template <typename E, int FT>
+// R ROIs per wave: lane is the lane within the ROI, the window points are
+// dealt over the ROI's 64 / R lanes, and the ROI's tap offset (uniform for
+// the whole wave when R = 1, then an SGPR) joins the per-lane offset.
+template <typename E, int FT, int R = 1>
 __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *planes, int plane_stride, int lane)
 {
     constexpr int K = Ft<FT>::K;
@@ -286,16 +344,18 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
     int tapb = (g.knx[0] - 2) * s2;
     if constexpr (K >= 2) tapb += (g.knx[1] - 2) * s3;
     if constexpr (K >= 3) tapb += (g.knx[2] - 2) * s4;
-    tapb = __builtin_amdgcn_readfirstlane(tapb);
+    if constexpr (R == 1) tapb = __builtin_amdgcn_readfirstlane(tapb);
+    const int vtap = R == 1 ? 0 : tapb;   // per-lane part of the tap offset
+    if constexpr (R != 1) tapb = 0;
     // window point q = b*PR + a without an integer divide: for q < 2^12 the
     // FP32 quotient (q + 0.5) / PR stays within 1e-4 of its exact value,
     // which is at least 0.5/PR away from an integer, so the floor is exact
     const float invPR = 1.0f / (float)PR;
 #pragma nounroll
-    for (int q = lane; q < npts; q += 64) {
+    for (int q = lane; q < npts; q += 64 / R) {
         const int b = (int)(((float)q + 0.5f) * invPR), a = q - b * PR;
         const int pq = b * g.CS + a;
-        const int vo = ((g.row0 + a) + (g.col0 + b) * (int)A.s1) * 4;
+        const int vo = ((g.row0 + a) + (g.col0 + b) * (int)A.s1) * 4 + vtap;
         if constexpr (K == 1) {
             const Axis<E> &z = g.ax[2];
             E cv[4];
@@ -617,14 +677,14 @@ struct alignas(16) Vec4 { E v[4]; };
 
 // First pass.  Every lane must call it; it ends with the sums visible to
 // the wave.
-template <typename E, int FT>
+template <typename E, int FT, int R = 1>
 __device__ __forceinline__ void stage_b_cols(const Geo<E, FT> &g, const E *planes, int plane_stride, const int *ctap,
                                              int S, float invWd, E *h, int lane)
 {
     constexpr int NHP = ColSums<FT>::NHP;
     const int Wd = S + 4, items = S * Wd;
 #pragma nounroll
-    for (int q = lane; q < items; q += 64) {
+    for (int q = lane; q < items; q += 64 / R) {
         E hv[NHP];
         col_sums<E, FT>(g, planes, plane_stride, ctap, Wd, invWd, q, hv);
         Vec4<E> *dst = (Vec4<E> *)(h + (size_t)q * NHP);
@@ -719,29 +779,6 @@ __device__ __forceinline__ E deviance_term(E model, E data, float rdat, E &d0)
 }
 
 // ------------------------------------------------------------- LM solve
-// The NV x NV systems are spread over the wave: lane i holds row i, pivots
-// and columns travel by v_readlane (uniform lane index), so the solve costs a
-// few registers per lane instead of NV^2 doubles.
-__device__ __forceinline__ double readlane_d(double v, int l)
-{
-    long long b = __double_as_longlong(v);
-    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-__device__ __forceinline__ float readlane_f(float v, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-template <typename E>
-__device__ __forceinline__ E readlane_e(E v, int l)
-{
-    if constexpr (sizeof(E) == 8) return readlane_d(v, l);
-    else return readlane_f(v, l);
-}
-
 // Damped normal equations H x = J in FP64 by Cholesky (right-looking), the
 // two triangular solves; lane i holds H row i (a[]) and J_i (b).  Fails, as
 // the reference's kernel_cholesky (CPUsplineLib.cpp:1363-1388), when a pivot
@@ -794,8 +831,8 @@ __device__ __forceinline__ int chol_solve_rows(double (&a)[NV], double b, int la
 // are zero, so it decouples; its CRLB is +inf and the others come from the
 // rest of the matrix.  (The reference divides by its rounding residue there
 // and reports 1e7-1e31 for that parameter, the others as here.)
-template <int NV>
-__device__ __forceinline__ double inverse_diag_rows(double (&m)[NV], int lane)
+template <int NV, int R = 1>
+__device__ __forceinline__ double inverse_diag_rows(double (&m)[NV], int lane, int hb = 0)
 {
     double v[NV];
     bool none = false;   // this lane's parameter has no information
@@ -803,14 +840,14 @@ __device__ __forceinline__ double inverse_diag_rows(double (&m)[NV], int lane)
     for (int j = 0; j < NV; j++) v[j] = (j == lane) ? 1.0 : 0.0;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
-        const double mkk = readlane_d(m[k], k);
+        const double mkk = roi_read<R, double>(m[k], k, hb);
         const bool zp = mkk == 0.0;
         none = none || (zp && lane == k);
         const double r = zp ? 0.0 : 1.0 / mkk;
         const double fac = zp ? 0.0 : m[k] * r;
 #pragma unroll
         for (int j = 0; j < NV; j++) {
-            const double mk = readlane_d(m[j], k), vk = readlane_d(v[j], k);
+            const double mk = roi_read<R, double>(m[j], k, hb), vk = roi_read<R, double>(v[j], k, hb);
             m[j] = (lane == k) ? (zp ? m[j] : mk * r) : m[j] - fac * mk;
             v[j] = (lane == k) ? (zp ? v[j] : vk * r) : v[j] - fac * vk;
         }
@@ -865,14 +902,15 @@ constexpr double kEscapeStep = -1e30;
 // means dudt_k vanishes at every pixel with weight and the whole row is zero;
 // J_k == 0 as well (otherwise the pixels without weight still see the
 // parameter).  jh: J[NV] then the packed H, as ldlt_solve takes them.
-template <int NV>
+template <int NV, int R = 1>
 __device__ __forceinline__ unsigned zero_rows(const double *jh)
 {
     unsigned zr = 0;
 #pragma unroll
     for (int k = 0; k < NV; k++)
         zr |= (jh[NV + packed_index<NV>(k, k)] == 0.0 && jh[k] == 0.0) ? 1u << k : 0u;
-    return __builtin_amdgcn_readfirstlane(zr);
+    if constexpr (R == 1) return __builtin_amdgcn_readfirstlane(zr);
+    else return zr;   // the same in the ROI's lanes, different between ROIs
 }
 
 // zr: rows without information (zero_rows) -- their diagonal is taken as 1,
@@ -972,6 +1010,27 @@ constexpr int kQuadXor2 = 0x4E;   // quad_perm [2,3,0,1]
 constexpr int kRedChunk = 16;
 __host__ __device__ constexpr int red_elems(int V) { return kRedChunk * 65 + V; }
 
+
+// LDS bytes of one ROI of a fit workgroup (fittypes 6-8) whose largest ROI
+// has size S, a multiple of 16: the planes, the column sums of the two-pass
+// stage B after them and the reduction scratch over both, with the ROI data,
+// tap tables and J / H before them (fit_lds_layout).  A wave that fits R
+// ROIs holds R such regions back to back.
+__host__ __device__ inline size_t fit_lds_bytes(int fittype, int f64, int S)
+{
+    const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
+    const int NPL = fittype == 6 ? 2 : fittype == 7 ? 3 : 8;
+    const size_t W = S + 4;
+    const size_t pl = (size_t)NPL * W * W;
+    const size_t h = col_sum_elems(fittype, S) + 3;   // after the planes, 16-byte aligned
+    const size_t red = (size_t)red_elems(1 + NV + NV * (NV + 1) / 2);
+    const size_t e0 = (fittype != 8 && !f64) || stage_b_split(fittype, f64, S) ? pl + h : pl;
+    const size_t e = e0 > red ? e0 : red;
+    return lds_align16(fit_lds_layout(S, f64 ? 8 : 4, e, (NV + NV * (NV + 1) / 2) * (f64 ? 1 : 2),
+                                      fit_data_in_lds(fittype))
+                           .total);
+}
+
 // Sum V per-lane values across the wave, 16 values at a time: the chunk is
 // written transposed to LDS, lane 4v+q adds entries 16q..16q+15 of value v,
 // and the four quarters combine by quad permutes (the same total in all four
@@ -1000,6 +1059,45 @@ __device__ __forceinline__ void wave_sum_lds(const E (&v)[V], E *scratch, int la
         t += dpp_move<kQuadXor1>(t);
         t += dpp_move<kQuadXor2>(t);
         if (vi < C && q4 == 0) tot[k0 + vi] = t;
+        __syncthreads();
+    }
+}
+
+// wave_sum_lds for a wave that fits two ROIs, one per 32 lanes: lane l of a
+// half holds the partial sums of the one-ROI kernel's lanes l (v0) and
+// l + 32 (v1), and each half reduces its 64 virtual lanes in wave_sum_lds's
+// order, so that the totals are the same bits.  scratch is the half's own.
+template <typename E, int V>
+__device__ __forceinline__ void wave_sum_lds2(const E (&v0)[V], const E (&v1)[V], E *scratch, int hl)
+{
+    E *tot = scratch + kRedChunk * 65;
+    const int vi = hl >> 2, q4 = hl & 3;
+#pragma unroll
+    for (int k0 = 0; k0 < V; k0 += kRedChunk) {
+#pragma unroll
+        for (int k = k0; k < V && k < k0 + kRedChunk; k++) {
+            scratch[(k - k0) * 65 + hl] = v0[k];
+            scratch[(k - k0) * 65 + hl + 32] = v1[k];
+        }
+        __syncthreads();
+        const int C = (V - k0) < kRedChunk ? (V - k0) : kRedChunk;
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+            const int vv = vi + 8 * half;
+            E s0 = E(0), s1 = E(0);
+            if (vv < C) {
+                const E *row = scratch + vv * 65 + q4 * 16;
+#pragma unroll
+                for (int j = 0; j < 16; j += 2) {
+                    s0 += row[j];
+                    s1 += row[j + 1];
+                }
+            }
+            E t = s0 + s1;
+            t += dpp_move<kQuadXor1>(t);
+            t += dpp_move<kQuadXor2>(t);
+            if (vv < C && q4 == 0) tot[k0 + vv] = t;
+        }
         __syncthreads();
     }
 }

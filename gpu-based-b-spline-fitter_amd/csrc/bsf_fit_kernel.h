@@ -63,18 +63,27 @@ __device__ __forceinline__ void roi_init4(const float *d, int S, E *v)
 // ----------------------------------------------------------------- kernel
 // SC > 0: every ROI of the launch has size SC, so the index arithmetic and
 // LDS offsets that depend on the size fold to constants; SC = 0: any size.
-template <int FT, typename E, int SC>
+// R ROIs per wave: R = 1, one wave per ROI; R = 2, each half of the wave
+// (32 lanes) fits its own ROI with its own LDS region and LM state, and the
+// per-ROI work that does not spread over lanes (geometry, the damped solve,
+// the LM control, the reductions' tails) runs once for both.  A half's lane
+// holds the pixel sums of two lanes of the one-ROI kernel and reduces them in
+// its order, so both forms give the same bits.  (The geometry takes 8 lanes
+// per spline axis: fittype 8's five axes need 40, so it runs with R = 1.)
+template <int FT, typename E, int SC, int R = 1>
 __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 {
+    static_assert(R == 1 || (R == 2 && FT != 8), "two ROIs per wave: fittypes 6 and 7");
     constexpr int NV = Ft<FT>::NV;
     constexpr int NH = NV * (NV + 1) / 2;
     constexpr int NRED = 1 + NV + NH;
-    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int L = 64 / R;   // lanes per ROI
+    extern __shared__ __align__(16) unsigned char smem_all[];
 
     // with a device-side ROI count, the blocks below it are spread over the
     // XCDs as the whole grid otherwise is (the rest exit at once)
     int G = (int)gridDim.x;
-    if (A.ncount) {
+    if (R == 1 && A.ncount) {   // (R = 2 launches take no device-side count)
         const int c = *A.ncount - A.roi_offset;
         G = c < 0 ? 0 : c < G ? c : G;
         if ((int)blockIdx.x >= G) return;
@@ -84,11 +93,20 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     // whose tables list the largest ROIs first: block b takes ROI b, so that
     // every XCD (blocks are dealt round-robin over the 8) gets the same mix
     // of sizes instead of a range of one size
-    const int n = (A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, G)) + A.roi_offset;
-    if (n >= A.N) return;
     const int lane = threadIdx.x;
+    const int hl = R == 1 ? lane : (lane & (L - 1));   // lane within the ROI
+    const int hb = lane - hl;                          // the ROI's first lane
+    int n = ((A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, G)) * R + hb / L) + A.roi_offset;
+    bool live = true;   // R = 2: the last block's second half may have no ROI of its own
+    if (n >= A.N) {
+        if constexpr (R == 1) return;
+        live = false;
+        n = A.N - 1;    // fits a copy of the last ROI (barriers and LDS stay per half) and writes nothing
+    }
     // the launch's largest ROI size, which the LDS allocation is sized for
     const int Smax = SC ? SC : A.S;
+    unsigned char *smem = smem_all;
+    if constexpr (R != 1) smem += (hb / L) * fit_lds_bytes(FT, sizeof(E) == 8, Smax);
     const float *dptr = A.data + (size_t)n * Smax * Smax;
     if (A.rois) {
         const RoiDesc d = A.rois[n];
@@ -122,7 +140,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     const float invS = 1.0f / (float)S, invW = 1.0f / (float)W;
 
     // stage the ROI (fittypes 6 / 7: only for the initial values) and 1/data
-    for (int p = lane; p < S2; p += 64) {
+    for (int p = hl; p < S2; p += L) {
         const float d = dptr[p];
         s_data[p] = d;
         if constexpr (sizeof(E) == 4) s_rdat[p] = d > 0.f ? 1.0f / d : 0.f;
@@ -133,7 +151,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     // summed in the reference's sequential order so they match it bit for bit.
     // Lane l < NV owns parameter l: its value and LM state (sign, clamp,
     // update and the copies restored on a rejected step) are per-lane scalars.
-    const int myl = lane < NV ? lane : NV - 1;
+    const int myl = hl < NV ? hl : NV - 1;
     E th;
     {
         E init[8];
@@ -188,64 +206,85 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 
     // one ROI evaluation at the current parameters: fills s_jh and returns
     // the deviance
+    // the per-pixel terms of one evaluation (deviance, J, H) added to acc
+    auto pixel_terms = [&](int p, bool first, E (&acc)[NRED]) __attribute__((always_inline)) {
+        int ii, jj;
+        pixel_rc(p, invS, S, ii, jj);
+        E dudt[8], model;
+        if (split) {
+            const int r = gl.rtap[ii];
+            const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
+            pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
+                                 cth[3], g2, dudt, model);
+        }
+        else
+            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
+        if constexpr (FT == 7) {
+            if (first) dudt[5] = dudt[4];
+        }
+        E data;
+        acc[0] += deviance_term<E>(model, DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0), s_rdat[p], data);
+        const E rm = recip<E>(model);
+        const E dr = data * rm;
+        E t1 = E(1) - dr;
+        E t2 = dr * rm;
+#pragma unroll
+        for (int l = 0; l < NV; l++) acc[1 + l] += t1 * dudt[l];
+        int h = 1 + NV;
+#pragma unroll
+        for (int l = 0; l < NV; l++)
+#pragma unroll
+            for (int m = l; m < NV; m++) acc[h++] += t2 * dudt[l] * dudt[m];
+    };
+
+    // one ROI evaluation at the current parameters: fills s_jh and returns
+    // the deviance
     auto evaluate = [&](bool first) __attribute__((always_inline)) -> E {
 #pragma unroll
-        for (int i = 0; i < NV; i++) cth[i] = readlane_e<E>(th, i);
+        for (int i = 0; i < NV; i++) cth[i] = roi_read<R, E>(th, i, hb);
         if constexpr (FT == 8) g2 = cth[7];
         __syncthreads();   // the scratch of the last reduction is read out
-        make_geo<E, FT>(A, cth, g, lane, gl, SC != 0);
-        stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
+        make_geo<E, FT, R>(A, cth, g, hl, gl, SC != 0, hb);
+        stage_a<E, FT, R>(A, g, s_pl, plane_stride, hl);
         __syncthreads();
-        if (split) stage_b_cols<E, FT>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, lane);
+        if (split) stage_b_cols<E, FT, R>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, hl);
         E acc[NRED];
 #pragma unroll
         for (int k = 0; k < NRED; k++) acc[k] = E(0);
-        // with a compile-time size every lane has a first pixel (S * S > 64), so
-        // the loop needs no entry test (with one, the compiler zeroes the
-        // accumulators a second time under the entry mask); not unrolled, which
-        // with a fixed size would cost registers (a wave per SIMD)
-        int p = lane;
-        if (SC || p < S2) {
+        if constexpr (R == 1) {
+            // with a compile-time size every lane has a first pixel (S * S > 64), so
+            // the loop needs no entry test (with one, the compiler zeroes the
+            // accumulators a second time under the entry mask); not unrolled, which
+            // with a fixed size would cost registers (a wave per SIMD)
+            int p = lane;
+            if (SC || p < S2) {
 #pragma nounroll
-            do {
-                int ii, jj;
-                pixel_rc(p, invS, S, ii, jj);
-                E dudt[8], model;
-                if (split) {
-                    const int r = gl.rtap[ii];
-                    const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
-                    pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
-                                         cth[3], g2, dudt, model);
-                }
-                else
-                    pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
-                if constexpr (FT == 7) {
-                    if (first) dudt[5] = dudt[4];
-                }
-                E data;
-                acc[0] += deviance_term<E>(model, DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0), s_rdat[p], data);
-                const E rm = recip<E>(model);
-                const E dr = data * rm;
-                E t1 = E(1) - dr;
-                E t2 = dr * rm;
+                do {
+                    pixel_terms(p, first, acc);
+                    p += 64;
+                } while (p < S2);
+            }
+            __syncthreads();   // planes are dead; the scratch reuses them
+            wave_sum_lds<E, NRED>(acc, s_red, lane);
+        } else {
+            // pixels of the one-ROI kernel's lanes hl (acc) and hl + 32 (acc1)
+            E acc1[NRED];
 #pragma unroll
-                for (int l = 0; l < NV; l++) acc[1 + l] += t1 * dudt[l];
-                int h = 1 + NV;
-#pragma unroll
-                for (int l = 0; l < NV; l++)
-#pragma unroll
-                    for (int m = l; m < NV; m++) acc[h++] += t2 * dudt[l] * dudt[m];
-                p += 64;
-            } while (p < S2);
+            for (int k = 0; k < NRED; k++) acc1[k] = E(0);
+#pragma nounroll
+            for (int p = hl; p < S2; p += 64) {
+                pixel_terms(p, first, acc);
+                if (p + 32 < S2) pixel_terms(p + 32, first, acc1);
+            }
+            __syncthreads();   // planes are dead; the scratch reuses them
+            wave_sum_lds2<E, NRED>(acc, acc1, s_red, hl);
         }
-        __syncthreads();   // planes are dead; the scratch reuses them
-        wave_sum_lds<E, NRED>(acc, s_red, lane);
         const E *tot = s_red + kRedChunk * 65;
-        if (lane < NV + NH) s_jh[lane] = (double)tot[1 + lane];
+        if (hl < NV + NH) s_jh[hl] = (double)tot[1 + hl];
         return tot[0];
     };
 
-    if (A.theta_in) {
+    if (R == 1 && A.theta_in) {
         // test hook: one evaluation at a given theta; write err, J, H
         th = (E)A.theta_in[(size_t)n * NV + myl];
         const E e = evaluate(false);
@@ -279,14 +318,14 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         }
         // damp the diagonal in place (it persists, as the reference's)
         __syncthreads();
-        if (lane < NV) {
-            double *dg = s_jh + NV + packed_index<NV>(lane, lane);
+        if (hl < NV) {
+            double *dg = s_jh + NV + packed_index<NV>(hl, hl);
             *dg = (double)((E)*dg * newLambda);
         }
         __syncthreads();
         // rows without information (see kEscapeStep): solved with a 1 on their
         // diagonal, which decouples them, and given the escape step
-        const unsigned zr = zero_rows<NV>(s_jh);
+        const unsigned zr = zero_rows<NV, R>(s_jh);
         double xs[NV];
         int info = ldlt_solve<NV>(s_jh, xs, zr);
         if (info == 0) {
@@ -313,7 +352,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         oldLambda = newLambda;
         oldErr = newErr;
         if constexpr (FT == 7) {
-            const E u4 = readlane_e<E>(upd, 4);
+            const E u4 = roi_read<R, E>(upd, 4, hb);
             if ((double)fabs(u4) < 0.005 && turn) {
                 if (myl == 5) clampv = oclamp = (E)A.nd[3] / E(10);
                 if (myl == 4) clampv = oclamp = E(2);
@@ -378,9 +417,9 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     // and g2 with the final N, bg (CPUmleFit_LM.cpp:3566-3583).
     if constexpr (FT != 8) {
 #pragma unroll
-        for (int i = 0; i < NV; i++) cth[i] = readlane_e<E>(th, i);
+        for (int i = 0; i < NV; i++) cth[i] = roi_read<R, E>(th, i, hb);
         __syncthreads();
-        make_geo<E, FT>(A, cth, g, lane, gl, SC != 0);
+        make_geo<E, FT, R>(A, cth, g, hl, gl, SC != 0, hb);
     } else {
         // the last evaluation's geometry again (x, y, z, w, u unchanged in
         // cth; recomputing it is cheaper than keeping it live in registers)
@@ -390,55 +429,73 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         make_geo<E, FT>(A, cth, g, lane, gl, SC != 0);
     }
     __syncthreads();
-    stage_a<E, FT>(A, g, s_pl, plane_stride, lane);
+    stage_a<E, FT, R>(A, g, s_pl, plane_stride, hl);
     __syncthreads();
-    if (split) stage_b_cols<E, FT>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, lane);
+    if (split) stage_b_cols<E, FT, R>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, hl);
     constexpr int NF = 1 + NH;
+    // the per-pixel Fisher terms and log-likelihood added to acc
+    auto fisher_terms = [&](int p, E (&acc)[NF]) __attribute__((always_inline)) {
+        int ii, jj;
+        pixel_rc(p, invS, S, ii, jj);
+        E dudt[8], model;
+        if (split) {
+            const int r = gl.rtap[ii];
+            const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
+            pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
+                                 cth[3], g2, dudt, model);
+        }
+        else
+            pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
+        E data = DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0);
+        const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
+        int h = 1;
+#pragma unroll
+        for (int a = 0; a < NV; a++)
+#pragma unroll
+            for (int b = a; b < NV; b++) acc[h++] += dudt[b] * dudt[a] * rmf;
+        if (model > E(0)) {
+            if (data > E(0)) acc[0] += data * log(model) - model - data * log(data) + data;
+            else acc[0] += -model;
+        }
+    };
     E acc[NF];
 #pragma unroll
     for (int k = 0; k < NF; k++) acc[k] = E(0);
-    // with a compile-time size every lane has a first pixel (S * S > 64), so
-    // the loop needs no entry test (with one, the compiler zeroes the
-    // accumulators a second time under the entry mask); not unrolled, which
-    // with a fixed size would cost registers (a wave per SIMD)
-    int p = lane;
-    if (SC || p < S2) {
+    if constexpr (R == 1) {
+        // with a compile-time size every lane has a first pixel (S * S > 64), so
+        // the loop needs no entry test (with one, the compiler zeroes the
+        // accumulators a second time under the entry mask); not unrolled, which
+        // with a fixed size would cost registers (a wave per SIMD)
+        int p = lane;
+        if (SC || p < S2) {
 #pragma nounroll
-        do {
-            int ii, jj;
-            pixel_rc(p, invS, S, ii, jj);
-            E dudt[8], model;
-            if (split) {
-                const int r = gl.rtap[ii];
-                const E *pp = s_pl + gl.ctap[jj] * g.CS + r;
-                pixel_model_h<E, FT>(g, s_h + jj * W * NHP, r, pp + 4 * plane_stride, pp + 5 * plane_stride, cth[2],
-                                     cth[3], g2, dudt, model);
-            }
-            else
-                pixel_model<E, FT>(g, s_pl, plane_stride, gl.rtap[ii], gl.ctap[jj], cth[2], cth[3], g2, dudt, model);
-            E data = DL ? (E)s_data[p] : (E)bload(rsd, p * 4, 0);
-            const E rmf = recip<E>(model);   // Fisher terms dudt_a dudt_b / model
-            int h = 1;
+            do {
+                fisher_terms(p, acc);
+                p += 64;
+            } while (p < S2);
+        }
+        __syncthreads();
+        wave_sum_lds<E, NF>(acc, s_red, lane);
+    } else {
+        E acc1[NF];
 #pragma unroll
-            for (int a = 0; a < NV; a++)
-#pragma unroll
-                for (int b = a; b < NV; b++) acc[h++] += dudt[b] * dudt[a] * rmf;
-            if (model > E(0)) {
-                if (data > E(0)) acc[0] += data * log(model) - model - data * log(data) + data;
-                else acc[0] += -model;
-            }
-            p += 64;
-        } while (p < S2);
+        for (int k = 0; k < NF; k++) acc1[k] = E(0);
+#pragma nounroll
+        for (int p = hl; p < S2; p += 64) {
+            fisher_terms(p, acc);
+            if (p + 32 < S2) fisher_terms(p + 32, acc1);
+        }
+        __syncthreads();
+        wave_sum_lds2<E, NF>(acc, acc1, s_red, hl);
     }
-    __syncthreads();
-    wave_sum_lds<E, NF>(acc, s_red, lane);
     const E *tot = s_red + kRedChunk * 65;
     double mrow[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) mrow[j] = (double)tot[1 + packed_index<NV>(myrow, j)];
-    const double crlb = inverse_diag_rows<NV>(mrow, lane);
+    const double crlb = inverse_diag_rows<NV, R>(mrow, hl, hb);
     const float llf = (float)tot[0];
-    if (A.evals && lane == 0) A.evals[n] = nev + 1;
+    if (!live) return;
+    if (A.evals && hl == 0) A.evals[n] = nev + 1;
     // A later start of a multi-start fit replaces the kept result only where
     // its log-likelihood beats it by 1e-4, compared in FP32 as the
     // reference's callers compare the MEX outputs
@@ -447,11 +504,11 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         const float d = llf - A.LL[n];
         if (!(A.keep == 2 ? d > 1e-4f : d >= 1e-4f)) return;
     }
-    if (lane < NV) {
-        A.P[(size_t)A.N * lane + n] = (float)th;
-        A.CRLB[(size_t)A.N * lane + n] = (float)crlb;
+    if (hl < NV) {
+        A.P[(size_t)A.N * hl + n] = (float)th;
+        A.CRLB[(size_t)A.N * hl + n] = (float)crlb;
     }
-    if (lane == 0) {
+    if (hl == 0) {
         A.P[(size_t)A.N * NV + n] = (float)kk;
         A.LL[n] = llf;
     }
@@ -459,9 +516,9 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
         // per-ROI status flags, as write_roi_status writes them; kept inline
         // here, where the call form costs this kernel SGPR spills
         const float cf = (float)crlb, tf = (float)th;
-        const unsigned long long badc = __ballot(lane < NV && !(cf > 0.f && isfinite(cf)));
-        const unsigned long long badp = __ballot(lane < NV && !isfinite(tf));
-        if (lane == 0) {
+        const unsigned long long badc = roi_ballot<R>(hl < NV && !(cf > 0.f && isfinite(cf)), hb);
+        const unsigned long long badp = roi_ballot<R>(hl < NV && !isfinite(tf), hb);
+        if (hl == 0) {
             unsigned f = 0;
             if (kk >= A.iterations) f |= 1u;
             if (lastinfo != 0) f |= 2u;
@@ -474,11 +531,12 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 }
 
 // ------------------------------------------------------------ launcher
-template <int FT, typename E, int SC = 0>
-inline hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
+// nrois ROIs in R-ROI waves (one wave per workgroup, R LDS regions)
+template <int FT, typename E, int SC = 0, int R = 1>
+inline hipError_t launch_t(const Dev &A, int nrois, hipStream_t st)
 {
-    const size_t lds = lds_bytes(FT, sizeof(E) == 8, A.S);   // A.S = largest ROI size of the launch
-    hipLaunchKernelGGL((fit_kernel<FT, E, SC>), dim3(nblocks), dim3(64), lds, st, A);
+    const size_t lds = R * lds_bytes(FT, sizeof(E) == 8, A.S);   // A.S = largest ROI size of the launch
+    hipLaunchKernelGGL((fit_kernel<FT, E, SC, R>), dim3((nrois + R - 1) / R), dim3(64), lds, st, A);
     return hipGetLastError();
 }
 
@@ -491,8 +549,8 @@ inline hipError_t launch_t(const Dev &A, int nblocks, hipStream_t st)
 inline bool sized_kernel(int, int S) { return S == 11 || S == 13 || S == 15 || S == 17; }
 // Launch the specialised kernel for S (sized_kernel(ft, S) must hold);
 // in bsf_fit_sized6.hip / bsf_fit_sized7.hip.
-hipError_t launch_sized6(const Dev &A, int nblocks, hipStream_t st, int S);
-hipError_t launch_sized7(const Dev &A, int nblocks, hipStream_t st, int S);
+hipError_t launch_sized6(const Dev &A, int nrois, hipStream_t st, int S, int R = 1);
+hipError_t launch_sized7(const Dev &A, int nrois, hipStream_t st, int S, int R = 1);
 // FP64: fittype 6 at every one of these sizes, fittype 7 up to S = 15
 inline bool sized_kernel_f64(int ft, int S)
 {

@@ -6,29 +6,46 @@
 // kernels (bsf_fit_sized6.hip, bsf_fit_sized7.hip).
 #include "bsf_fit_kernel.h"
 
+#include <atomic>
+
 namespace bsf {
 
 // ------------------------------------------------------------ launchers
-// planes, the column sums of the two-pass stage B after them, and the
-// reduction scratch over both
-static size_t plane_elems(int fittype, int f64, int S)
+size_t lds_bytes(int fittype, int f64, int S) { return fit_lds_bytes(fittype, f64, S); }
+
+// --------------------------------------------------------------- tiles
+namespace {
+// recorded tiles, [fittype 6 / 7][FP64][ROI size]; 0: the default
+std::atomic<int> g_tile[2][2][kTileMaxS + 1];
+
+// measured on MI355X (131,072 ROIs per launch, both tiles' bsf_tune
+// timings): two ROIs per wave win for FP32 3D fits of up to 9 x 9 pixels
+// (2-9 %), where the per-ROI work that does not spread over pixels weighs
+// most; from 11 x 11 up, and for fittype 7 and FP64 throughout, one ROI per
+// wave is faster (a two-ROI wave holds twice the LDS and registers, so fewer
+// waves hide each other's latency)
+int default_tile(int fittype, int f64, int S) { return fittype == 6 && !f64 && S <= 9 ? 2 : 1; }
+}  // namespace
+
+int tile_possible(int fittype, int f64, int S, int tile)
 {
-    const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
-    const int NPL = fittype == 6 ? 2 : fittype == 7 ? 3 : 8;
-    const size_t W = S + 4;
-    const size_t pl = (size_t)NPL * W * W;
-    const size_t h = col_sum_elems(fittype, S) + 3;   // after the planes, 16-byte aligned
-    const size_t red = (size_t)red_elems(1 + NV + NV * (NV + 1) / 2);
-    const size_t e = (fittype != 8 && !f64) || stage_b_split(fittype, f64, S) ? pl + h : pl;
-    return e > red ? e : red;
+    if (tile == 1) return 1;
+    return tile == 2 && (fittype == 6 || fittype == 7) && 2 * lds_bytes(fittype, f64, S) <= kTileLdsMax;
 }
 
-size_t lds_bytes(int fittype, int f64, int S)
+int fit_tile(int fittype, int f64, int S)
 {
-    const int NV = fittype == 6 ? 5 : fittype == 7 ? 6 : 8;
-    return fit_lds_layout(S, f64 ? 8 : 4, plane_elems(fittype, f64, S), (NV + NV * (NV + 1) / 2) * (f64 ? 1 : 2),
-                          fit_data_in_lds(fittype))
-        .total;
+    if ((fittype != 6 && fittype != 7) || S < 1 || S > kTileMaxS) return 1;
+    const int t = g_tile[fittype - 6][f64 ? 1 : 0][S].load(std::memory_order_relaxed);
+    return t ? t : default_tile(fittype, f64, S);
+}
+
+int set_fit_tile(int fittype, int f64, int S, int tile)
+{
+    if ((fittype != 6 && fittype != 7) || S < 1 || S > kTileMaxS || tile < 0 || tile > 2) return -1;
+    if (tile && !tile_possible(fittype, f64, S, tile)) return -1;
+    g_tile[fittype - 6][f64 ? 1 : 0][S].store(tile, std::memory_order_relaxed);
+    return 0;
 }
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
@@ -66,6 +83,18 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     A.evals = L.evals;
     const int nb = L.count;
     const int sized = !L.rois || L.uniform ? L.S : 0;
+    // two ROIs per wave where the launch's tile asks for it (fittypes 6 / 7,
+    // no device-side count and no test hook)
+    const int tile = L.tile ? L.tile : fit_tile(L.fittype, L.f64, L.S);
+    if (tile == 2 && tile_possible(L.fittype, L.f64, L.S, 2) && !L.ncount && !L.theta_in) {
+        switch (L.fittype * 2 + (L.f64 ? 1 : 0)) {
+        case 12: return sized_kernel(6, sized) ? launch_sized6(A, nb, st, sized, 2) : launch_t<6, float, 0, 2>(A, nb, st);
+        case 13: return launch_t<6, double, 0, 2>(A, nb, st);
+        case 14: return sized_kernel(7, sized) ? launch_sized7(A, nb, st, sized, 2) : launch_t<7, float, 0, 2>(A, nb, st);
+        case 15: return launch_t<7, double, 0, 2>(A, nb, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (L.fittype * 2 + (L.f64 ? 1 : 0)) {
     case 12: return sized_kernel(6, sized) ? launch_sized6(A, nb, st, sized) : launch_t<6, float>(A, nb, st);
     case 13: return sized_kernel_f64(6, sized) ? launch_sized6d(A, nb, st, sized) : launch_t<6, double>(A, nb, st);

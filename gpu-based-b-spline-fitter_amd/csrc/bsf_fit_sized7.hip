@@ -5,13 +5,18 @@
 
 namespace bsf {
 
-hipError_t launch_sized7(const Dev &A, int nblocks, hipStream_t st, int S)
+hipError_t launch_sized7(const Dev &A, int nrois, hipStream_t st, int S, int R)
 {
-    switch (S) {
-    case 11: return launch_t<7, float, 11>(A, nblocks, st);
-    case 13: return launch_t<7, float, 13>(A, nblocks, st);
-    case 15: return launch_t<7, float, 15>(A, nblocks, st);
-    case 17: return launch_t<7, float, 17>(A, nblocks, st);
+    // R = 2: two ROIs per wave (fit_kernel's tile of two half-waves)
+    switch (S * 4 + R) {
+    case 11 * 4 + 1: return launch_t<7, float, 11>(A, nrois, st);
+    case 13 * 4 + 1: return launch_t<7, float, 13>(A, nrois, st);
+    case 15 * 4 + 1: return launch_t<7, float, 15>(A, nrois, st);
+    case 17 * 4 + 1: return launch_t<7, float, 17>(A, nrois, st);
+    case 11 * 4 + 2: return launch_t<7, float, 11, 2>(A, nrois, st);
+    case 13 * 4 + 2: return launch_t<7, float, 13, 2>(A, nrois, st);
+    case 15 * 4 + 2: return launch_t<7, float, 15, 2>(A, nrois, st);
+    case 17 * 4 + 2: return launch_t<7, float, 17, 2>(A, nrois, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -37,9 +37,21 @@ struct FitLaunch {
     float *dbg;             // test hook (device): err, J, H per ROI
     unsigned char *status;  // device: per-ROI status flags (bsf_roi_flag), or NULL
     int *evals;             // device: per-ROI evaluations run, or NULL
+    int tile;               // ROIs per wave: 1, or 2 (fittypes 6 / 7)
 };
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st);
+
+// ROIs per wave of the fit kernel, per (fittype, precision, ROI size) of a
+// launch (its largest size with a ROI table): the measured default, or the
+// tile set_fit_tile / the autotuner recorded.  A launch whose FitLaunch.tile
+// is 0 takes it from here.  Fittype 8, device-side counts and tiles whose
+// LDS (two regions) exceed kTileLdsMax run one ROI per wave.
+constexpr int kTileMaxS = 64;
+constexpr size_t kTileLdsMax = 64 * 1024;
+int fit_tile(int fittype, int f64, int S);
+int set_fit_tile(int fittype, int f64, int S, int tile);   // tile 0: back to the default; -1 if out of range
+int tile_possible(int fittype, int f64, int S, int tile);  // the launch would run with this tile
 // LDS bytes of one fit workgroup (one wave) for ROI size S
 size_t lds_bytes(int fittype, int f64, int S);
 

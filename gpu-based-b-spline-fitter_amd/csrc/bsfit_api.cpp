@@ -200,16 +200,18 @@ int bsf_num_params(int fittype) { return nv_of(fittype); }
 const char *bsf_last_error(void) { return g_err.c_str(); }
 
 static int fit_device(const bsf_request *r, bsf_result *res, unsigned char *status, void *stream,
-                      const bsf_roi *rois = nullptr, int *evals = nullptr)
+                      const bsf_roi *rois = nullptr, int *evals = nullptr, int tile = 0)
 {
     int rc = validate(r, res);
     if (rc) return rc;
+    if (tile < 0 || tile > 2) return fail(BSF_ERR_ARG, "bsf: tile %d (0, 1 or 2 ROIs per wavefront)", tile);
     if ((rc = check_device(r->sz, r->fittype, r->precision == BSF_PREC_F64))) return rc;
     if (r->nfits == 0) return BSF_OK;
     bsf::FitLaunch L = make_launch(r, r->imstack, r->coeff, r->coeff3, res->P, res->CRLB, res->LL);
     L.status = status;
     L.evals = evals;
     L.rois = (const bsf::RoiDesc *)rois;
+    L.tile = tile;
     hipError_t e = launch_all(L, (hipStream_t)stream);
     if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: kernel launch failed: %s", hipGetErrorString(e));
     return BSF_OK;
@@ -237,7 +239,59 @@ int bsf_fit_device_ex(const bsf_request *r, bsf_result *res, const bsf_fit_extra
 {
     g_err.clear();
     if (!x) return fit_device(r, res, nullptr, stream);
-    return fit_device(r, res, x->status, stream, x->rois, x->evals);
+    return fit_device(r, res, x->status, stream, x->rois, x->evals, x->tile);
+}
+
+int bsf_tune(const bsf_request *r, bsf_result *res, int reps, int *tile, float *ms, void *stream)
+{
+    g_err.clear();
+    int rc = validate(r, res);
+    if (rc) return rc;
+    if (!tile) return fail(BSF_ERR_ARG, "bsf: NULL tile");
+    if (reps < 1 || reps > 1000) return fail(BSF_ERR_ARG, "bsf: reps %d (1..1000)", reps);
+    const int f64 = r->precision == BSF_PREC_F64;
+    if ((rc = check_device(r->sz, r->fittype, f64))) return rc;
+    if (ms) ms[0] = ms[1] = 0.f;
+    *tile = 1;
+    if (r->nfits == 0) return BSF_OK;
+    const hipStream_t st = (hipStream_t)stream;
+    bsf::FitLaunch L = make_launch(r, r->imstack, r->coeff, r->coeff3, res->P, res->CRLB, res->LL);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best = 0.f;
+    for (int t = 1; t <= 2 && e == hipSuccess; t++) {
+        if (!bsf::tile_possible(r->fittype, f64, r->sz, t)) continue;
+        L.tile = t;
+        e = launch_all(L, st);   // warm-up: code object load, caches
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int k = 0; k < reps && e == hipSuccess; k++) e = launch_all(L, st);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t_ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t_ms, e0, e1);
+        if (e != hipSuccess) break;
+        t_ms /= reps;
+        if (ms) ms[t - 1] = t_ms;
+        if (t == 1 || t_ms < best) best = t_ms, *tile = t;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: tuning launch failed: %s", hipGetErrorString(e));
+    if (r->fittype == 6 || r->fittype == 7) bsf::set_fit_tile(r->fittype, f64, r->sz, *tile);
+    return BSF_OK;
+}
+
+int bsf_fit_tile(int fittype, int precision, int sz) { return bsf::fit_tile(fittype, precision == BSF_PREC_F64, sz); }
+
+int bsf_set_fit_tile(int fittype, int precision, int sz, int tile)
+{
+    g_err.clear();
+    if (precision != BSF_PREC_MIX && precision != BSF_PREC_F64)
+        return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", precision);
+    if (bsf::set_fit_tile(fittype, precision == BSF_PREC_F64, sz, tile))
+        return fail(BSF_ERR_ARG, "bsf: tile %d is not possible for fittype %d, ROI size %d", tile, fittype, sz);
+    return BSF_OK;
 }
 
 // Host-array fit from nstarts starts (rows of `starts`, n_initP values each;

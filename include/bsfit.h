@@ -144,11 +144,15 @@ typedef struct bsf_roi {
  *   status  [nfits] per-ROI flags (enum bsf_roi_flag)
  *   evals   [nfits] ROI evaluations each fit ran (model, derivatives and
  *           pixel sums over the whole ROI; the initial one, one per accepted or
- *           rejected step, and the final Fisher pass) */
+ *           rejected step, and the final Fisher pass)
+ * and the launch's tile (ROIs per wavefront, see bsf_tune): 0 takes the
+ * recorded choice for (fittype, precision, sz), 1 or 2 forces it (2 is
+ * taken by fittypes 6 and 7 only; elsewhere the launch runs with 1). */
 typedef struct bsf_fit_extras {
     const bsf_roi *rois;
     unsigned char *status;
     int *evals;
+    int tile;
 } bsf_fit_extras;
 
 /* bsf_fit_device (the launch wrappers, GPUmleFit_LM/wrapper.cu:256-279) with
@@ -156,6 +160,28 @@ typedef struct bsf_fit_extras {
  * fitted by the same kernel and code as a uniform launch of its size; a
  * launch of mixed sizes gives each ROI the results it has alone, bit for bit. */
 int bsf_fit_device_ex(const bsf_request *req, bsf_result *res, const bsf_fit_extras *x, void *stream);
+
+/* Autotuner of the fit kernel's tile: one ROI per 64-lane wavefront, or two
+ * (each half-wave fits its own ROI, and the work of a fit that does not
+ * spread over pixels -- spline geometry, the damped solve, the LM control --
+ * runs once for both).  The reference has one shape, a thread per ROI in
+ * blocks of 64 (GPUmleFit_LM/mexFunction.cpp:596-601), so this has no
+ * counterpart there; it is part of the session API's tuning surface.
+ * Fits `req` (device buffers, as bsf_fit_device) with each possible tile:
+ * one untimed launch, then `reps` launches timed with HIP events on
+ * `stream` (waited for); records the fastest for (fittype, precision, sz)
+ * in the process-wide table that later fits of that kind use, and returns
+ * it in *tile; ms (2 floats, may be NULL) receives each tile's mean launch
+ * time in milliseconds (0 where the tile is not possible).  Both tiles give
+ * the same results bit for bit; res holds them afterwards.  Fittype 8 and
+ * the 4Pi fitter have one tile. */
+int bsf_tune(const bsf_request *req, bsf_result *res, int reps, int *tile, float *ms, void *stream);
+
+/* The recorded tile for (fittype, precision, sz) (1 or 2), and setting it
+ * by hand (tile 0 restores the measured default; BSF_ERR_ARG where the tile
+ * is not possible).  No reference counterpart (see bsf_tune). */
+int bsf_fit_tile(int fittype, int precision, int sz);
+int bsf_set_fit_tile(int fittype, int precision, int sz, int tile);
 
 /* Multi-start fit: every ROI is fitted from nstarts starts (rows of
  * `starts`, req->n_initP values each: fittype 7 {z, w}, fittype 8

@@ -203,3 +203,42 @@ def test_roi_table_layout():
     assert t.dtype.itemsize == C.sizeof(RoiDesc) == 16
     assert list(t["off"]) == [0, 121, 121 + 169]
     assert list(t["sz"]) == [11, 13, 17]
+
+
+def test_tile_table_without_device():
+    """The tile table (bsf_fit_tile / bsf_set_fit_tile) is host state:
+    fittypes 6 and 7 take tiles 1 and 2, fittype 8 only 1, tile 0 restores
+    the default; bsf_tune and bsf_fit_device_ex check their arguments before
+    any device work."""
+    import bsfit
+    from bsfit.api import Extras, Request, Result, _request
+    lib = bsfit.load()
+    for ft in (6, 7):
+        for prec in (bsfit.PREC_MIX, bsfit.PREC_F64):
+            d = bsfit.fit_tile(ft, prec, 13)
+            assert d in (1, 2)
+            try:
+                bsfit.set_fit_tile(ft, prec, 13, 2)
+                assert bsfit.fit_tile(ft, prec, 13) == 2
+                bsfit.set_fit_tile(ft, prec, 13, 1)
+                assert bsfit.fit_tile(ft, prec, 13) == 1
+            finally:
+                bsfit.set_fit_tile(ft, prec, 13, 0)
+            assert bsfit.fit_tile(ft, prec, 13) == d
+    assert bsfit.fit_tile(8, bsfit.PREC_MIX, 13) == 1
+    for bad in ((8, bsfit.PREC_MIX, 13, 2), (6, bsfit.PREC_MIX, 13, 3), (6, bsfit.PREC_MIX, 0, 1),
+                (6, 7, 13, 1), (7, bsfit.PREC_MIX, 65, 1)):
+        with pytest.raises(bsfit.BsfError) as e:
+            bsfit.set_fit_tile(*bad)
+        assert e.value.code == 1
+    buf = np.zeros(4096, np.float32)
+    res = Result(buf.ctypes.data, buf.ctypes.data, buf.ctypes.data)
+    req = _request(6, 13, 1, 50, buf.ctypes.data, (16, 16, 16), None, None, bsfit.PREC_MIX, buf.ctypes.data)
+    t, ms = C.c_int(0), (C.c_float * 2)()
+    lib.bsf_tune.argtypes = [C.POINTER(Request), C.POINTER(Result), C.c_int, C.POINTER(C.c_int),
+                             C.POINTER(C.c_float), C.c_void_p]
+    assert lib.bsf_tune(C.byref(req), C.byref(res), 0, C.byref(t), ms, None) == 1
+    lib.bsf_fit_device_ex.argtypes = [C.POINTER(Request), C.POINTER(Result), C.POINTER(Extras), C.c_void_p]
+    x = Extras(None, None, None, 3)
+    assert lib.bsf_fit_device_ex(C.byref(req), C.byref(res), C.byref(x), None) == 1
+    assert "tile" in lib.bsf_last_error().decode()
