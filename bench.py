@@ -156,6 +156,10 @@ def serve(args):
         pools["ft8", S] = gen.make_rois(8, pool_n, S, seed + 40 + S, c5, c53)
     prec = bsfit.PREC_F64 if args.precision == "f64" else bsfit.PREC_MIX
     sess = bsfit.Session(arena_bytes=256 << 20, kernel_streams=args.kernel_streams, precision=prec)
+    if args.graphs:
+        sess.set_graphs(args.graphs)
+    if args.hetero:
+        sess.set_hetero(True)
     w4 = pools["4pi", 11]
     models = {"ft6": sess.register_model(6, c3), "ft7": sess.register_model(7, c4),
               "4pi": sess.register_model(bsfit.FT_4PI, c4pi, nch=w4.nch, shared=w4.shared,
@@ -234,6 +238,7 @@ def serve(args):
     cspan = time.perf_counter() - cstart
     gc.enable()
     gc.unfreeze()
+    gstats = sess.graph_stats() if args.graphs else None
     sess.close()
     out = {
         "metric": "serving: ROI fits/sec and frame latency, streamed 3D/4D/4Pi/5D mix",
@@ -248,6 +253,8 @@ def serve(args):
                                "40/20/20/20 3D/colour-4D/4Pi(4 ch)/vortex-5D, S uniform in {11,13,15,17}, "
                                "50 LM iterations" % (args.rate, args.rois),
                    "kernel_streams": args.kernel_streams, "precision": args.precision,
+                   "submission": (("graphs, %d slots" % args.graphs) if args.graphs else "per-call launches")
+                   + (", heterogeneous fittype 6-8 launch" if args.hetero else ""),
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "offered_fits_per_s": nroi / float(arrivals[-1]),
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p90": float(np.percentile(lat_ms, 90)),
@@ -255,6 +262,8 @@ def serve(args):
                        "mean": float(lat_ms.mean())},
         "saturated": {"fits_per_s": nroi / cspan, "frames_per_s": args.frames / cspan, "depth": args.depth},
     }
+    if gstats:
+        out["graphs"] = {"captured": gstats[0], "replayed": gstats[1], "per_call": gstats[2]}
     print(json.dumps(out))
 
 
@@ -467,6 +476,11 @@ def main():
     ap.add_argument("--rois", type=int, default=2000, help="serve: mean ROIs per frame")
     ap.add_argument("--kernel-streams", type=int, default=16, help="serve: session kernel streams")
     ap.add_argument("--depth", type=int, default=16, help="serve: frames in flight in the closed-loop phase")
+    ap.add_argument("--hetero", action="store_true",
+                    help="serve: one heterogeneous launch for a frame's fittype 6-8 groups")
+    ap.add_argument("--graphs", type=int, default=0, metavar="SLOTS",
+                    help="serve: graph mode with SLOTS arena slots (frames replay captured graphs of their "
+                         "launches); 0: per-call launches")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()

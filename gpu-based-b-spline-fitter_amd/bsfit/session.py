@@ -64,8 +64,11 @@ def _bind(lib):
     lib.bsf_submit.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_int, C.POINTER(C.c_ulonglong)]
     lib.bsf_poll.argtypes = [C.c_void_p, C.c_ulonglong, C.c_int]
     lib.bsf_session_inflight.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
+    lib.bsf_session_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    lib.bsf_session_set_hetero.argtypes = [C.c_void_p, C.c_int]
+    lib.bsf_session_graph_stats.argtypes = [C.c_void_p] + [C.POINTER(C.c_ulonglong)] * 3
     for f in ("bsf_session_create", "bsf_session_destroy", "bsf_model_register", "bsf_model_register_samples",
-              "bsf_model_release",
+              "bsf_model_release", "bsf_session_set_graphs", "bsf_session_graph_stats", "bsf_session_set_hetero",
               "bsf_session_set_precision", "bsf_submit", "bsf_poll", "bsf_session_inflight"):
         getattr(lib, f).restype = C.c_int
     lib._bsf_session_bound = True
@@ -208,6 +211,23 @@ class Session:
 
     def set_precision(self, precision):
         _check(self._lib.bsf_session_set_precision(self._h, int(precision)))
+
+    def set_graphs(self, slots=32):
+        """Graph mode (bsf_session_set_graphs) with `slots` arena slots:
+        frames replay captured graphs of their launches; 0 (or False) returns
+        to per-call launches.  Waits for the frames in flight."""
+        _check(self._lib.bsf_session_set_graphs(self._h, int(slots)))
+
+    def set_hetero(self, on=True):
+        """Heterogeneous launches (bsf_session_set_hetero): a frame's
+        single-start fittype 6-8 groups run as one launch."""
+        _check(self._lib.bsf_session_set_hetero(self._h, 1 if on else 0))
+
+    def graph_stats(self):
+        """(graphs captured, frames replayed, frames launched call by call in graph mode)."""
+        a, b, c = C.c_ulonglong(), C.c_ulonglong(), C.c_ulonglong()
+        _check(self._lib.bsf_session_graph_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def inflight(self):
         n, b = C.c_int(), C.c_size_t()

@@ -53,6 +53,8 @@ template <typename E, int NVMAX>
 __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 {
     Dev A = A4.b;
+    // a device-side ROI count (launches with a ROI table only): blocks past it exit
+    if (A.ncount && (int)blockIdx.x + A.roi_offset >= *A.ncount) return;
     // mixed sizes: block b takes ROI b (every XCD the same size mix, as fit_kernel)
     const int n = (A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + A.roi_offset;
     if (n >= A.N) return;
@@ -399,6 +401,8 @@ template <typename E, int NVMAX>
 __global__ __launch_bounds__(256) void fit4pi_wpc_kernel(Dev4 A4)
 {
     Dev A = A4.b;
+    // a device-side ROI count (launches with a ROI table only): blocks past it exit
+    if (A.ncount && (int)blockIdx.x + A.roi_offset >= *A.ncount) return;
     // mixed sizes: block b takes ROI b (every XCD the same size mix, as fit_kernel)
     const int n = (A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + A.roi_offset;
     if (n >= A.N) return;
@@ -750,6 +754,8 @@ hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st)
     A.dbg = nullptr;
     A.status = L.status;
     A.evals = L.evals;
+    if (L.ncount && !L.rois) return hipErrorInvalidValue;   // a device count needs the identity block mapping
+    A.ncount = L.ncount;
     A4.nch = L.nch;
     int n = 0;
     for (int p = 0; p < 6; p++) {

@@ -70,8 +70,11 @@ __device__ __forceinline__ void roi_init4(const float *d, int S, E *v)
 // holds the pixel sums of two lanes of the one-ROI kernel and reduces them in
 // its order, so both forms give the same bits.  (The geometry takes 8 lanes
 // per spline axis: fittype 8's five axes need 40, so it runs with R = 1.)
+// The fit of the ROIs of workgroup `block` of `nblocks`; fit_kernel runs it
+// for its own workgroup, the heterogeneous kernel (bsf_fit_hetero.hip) for
+// the workgroup's place within its launch entry.
 template <int FT, typename E, int SC, int R = 1>
-__global__ __launch_bounds__(64) void fit_kernel(Dev A)
+__device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblocks)
 {
     static_assert(R == 1 || (R == 2 && FT != 8), "two ROIs per wave: fittypes 6 and 7");
     constexpr int NV = Ft<FT>::NV;
@@ -82,11 +85,11 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
 
     // with a device-side ROI count, the blocks below it are spread over the
     // XCDs as the whole grid otherwise is (the rest exit at once)
-    int G = (int)gridDim.x;
+    int G = nblocks;
     if (R == 1 && A.ncount) {   // (R = 2 launches take no device-side count)
         const int c = *A.ncount - A.roi_offset;
         G = c < 0 ? 0 : c < G ? c : G;
-        if ((int)blockIdx.x >= G) return;
+        if (block >= G) return;
     }
     // uniform launches: each XCD takes a contiguous range of ROIs (their
     // column-major output stores meet in one L2); launches of mixed sizes,
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     const int lane = threadIdx.x;
     const int hl = R == 1 ? lane : (lane & (L - 1));   // lane within the ROI
     const int hb = lane - hl;                          // the ROI's first lane
-    int n = ((A.rois ? (int)blockIdx.x : xcd_block(blockIdx.x, G)) * R + hb / L) + A.roi_offset;
+    int n = ((A.rois ? block : xcd_block(block, G)) * R + hb / L) + A.roi_offset;
     bool live = true;   // R = 2: the last block's second half may have no ROI of its own
     if (n >= A.N) {
         if constexpr (R == 1) return;
@@ -530,7 +533,50 @@ __global__ __launch_bounds__(64) void fit_kernel(Dev A)
     }
 }
 
+template <int FT, typename E, int SC, int R = 1>
+__global__ __launch_bounds__(64) void fit_kernel(Dev A)
+{
+    fit_roi<FT, E, SC, R>(A, (int)blockIdx.x, (int)gridDim.x);
+}
+
 // ------------------------------------------------------------ launcher
+// Kernel arguments of a launch.
+inline Dev fit_dev(const FitLaunch &L)
+{
+    Dev A;
+    A.data = L.data;
+    A.coeff = L.coeff;
+    A.coeff3 = L.coeff3;
+    A.rois = L.rois;
+    A.P = L.P;
+    A.CRLB = L.CRLB;
+    A.LL = L.LL;
+    A.N = L.N;
+    A.S = L.S;
+    A.iterations = L.iterations;
+    A.roi_offset = L.roi_offset;
+    A.ncount = L.ncount;
+    for (int a = 0; a < 5; a++) A.nd[a] = L.cdims[a] - 3;
+    A.s1 = L.cdims[0];
+    A.s2 = A.s1 * L.cdims[1];
+    A.s3 = A.s2 * (L.ndim > 2 ? L.cdims[2] : 1);
+    A.s4 = A.s3 * (L.ndim > 3 ? L.cdims[3] : 1);
+    {
+        long nc = 1;
+        for (int a = 0; a < L.ndim; a++) nc *= L.cdims[a];
+        A.cbytes = (int)(nc * 4);
+        A.c3bytes = (int)(A.s3 * 4);
+    }
+    for (int i = 0; i < 4; i++) A.initP[i] = L.initP[i];
+    A.has_initP = L.has_initP;
+    A.keep = L.keep;
+    A.theta_in = L.theta_in;
+    A.dbg = L.dbg;
+    A.status = L.status;
+    A.evals = L.evals;
+    return A;
+}
+
 // nrois ROIs in R-ROI waves (one wave per workgroup, R LDS regions)
 template <int FT, typename E, int SC = 0, int R = 1>
 inline hipError_t launch_t(const Dev &A, int nrois, hipStream_t st)

@@ -50,37 +50,7 @@ int set_fit_tile(int fittype, int f64, int S, int tile)
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
 {
-    Dev A;
-    A.data = L.data;
-    A.coeff = L.coeff;
-    A.coeff3 = L.coeff3;
-    A.rois = L.rois;
-    A.P = L.P;
-    A.CRLB = L.CRLB;
-    A.LL = L.LL;
-    A.N = L.N;
-    A.S = L.S;
-    A.iterations = L.iterations;
-    A.roi_offset = L.roi_offset;
-    A.ncount = L.ncount;
-    for (int a = 0; a < 5; a++) A.nd[a] = L.cdims[a] - 3;
-    A.s1 = L.cdims[0];
-    A.s2 = A.s1 * L.cdims[1];
-    A.s3 = A.s2 * (L.ndim > 2 ? L.cdims[2] : 1);
-    A.s4 = A.s3 * (L.ndim > 3 ? L.cdims[3] : 1);
-    {
-        long nc = 1;
-        for (int a = 0; a < L.ndim; a++) nc *= L.cdims[a];
-        A.cbytes = (int)(nc * 4);
-        A.c3bytes = (int)(A.s3 * 4);
-    }
-    for (int i = 0; i < 4; i++) A.initP[i] = L.initP[i];
-    A.has_initP = L.has_initP;
-    A.keep = L.keep;
-    A.theta_in = L.theta_in;
-    A.dbg = L.dbg;
-    A.status = L.status;
-    A.evals = L.evals;
+    const Dev A = fit_dev(L);
     const int nb = L.count;
     const int sized = !L.rois || L.uniform ? L.S : 0;
     // two ROIs per wave where the launch's tile asks for it (fittypes 6 / 7,

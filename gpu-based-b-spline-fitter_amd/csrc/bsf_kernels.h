@@ -42,6 +42,14 @@ struct FitLaunch {
 
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st);
 
+// One launch over the ROIs of up to kHeteroMax fit launches of fittypes 6-8
+// (one precision, one ROI per wave): workgroups are dealt to the entries in
+// order, count blocks each, and every workgroup runs its entry's fittype.
+// The kernel holds every fittype's code, so it takes the largest register
+// footprint (fittype 8's) for all of them.
+constexpr int kHeteroMax = 4;
+hipError_t launch_fit_hetero(const FitLaunch *L, int n, hipStream_t st);
+
 // ROIs per wave of the fit kernel, per (fittype, precision, ROI size) of a
 // launch (its largest size with a ROI table): the measured default, or the
 // tile set_fit_tile / the autotuner recorded.  A launch whose FitLaunch.tile
@@ -71,6 +79,7 @@ struct Fit4PiLaunch {
     int wpc;                  // 1: a workgroup of nch waves per ROI, one per channel; 0: one wave per ROI
     unsigned char *status;    // device: per-ROI status flags (bsf_roi_flag), or NULL
     int *evals;               // device: per-ROI evaluations run, or NULL
+    const int *ncount;        // device: ROIs to fit (the rest of the grid exits), or NULL; needs rois
 };
 
 hipError_t launch_fit_4pi(const Fit4PiLaunch &L, hipStream_t st);

@@ -14,6 +14,16 @@
 // The arena is a ring: frames take slots in submission order and give them
 // back in submission order once their results have been delivered, so the
 // device and pinned slots of a frame are never reused while it runs.
+//
+// Graph mode (bsf_session_set_graphs): the arena is cut into equal slots
+// taken in turn.  A frame's launches are matched to a plan
+// (same models, iterations, starts and optional inputs, with ROI counts and
+// sizes within the plan's capacities), whose inputs and outputs sit at fixed
+// offsets of a slot; the plan's launches -- one graph branch each, reading
+// their ROI counts from the slot's header -- are captured once per slot, so
+// a frame is one H2D copy, one hipGraphLaunch and one D2H copy of the output
+// section.  A frame that outgrows its plan gets a plan of twice its counts
+// (and is captured anew); one that no slot can hold goes the per-call way.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,6 +94,23 @@ struct Frame {
     bool delivered = false;
 };
 
+// Graph mode: the arena splits into equal slots (at most kMaxGraphSlots),
+// and a frame whose launches match a plan (same models, iterations, starts
+// and optional inputs; counts and ROI sizes within the plan's capacities)
+// replays that plan's graph, captured once per slot: the launches with
+// device-side ROI counts, at fixed slot offsets.
+constexpr int kMaxGraphSlots = 64;
+
+struct Plan {
+    std::vector<Launch> L;   // the keys; n = capacity (ROIs), Smax = largest ROI size; slot offsets
+    int f64 = 0;
+    size_t o_hdr = 0;        // per-launch ROI counts (int32), read by the kernels
+    size_t o_data = 0;       // ROI data, packed launch after launch
+    size_t data_cap = 0;     // bytes available to it
+    size_t o_out = 0;        // output section: [o_out, slot end)
+    hipGraphExec_t exec[kMaxGraphSlots] = {};
+};
+
 struct DeviceGuard {
     int old = -1;
     explicit DeviceGuard(int dev)
@@ -114,6 +141,14 @@ struct bsf_session {
     // per-submit scratch, kept to avoid reallocation on the steady-state path
     std::vector<Launch> launches;
     std::vector<int> order;
+    int hetero = 0;          // bsf_session_set_hetero
+    // graph mode (bsf_session_set_graphs)
+    int graphs = 0;          // slots (0: per-call launches)
+    size_t slot_bytes = 0;
+    std::vector<Plan *> plans;
+    std::vector<hipGraphExec_t> old_execs;   // of replaced plans, destroyed once nothing is in flight
+    hipStream_t s_cap[kMaxKernelStreams + 1] = {};   // capture only: [0] origin, [1..] one per launch
+    unsigned long long n_capture = 0, n_replay = 0, n_eager = 0;
 };
 
 namespace {
@@ -279,6 +314,548 @@ int add_model(bsf_session *s, Model &M, hipError_t e, int *model_id)
     return BSF_OK;
 }
 
+// ---------------------------------------------------------------- submission
+// Launch checks of a frame's launch at its largest ROI size: LDS, and the 4Pi
+// kernel choice.
+int launch_lds(const bsf_session *s, Launch &l, int f64)
+{
+    const Model &M = s->models[l.model];
+    // 4Pi: one wave per ROI.  The wave-per-channel kernel halves one
+    // launch's latency on an idle GPU, but with frames overlapping it
+    // gained nothing measurable in the streamed mix (p50 2.17 vs 2.25 ms,
+    // p99 5.19 vs 4.88 ms) and saturated lower (3.15M vs 3.24M fits/s)
+    l.wpc = M.ft == BSF_FT_4PI && bsf::use_wpc_4pi(0, f64, l.Smax, M.nch, M.nv, s->lds_limit);
+    const size_t lds = M.ft == BSF_FT_4PI ? (l.wpc ? bsf::lds_bytes_4pi_wpc(f64, l.Smax, M.nch, M.nv)
+                                                   : bsf::lds_bytes_4pi(f64, l.Smax, M.nch, M.nv))
+                                          : bsf::lds_bytes(M.ft, f64, l.Smax);
+    if (lds > s->lds_limit)
+        return fail(BSF_ERR_ROI_SIZE, "bsf: ROI size %d of model %d needs %zu B of LDS (limit %zu B)", l.Smax,
+                    l.model, lds, s->lds_limit);
+    return BSF_OK;
+}
+
+// Orders a launch's groups largest ROI size first and numbers their rows;
+// returns the launch's data floats.
+size_t launch_rows(Launch &l, const bsf_frame *f)
+{
+    std::stable_sort(l.groups.begin(), l.groups.end(),
+                     [&](int a, int b) { return f->groups[a].sz > f->groups[b].sz; });
+    size_t nd = 0;
+    l.start.clear();
+    int row = 0;
+    for (int gi : l.groups) {
+        const bsf_roi_group &g = f->groups[gi];
+        l.start.push_back(row);
+        row += g.count;
+        nd += (size_t)g.count * g.sz * g.sz * l.nch;
+    }
+    return nd;
+}
+
+// Packs launch l's inputs into the pinned slot h: ROI descriptors at
+// at.o_desc (their offsets count from the float at desc_base), images at
+// byte offset o_data, 4Pi starts / offsets at at.o_dT / o_iz / o_ip.
+void pack_launch(const bsf_frame *f, const Launch &l, const Launch &at, char *h, size_t o_data, long long desc_base)
+{
+    bsf::RoiDesc *desc = (bsf::RoiDesc *)(h + at.o_desc);
+    float *dst = (float *)(h + o_data);
+    long long e0 = 0;
+    int row = 0;
+    for (size_t j = 0; j < l.groups.size(); j++) {
+        const bsf_roi_group &g = f->groups[l.groups[j]];
+        const size_t S2 = (size_t)g.sz * g.sz, cnt = (size_t)g.count;
+        const size_t cs = g.chan_stride ? g.chan_stride : cnt * S2;
+        for (int k = 0; k < l.nch; k++) std::memcpy(dst + e0 + k * cnt * S2, g.data + k * cs, cnt * S2 * 4);
+        for (size_t i = 0; i < cnt; i++) {
+            desc[row + i].off = desc_base + e0 + (long long)(i * S2);
+            desc[row + i].cstride = (int)(cnt * S2);
+            desc[row + i].S = g.sz;
+        }
+        const int nch = l.nch;
+        if (l.has_dT) std::memcpy((float *)(h + at.o_dT) + (size_t)row * 6 * nch, g.dT, cnt * 6 * nch * 4);
+        if (l.has_iz) std::memcpy((float *)(h + at.o_iz) + row, g.initZ, cnt * 4);
+        if (l.has_ip) std::memcpy((float *)(h + at.o_ip) + row, g.initPhi, cnt * 4);
+        e0 += (long long)(cnt * S2 * l.nch);
+        row += g.count;
+    }
+}
+
+// The fit launch (fittypes 6-8) of launch l's first start: slot base d, the
+// kernels' data pointer `data`, ncount as for launch_one.
+bsf::FitLaunch fit_launch(const bsf_session *s, const Launch &l, int f64, char *d, const float *data,
+                          const int *ncount)
+{
+    const Model &M = s->models[l.model];
+    bsf::FitLaunch K;
+    std::memset(&K, 0, sizeof K);
+    K.fittype = M.ft;
+    K.f64 = f64;
+    K.data = data;
+    K.coeff = M.d_coeff;
+    K.coeff3 = M.d_coeff3;
+    K.P = (float *)(d + l.o_P);
+    K.CRLB = (float *)(d + l.o_CR);
+    K.LL = (float *)(d + l.o_LL);
+    K.N = l.n;
+    K.S = l.Smax;
+    K.iterations = l.iterations;
+    K.roi_offset = 0;
+    K.count = l.n;
+    K.rois = (const bsf::RoiDesc *)(d + l.o_desc);
+    K.uniform = !ncount && l.Smin == l.Smax;
+    K.ndim = M.ndim;
+    for (int a = 0; a < 5; a++) K.cdims[a] = M.cdims[a];
+    K.has_initP = l.has_initP;
+    for (int q = 0; q < 4; q++) K.initP[q] = l.initP[0][q];
+    K.ncount = ncount;
+    return K;
+}
+
+// Queues launch l (all its starts) on ks: slot base d, the kernels' data
+// pointer `data` (descriptor offsets count from it), outputs and tables at
+// l's offsets; ncount: a device-side ROI count (l.n is then the capacity).
+hipError_t launch_one(const bsf_session *s, const Launch &l, int f64, char *d, const float *data,
+                      const int *ncount, hipStream_t ks)
+{
+    const Model &M = s->models[l.model];
+    const bsf::RoiDesc *d_desc = (const bsf::RoiDesc *)(d + l.o_desc);
+    float *dP = (float *)(d + l.o_P), *dCR = (float *)(d + l.o_CR), *dLL = (float *)(d + l.o_LL);
+    if (M.ft == BSF_FT_4PI) {
+        bsf::Fit4PiLaunch K;
+        std::memset(&K, 0, sizeof K);
+        K.f64 = f64;
+        K.data = data;
+        K.coeff = M.d_coeff;
+        K.P = dP;
+        K.CRLB = dCR;
+        K.LL = dLL;
+        K.N = l.n;
+        K.S = l.Smax;
+        K.iterations = l.iterations;
+        K.nch = M.nch;
+        K.roi_offset = 0;
+        K.count = l.n;
+        K.rois = d_desc;
+        for (int a = 0; a < 4; a++) K.cdims[a] = M.cdims[a];
+        for (int p = 0; p < 6; p++) K.shared[p] = M.shared[p];
+        for (int k = 0; k < 8; k++) K.phaseShift[k] = M.shift[k];
+        K.dT = l.has_dT ? (const float *)(d + l.o_dT) : nullptr;
+        K.initZ = l.has_iz ? (const float *)(d + l.o_iz) : nullptr;
+        K.initPhi = l.has_ip ? (const float *)(d + l.o_ip) : nullptr;
+        K.wpc = l.wpc;
+        K.ncount = ncount;
+        return bsf::launch_fit_4pi(K, ks);
+    }
+    bsf::FitLaunch K = fit_launch(s, l, f64, d, data, ncount);
+    // multi-start: one launch per start on this stream; after the first,
+    // each keeps the results whose log-likelihood it beats
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < l.nstarts && e == hipSuccess; k++) {
+        for (int q = 0; q < 4; q++) K.initP[q] = l.initP[k][q];
+        K.keep = k == 0 ? 0 : (l.strict ? 2 : 1);
+        e = bsf::launch_fit(K, ks);
+    }
+    return e;
+}
+
+// Room for `bytes` in the arena behind the frames in flight (waiting for the
+// oldest as needed, or BSF_BUSY with BSF_SUBMIT_NONBLOCK).
+int admit(bsf_session *s, size_t bytes, int flags, size_t *off)
+{
+    while ((*off = ring_alloc(s, bytes)) == SIZE_MAX) {
+        if (flags & BSF_SUBMIT_NONBLOCK)
+            return fail(BSF_BUSY, "bsf: arena full (%d frames in flight)", (int)s->inflight.size());
+        Frame &oldest = s->inflight.front();
+        if (oldest.delivered) {
+            retire(s);
+            continue;
+        }
+        int rc = finish(s, oldest);
+        if (rc) return rc;
+    }
+    return BSF_OK;
+}
+
+// Work queued for a frame whose submission failed may still read its slot:
+// wait for it before the slot can be handed out again.
+int abandon(bsf_session *s, Frame &F, hipError_t e)
+{
+    (void)hipStreamSynchronize(s->s_in);
+    for (int j = 0; j < s->nks; j++) (void)hipStreamSynchronize(s->ks[j]);
+    (void)hipStreamSynchronize(s->s_out);
+    for (hipEvent_t ev : F.events) s->event_pool.push_back(ev);
+    for (const auto &w : F.work) s->load[w.first] -= w.second;
+    return fail(BSF_ERR_DEVICE, "bsf: frame submission failed: %s", hipGetErrorString(e));
+}
+
+void add_outputs(Frame &F, const bsf_frame *f, const Launch &l, const Launch &at)
+{
+    for (size_t j = 0; j < l.groups.size(); j++) {
+        const bsf_roi_group &g = f->groups[l.groups[j]];
+        GroupOut go;
+        go.P = g.P;
+        go.CRLB = g.CRLB;
+        go.LL = g.LL;
+        go.count = g.count;
+        go.nv = l.nv;
+        go.start = l.start[j];
+        go.launch_n = at.n;
+        go.o_P = at.o_P;
+        go.o_CR = at.o_CR;
+        go.o_LL = at.o_LL;
+        F.outs.push_back(go);
+    }
+}
+
+// One frame, launched call by call: inputs then outputs in a ring slot of the
+// frame's size (graph mode: whole graph slots), one H2D copy, the launches
+// spread over the kernel streams, one D2H copy.
+int submit_eager(bsf_session *s, const bsf_frame *f, int flags, unsigned long long *ticket, std::vector<Launch> &L,
+                 int f64)
+{
+    size_t o = 0;
+    for (Launch &l : L) {
+        const Model &M = s->models[l.model];
+        int rc = launch_lds(s, l, f64);
+        if (rc) return rc;
+        const size_t nd = launch_rows(l, f);
+        double cost = 0;
+        for (int gi : l.groups) {
+            const bsf_roi_group &g = f->groups[gi];
+            cost += (double)g.count * g.sz * g.sz * l.nch * (double)(1 << (2 * (dim_of(M.ft) - 2)));
+        }
+        l.cost = cost * (l.iterations + 2) * l.nstarts;
+        l.o_desc = o;
+        o += up((size_t)l.n * sizeof(bsf::RoiDesc));
+        l.o_data = o;
+        o += up(nd * 4);
+        l.o_dT = o;
+        o += l.has_dT ? up((size_t)l.n * 6 * l.nch * 4) : 0;
+        l.o_iz = o;
+        o += l.has_iz ? up((size_t)l.n * 4) : 0;
+        l.o_ip = o;
+        o += l.has_ip ? up((size_t)l.n * 4) : 0;
+    }
+    const size_t in_bytes = o;
+    for (Launch &l : L) {
+        l.o_P = o;
+        o += up((size_t)l.n * (l.nv + 1) * 4);
+        l.o_CR = o;
+        o += up((size_t)l.n * l.nv * 4);
+        l.o_LL = o;
+        o += up((size_t)l.n * 4);
+    }
+    const size_t total = o;
+    // in graph mode a frame takes whole slots, so that slots stay aligned
+    const size_t take = s->graphs ? (total + s->slot_bytes - 1) / s->slot_bytes * s->slot_bytes : total;
+    if (take > s->arena)
+        return fail(BSF_ERR_NOMEM, "bsf: frame needs %zu B, more than the whole arena (%zu B)", take, s->arena);
+
+    size_t off;
+    int rc = admit(s, take, flags, &off);
+    if (rc) return rc;
+    char *h = s->h_arena + off, *d = s->d_arena + off;
+    for (Launch &l : L) pack_launch(f, l, l, h, l.o_data, 0);
+
+    Frame F;
+    F.off = off;
+    F.bytes = take;
+    hipEvent_t ev_in = take_event(s);
+    if (!ev_in) return fail(BSF_ERR_DEVICE, "bsf: hipEventCreate failed");
+    F.events.push_back(ev_in);
+    hipError_t e = hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s->s_in);
+    if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
+    // largest launches first, each to the kernel stream with the least queued
+    // work (launches of earlier frames not yet delivered included), so the
+    // big launches of consecutive frames run side by side
+    std::vector<int> &ord = s->order;
+    ord.resize(L.size());
+    for (size_t i = 0; i < L.size(); i++) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return L[a].cost > L[b].cost; });
+    // heterogeneous mode: the single-start launches of fittypes 6-8 go
+    // together, up to kHeteroMax per launch (the first of them carries it)
+    std::vector<std::vector<int>> het;
+    if (s->hetero) {
+        for (int i : ord) {
+            const int ft = s->models[L[i].model].ft;
+            if (ft < 6 || ft > 8 || L[i].nstarts != 1) continue;
+            if (het.empty() || het.back().size() == (size_t)bsf::kHeteroMax) het.emplace_back();
+            het.back().push_back(i);
+        }
+        for (auto &h : het)
+            for (size_t q = 1; q < h.size(); q++) L[h[0]].cost += L[h[q]].cost, L[h[q]].cost = -1;
+    }
+    for (int i : ord) {
+        if (e != hipSuccess) break;
+        Launch &l = L[i];
+        if (l.cost < 0) continue;   // part of a heterogeneous launch
+        int j = 0;
+        for (int q = 1; q < s->nks; q++)
+            if (s->load[q] < s->load[j]) j = q;
+        s->load[j] += l.cost;
+        F.work.emplace_back(j, l.cost);
+        l.stream = j;
+        hipStream_t ks = s->ks[j];
+        e = hipStreamWaitEvent(ks, ev_in, 0);
+        const std::vector<int> *h = nullptr;
+        for (const auto &x : het)
+            if (x[0] == i && x.size() > 1) h = &x;
+        if (e == hipSuccess && h) {
+            bsf::FitLaunch K[bsf::kHeteroMax];
+            for (size_t q = 0; q < h->size(); q++) {
+                const Launch &m = L[(*h)[q]];
+                K[q] = fit_launch(s, m, f64, d, (const float *)(d + m.o_data), nullptr);
+            }
+            e = bsf::launch_fit_hetero(K, (int)h->size(), ks);
+        } else if (e == hipSuccess) {
+            e = launch_one(s, l, f64, d, (const float *)(d + l.o_data), nullptr, ks);
+        }
+        if (e != hipSuccess) break;
+        hipEvent_t ev = take_event(s);
+        if (!ev) {
+            e = hipErrorOutOfMemory;
+            break;
+        }
+        F.events.push_back(ev);
+        e = hipEventRecord(ev, ks);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s->s_out, ev, 0);
+    }
+    hipEvent_t ev_done = nullptr;
+    if (e == hipSuccess) {
+        ev_done = take_event(s);
+        if (!ev_done) e = hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess) {
+        F.events.push_back(ev_done);
+        e = hipMemcpyAsync(h + in_bytes, d + in_bytes, total - in_bytes, hipMemcpyDeviceToHost, s->s_out);
+        if (e == hipSuccess) e = hipEventRecord(ev_done, s->s_out);
+    }
+    if (e != hipSuccess) return abandon(s, F, e);
+    for (Launch &l : L) add_outputs(F, f, l, l);
+    F.ticket = s->next_ticket++;
+    *ticket = F.ticket;
+    s->inflight.push_back(std::move(F));
+    return BSF_OK;
+}
+
+// ---------------------------------------------------------------- graph mode
+bool same_key(const Launch &a, const Launch &b)
+{
+    return a.model == b.model && a.iterations == b.iterations && a.has_initP == b.has_initP &&
+           a.nstarts == b.nstarts && a.strict == b.strict && std::memcmp(a.initP, b.initP, sizeof a.initP) == 0 &&
+           a.has_dT == b.has_dT && a.has_iz == b.has_iz && a.has_ip == b.has_ip;
+}
+
+bool key_less(const Launch &a, const Launch &b)
+{
+    if (a.model != b.model) return a.model < b.model;
+    if (a.iterations != b.iterations) return a.iterations < b.iterations;
+    if (a.nstarts != b.nstarts) return a.nstarts < b.nstarts;
+    if (a.strict != b.strict) return a.strict < b.strict;
+    if (a.has_initP != b.has_initP) return a.has_initP < b.has_initP;
+    if (a.has_dT != b.has_dT) return a.has_dT < b.has_dT;
+    if (a.has_iz != b.has_iz) return a.has_iz < b.has_iz;
+    if (a.has_ip != b.has_ip) return a.has_ip < b.has_ip;
+    return std::memcmp(a.initP, b.initP, sizeof a.initP) < 0;
+}
+
+void drop_plan(bsf_session *s, Plan *P, bool now)
+{
+    for (hipGraphExec_t &x : P->exec)
+        if (x) {
+            if (now) (void)hipGraphExecDestroy(x);
+            else s->old_execs.push_back(x);
+            x = nullptr;
+        }
+    delete P;
+}
+
+void drop_plans(bsf_session *s)   // nothing may be in flight
+{
+    for (Plan *P : s->plans) drop_plan(s, P, true);
+    s->plans.clear();
+    for (hipGraphExec_t x : s->old_execs) (void)hipGraphExecDestroy(x);
+    s->old_execs.clear();
+}
+
+// A plan for launches L (keys, counts n, sizes Smax) with room for twice
+// their counts and their sizes or `prev`'s, whichever is larger; NULL if even
+// the frame's own counts do not fit a slot with its data.
+Plan *make_plan(const bsf_session *s, const std::vector<Launch> &L, int f64, const Plan *prev, size_t data_bytes)
+{
+    for (int grow = 2; grow >= 1; grow--) {
+        Plan *P = new Plan;
+        P->f64 = f64;
+        P->L = L;
+        size_t o = up(L.size() * 4);
+        P->o_hdr = 0;
+        bool ok = true;
+        for (size_t i = 0; i < L.size(); i++) {
+            Launch &l = P->L[i];
+            int cap = L[i].n * grow;
+            if (prev) cap = std::max(cap, prev->L[i].n);
+            cap = std::max(cap, 64);
+            l.n = (cap + 63) / 64 * 64;
+            if (prev) l.Smax = std::max(l.Smax, prev->L[i].Smax);
+            l.Smin = 0;
+            l.groups.clear();
+            l.start.clear();
+            if (launch_lds(s, l, f64)) ok = false;
+            l.o_desc = o;
+            o += up((size_t)l.n * sizeof(bsf::RoiDesc));
+            l.o_dT = o;
+            o += l.has_dT ? up((size_t)l.n * 6 * l.nch * 4) : 0;
+            l.o_iz = o;
+            o += l.has_iz ? up((size_t)l.n * 4) : 0;
+            l.o_ip = o;
+            o += l.has_ip ? up((size_t)l.n * 4) : 0;
+        }
+        P->o_data = o;
+        size_t out = 0;
+        for (Launch &l : P->L)
+            out += up((size_t)l.n * (l.nv + 1) * 4) + up((size_t)l.n * l.nv * 4) + up((size_t)l.n * 4);
+        if (!ok || o + out + data_bytes > s->slot_bytes) {
+            delete P;
+            if (!ok) return nullptr;
+            continue;
+        }
+        P->o_out = s->slot_bytes - out;
+        P->data_cap = P->o_out - P->o_data;
+        size_t q = P->o_out;
+        for (Launch &l : P->L) {
+            l.o_P = q;
+            q += up((size_t)l.n * (l.nv + 1) * 4);
+            l.o_CR = q;
+            q += up((size_t)l.n * l.nv * 4);
+            l.o_LL = q;
+            q += up((size_t)l.n * 4);
+        }
+        return P;
+    }
+    return nullptr;
+}
+
+// Captures plan P's launches for slot `slot` (one branch per launch, all
+// reading their ROI counts from the slot header) and instantiates the graph.
+hipError_t capture(bsf_session *s, Plan &P, int slot)
+{
+    hipError_t e = hipSuccess;
+    const int nb = (int)std::min<size_t>(P.L.size(), kMaxKernelStreams);
+    for (int b = 0; b <= nb && e == hipSuccess; b++)
+        if (!s->s_cap[b]) e = hipStreamCreateWithFlags(&s->s_cap[b], hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    std::vector<hipEvent_t> ev(nb + 1, nullptr);
+    for (int b = 0; b <= nb && e == hipSuccess; b++) e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming);
+    hipGraph_t g = nullptr;
+    if (e == hipSuccess) e = hipStreamBeginCapture(s->s_cap[0], hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+        char *d = s->d_arena + (size_t)slot * s->slot_bytes;
+        const int *cnt = (const int *)(d + P.o_hdr);
+        hipError_t c = hipEventRecord(ev[0], s->s_cap[0]);
+        for (int b = 0; b < nb && c == hipSuccess; b++) c = hipStreamWaitEvent(s->s_cap[1 + b], ev[0], 0);
+        for (size_t i = 0; i < P.L.size() && c == hipSuccess; i++)
+            c = launch_one(s, P.L[i], P.f64, d, (const float *)d, cnt + i, s->s_cap[1 + i % nb]);
+        for (int b = 0; b < nb && c == hipSuccess; b++) {
+            c = hipEventRecord(ev[1 + b], s->s_cap[1 + b]);
+            if (c == hipSuccess) c = hipStreamWaitEvent(s->s_cap[0], ev[1 + b], 0);
+        }
+        e = hipStreamEndCapture(s->s_cap[0], &g);   // always ends the capture
+        if (c != hipSuccess) e = c;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&P.exec[slot], g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e == hipSuccess) s->n_capture++;
+    return e;
+}
+
+constexpr int kNoPlan = -1000;   // submit_graph: no plan fits, submit eagerly
+
+// One frame as a graph replay: find (or make) the plan of its launches, pack
+// its inputs at the plan's offsets of the next slot, one H2D copy, the slot's
+// graph on a kernel stream, one D2H copy of the output section.
+int submit_graph(bsf_session *s, const bsf_frame *f, int flags, unsigned long long *ticket, std::vector<Launch> &L,
+                 int f64)
+{
+    std::sort(L.begin(), L.end(), key_less);
+    size_t data_bytes = 0;
+    std::vector<size_t> nd(L.size());
+    for (size_t i = 0; i < L.size(); i++) {
+        nd[i] = launch_rows(L[i], f);
+        data_bytes += up(nd[i] * 4);
+    }
+    Plan *P = nullptr;
+    size_t pi = 0;
+    for (; pi < s->plans.size(); pi++) {
+        Plan *Q = s->plans[pi];
+        if (Q->f64 != f64 || Q->L.size() != L.size()) continue;
+        bool same = true;
+        for (size_t i = 0; i < L.size() && same; i++) same = same_key(Q->L[i], L[i]);
+        if (same) {
+            P = Q;
+            break;
+        }
+    }
+    bool fits = P && data_bytes <= P->data_cap;
+    for (size_t i = 0; i < L.size() && fits; i++) fits = L[i].n <= P->L[i].n && L[i].Smax <= P->L[i].Smax;
+    if (!fits) {
+        Plan *Q = make_plan(s, L, f64, P, data_bytes);
+        if (!Q) {
+            bsf::clear_error();   // the eager path reports its own errors
+            return kNoPlan;
+        }
+        if (P) {
+            drop_plan(s, P, false);   // its graphs may be in flight
+            s->plans[pi] = Q;
+        } else {
+            s->plans.push_back(Q);
+        }
+        P = Q;
+    }
+
+    size_t off;
+    int rc = admit(s, s->slot_bytes, flags, &off);
+    if (rc) return rc;
+    const int slot = (int)(off / s->slot_bytes);
+    if (!P->exec[slot]) {
+        hipError_t e = capture(s, *P, slot);
+        if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: graph capture failed: %s", hipGetErrorString(e));
+    }
+    char *h = s->h_arena + off, *d = s->d_arena + off;
+    size_t od = P->o_data;
+    for (size_t i = 0; i < L.size(); i++) {
+        ((int *)(h + P->o_hdr))[i] = L[i].n;
+        pack_launch(f, L[i], P->L[i], h, od, (long long)(od / 4));
+        od += up(nd[i] * 4);
+    }
+
+    Frame F;
+    F.off = off;
+    F.bytes = s->slot_bytes;
+    hipEvent_t ev_in = take_event(s), ev_k = take_event(s), ev_done = take_event(s);
+    for (hipEvent_t x : {ev_in, ev_k, ev_done})
+        if (x) F.events.push_back(x);
+    hipError_t e = F.events.size() == 3 ? hipSuccess : hipErrorOutOfMemory;
+    const hipStream_t gs = s->ks[slot % s->nks];
+    if (e == hipSuccess) e = hipMemcpyAsync(d, h, od, hipMemcpyHostToDevice, s->s_in);
+    if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
+    if (e == hipSuccess) e = hipStreamWaitEvent(gs, ev_in, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(P->exec[slot], gs);
+    if (e == hipSuccess) e = hipEventRecord(ev_k, gs);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s->s_out, ev_k, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h + P->o_out, d + P->o_out, s->slot_bytes - P->o_out, hipMemcpyDeviceToHost, s->s_out);
+    if (e == hipSuccess) e = hipEventRecord(ev_done, s->s_out);
+    if (e != hipSuccess) return abandon(s, F, e);
+    s->n_replay++;
+    for (size_t i = 0; i < L.size(); i++) add_outputs(F, f, L[i], P->L[i]);
+    F.ticket = s->next_ticket++;
+    *ticket = F.ticket;
+    s->inflight.push_back(std::move(F));
+    return BSF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -347,6 +924,9 @@ int bsf_session_destroy(bsf_session *s)
             if (m.d_coeff) (void)hipFree(m.d_coeff);
             if (m.d_coeff3) (void)hipFree(m.d_coeff3);
         }
+        drop_plans(s);
+        for (hipStream_t c : s->s_cap)
+            if (c) (void)hipStreamDestroy(c);
         for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
         if (s->s_in) (void)hipStreamDestroy(s->s_in);
         if (s->s_out) (void)hipStreamDestroy(s->s_out);
@@ -432,6 +1012,7 @@ int bsf_model_release(bsf_session *s, int model_id)
         return fail(BSF_ERR_ARG, "bsf: unknown model id %d", model_id);
     DeviceGuard g(s->device);
     int rc = drain(s);
+    drop_plans(s);   // their graphs launch with the model's coefficients
     Model &M = s->models[model_id];
     (void)hipFree(M.d_coeff);
     if (M.d_coeff3) (void)hipFree(M.d_coeff3);
@@ -449,6 +1030,43 @@ int bsf_session_set_precision(bsf_session *s, int precision)
         return fail(BSF_ERR_ARG, "bsf: unknown precision mode %d", precision);
     std::lock_guard<std::mutex> lk(s->mu);
     s->precision = precision;
+    return BSF_OK;
+}
+
+int bsf_session_set_graphs(bsf_session *s, int slots)
+{
+    bsf::clear_error();
+    if (!s) return fail(BSF_ERR_ARG, "bsf: NULL session");
+    if (slots < 0 || slots > kMaxGraphSlots)
+        return fail(BSF_ERR_ARG, "bsf: graph slots must be 0..%d (got %d)", kMaxGraphSlots, slots);
+    std::lock_guard<std::mutex> lk(s->mu);
+    DeviceGuard g(s->device);
+    int rc = drain(s);
+    drop_plans(s);
+    s->graphs = slots;
+    s->slot_bytes = slots ? s->arena / slots / kAlign * kAlign : 0;
+    return rc;
+}
+
+int bsf_session_set_hetero(bsf_session *s, int on)
+{
+    bsf::clear_error();
+    if (!s) return fail(BSF_ERR_ARG, "bsf: NULL session");
+    if (on != 0 && on != 1) return fail(BSF_ERR_ARG, "bsf: heterogeneous mode must be 0 or 1 (got %d)", on);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->hetero = on;
+    return BSF_OK;
+}
+
+int bsf_session_graph_stats(bsf_session *s, unsigned long long *captures, unsigned long long *replays,
+                            unsigned long long *eager)
+{
+    bsf::clear_error();
+    if (!s) return fail(BSF_ERR_ARG, "bsf: NULL session");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (captures) *captures = s->n_capture;
+    if (replays) *replays = s->n_replay;
+    if (eager) *eager = s->n_eager;
     return BSF_OK;
 }
 
@@ -516,10 +1134,7 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
         key.has_ip = M.ft == BSF_FT_4PI && g.initPhi;
         Launch *hit = nullptr;
         for (Launch &l : L)
-            if (l.model == key.model && l.iterations == key.iterations && l.has_initP == key.has_initP &&
-                l.nstarts == key.nstarts && l.strict == key.strict &&
-                std::memcmp(l.initP, key.initP, sizeof key.initP) == 0 && l.has_dT == key.has_dT &&
-                l.has_iz == key.has_iz && l.has_ip == key.has_ip) {
+            if (same_key(l, key)) {
                 hit = &l;
                 break;
             }
@@ -537,239 +1152,17 @@ int bsf_submit(bsf_session *s, const bsf_frame *f, int flags, unsigned long long
         hit->Smin = hit->Smin ? std::min(hit->Smin, g.sz) : g.sz;
     }
 
-    // ---- layout of the slot: inputs, then outputs
-    size_t o = 0;
-    for (Launch &l : L) {
-        const Model &M = s->models[l.model];
-        // 4Pi: one wave per ROI.  The wave-per-channel kernel halves one
-        // launch's latency on an idle GPU, but with frames overlapping it
-        // gained nothing measurable in the streamed mix (p50 2.17 vs 2.25 ms,
-        // p99 5.19 vs 4.88 ms) and saturated lower (3.15M vs 3.24M fits/s)
-        l.wpc = M.ft == BSF_FT_4PI && bsf::use_wpc_4pi(0, f64, l.Smax, M.nch, M.nv, s->lds_limit);
-        const size_t lds = M.ft == BSF_FT_4PI ? (l.wpc ? bsf::lds_bytes_4pi_wpc(f64, l.Smax, M.nch, M.nv)
-                                                       : bsf::lds_bytes_4pi(f64, l.Smax, M.nch, M.nv))
-                                              : bsf::lds_bytes(M.ft, f64, l.Smax);
-        if (lds > s->lds_limit)
-            return fail(BSF_ERR_ROI_SIZE, "bsf: ROI size %d of model %d needs %zu B of LDS (limit %zu B)", l.Smax,
-                        l.model, lds, s->lds_limit);
-        std::stable_sort(l.groups.begin(), l.groups.end(),
-                         [&](int a, int b) { return f->groups[a].sz > f->groups[b].sz; });
-        size_t nd = 0;
-        double cost = 0;
-        l.start.clear();
-        int row = 0;
-        for (int gi : l.groups) {
-            const bsf_roi_group &g = f->groups[gi];
-            l.start.push_back(row);
-            row += g.count;
-            nd += (size_t)g.count * g.sz * g.sz * l.nch;
-            cost += (double)g.count * g.sz * g.sz * l.nch * (double)(1 << (2 * (dim_of(M.ft) - 2)));
-        }
-        l.cost = cost * (l.iterations + 2) * l.nstarts;
-        l.o_desc = o;
-        o += up((size_t)l.n * sizeof(bsf::RoiDesc));
-        l.o_data = o;
-        o += up(nd * 4);
-        l.o_dT = o;
-        o += l.has_dT ? up((size_t)l.n * 6 * l.nch * 4) : 0;
-        l.o_iz = o;
-        o += l.has_iz ? up((size_t)l.n * 4) : 0;
-        l.o_ip = o;
-        o += l.has_ip ? up((size_t)l.n * 4) : 0;
-    }
-    const size_t in_bytes = o;
-    for (Launch &l : L) {
-        l.o_P = o;
-        o += up((size_t)l.n * (l.nv + 1) * 4);
-        l.o_CR = o;
-        o += up((size_t)l.n * l.nv * 4);
-        l.o_LL = o;
-        o += up((size_t)l.n * 4);
-    }
-    const size_t total = o;
-    if (total > s->arena)
-        return fail(BSF_ERR_NOMEM, "bsf: frame needs %zu B, more than the whole arena (%zu B)", total, s->arena);
     if (L.empty()) {   // nothing to fit: complete at once
         *ticket = s->next_ticket++;
         return BSF_OK;
     }
-
-    // ---- admission: room in the arena behind the frames in flight
     DeviceGuard dg(s->device);
-    size_t off;
-    while ((off = ring_alloc(s, total)) == SIZE_MAX) {
-        if (flags & BSF_SUBMIT_NONBLOCK)
-            return fail(BSF_BUSY, "bsf: arena full (%d frames in flight)", (int)s->inflight.size());
-        Frame &oldest = s->inflight.front();
-        if (oldest.delivered) {
-            retire(s);
-            continue;
-        }
-        int rc = finish(s, oldest);
-        if (rc) return rc;
+    if (s->graphs) {
+        int rc = submit_graph(s, f, flags, ticket, L, f64);
+        if (rc != kNoPlan) return rc;
+        s->n_eager++;   // no plan fits the slot: this frame goes the eager way, in a slot of its own
     }
-
-    // ---- pack the inputs into the pinned slot
-    char *h = s->h_arena + off, *d = s->d_arena + off;
-    for (Launch &l : L) {
-        bsf::RoiDesc *desc = (bsf::RoiDesc *)(h + l.o_desc);
-        float *dst = (float *)(h + l.o_data);
-        long long e0 = 0;
-        int row = 0;
-        for (size_t j = 0; j < l.groups.size(); j++) {
-            const bsf_roi_group &g = f->groups[l.groups[j]];
-            const size_t S2 = (size_t)g.sz * g.sz, cnt = (size_t)g.count;
-            const size_t cs = g.chan_stride ? g.chan_stride : cnt * S2;
-            for (int k = 0; k < l.nch; k++) std::memcpy(dst + e0 + k * cnt * S2, g.data + k * cs, cnt * S2 * 4);
-            for (size_t i = 0; i < cnt; i++) {
-                desc[row + i].off = e0 + (long long)(i * S2);
-                desc[row + i].cstride = (int)(cnt * S2);
-                desc[row + i].S = g.sz;
-            }
-            const int nch = l.nch;
-            if (l.has_dT) std::memcpy((float *)(h + l.o_dT) + (size_t)row * 6 * nch, g.dT, cnt * 6 * nch * 4);
-            if (l.has_iz) std::memcpy((float *)(h + l.o_iz) + row, g.initZ, cnt * 4);
-            if (l.has_ip) std::memcpy((float *)(h + l.o_ip) + row, g.initPhi, cnt * 4);
-            e0 += (long long)(cnt * S2 * l.nch);
-            row += g.count;
-        }
-    }
-
-    // ---- copies and launches
-    Frame F;
-    F.off = off;
-    F.bytes = total;
-    hipEvent_t ev_in = take_event(s);
-    if (!ev_in) return fail(BSF_ERR_DEVICE, "bsf: hipEventCreate failed");
-    F.events.push_back(ev_in);
-    hipError_t e = hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s->s_in);
-    if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
-    // largest launches first, each to the kernel stream with the least queued
-    // work (launches of earlier frames not yet delivered included), so the
-    // big launches of consecutive frames run side by side
-    std::vector<int> &ord = s->order;
-    ord.resize(L.size());
-    for (size_t i = 0; i < L.size(); i++) ord[i] = (int)i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return L[a].cost > L[b].cost; });
-    for (int i : ord) {
-        if (e != hipSuccess) break;
-        Launch &l = L[i];
-        int j = 0;
-        for (int q = 1; q < s->nks; q++)
-            if (s->load[q] < s->load[j]) j = q;
-        s->load[j] += l.cost;
-        F.work.emplace_back(j, l.cost);
-        l.stream = j;
-        hipStream_t ks = s->ks[j];
-        const Model &M = s->models[l.model];
-        e = hipStreamWaitEvent(ks, ev_in, 0);
-        if (e != hipSuccess) break;
-        const bsf::RoiDesc *d_desc = (const bsf::RoiDesc *)(d + l.o_desc);
-        const float *d_data = (const float *)(d + l.o_data);
-        float *dP = (float *)(d + l.o_P), *dCR = (float *)(d + l.o_CR), *dLL = (float *)(d + l.o_LL);
-        if (M.ft == BSF_FT_4PI) {
-            bsf::Fit4PiLaunch K;
-            std::memset(&K, 0, sizeof K);
-            K.f64 = f64;
-            K.data = d_data;
-            K.coeff = M.d_coeff;
-            K.P = dP;
-            K.CRLB = dCR;
-            K.LL = dLL;
-            K.N = l.n;
-            K.S = l.Smax;
-            K.iterations = l.iterations;
-            K.nch = M.nch;
-            K.roi_offset = 0;
-            K.count = l.n;
-            K.rois = d_desc;
-            for (int a = 0; a < 4; a++) K.cdims[a] = M.cdims[a];
-            for (int p = 0; p < 6; p++) K.shared[p] = M.shared[p];
-            for (int k = 0; k < 8; k++) K.phaseShift[k] = M.shift[k];
-            K.dT = l.has_dT ? (const float *)(d + l.o_dT) : nullptr;
-            K.initZ = l.has_iz ? (const float *)(d + l.o_iz) : nullptr;
-            K.initPhi = l.has_ip ? (const float *)(d + l.o_ip) : nullptr;
-            K.wpc = l.wpc;
-            e = bsf::launch_fit_4pi(K, ks);
-        } else {
-            bsf::FitLaunch K;
-            std::memset(&K, 0, sizeof K);
-            K.fittype = M.ft;
-            K.f64 = f64;
-            K.data = d_data;
-            K.coeff = M.d_coeff;
-            K.coeff3 = M.d_coeff3;
-            K.P = dP;
-            K.CRLB = dCR;
-            K.LL = dLL;
-            K.N = l.n;
-            K.S = l.Smax;
-            K.iterations = l.iterations;
-            K.roi_offset = 0;
-            K.count = l.n;
-            K.rois = d_desc;
-            K.uniform = l.Smin == l.Smax;
-            K.ndim = M.ndim;
-            for (int a = 0; a < 5; a++) K.cdims[a] = M.cdims[a];
-            K.has_initP = l.has_initP;
-            // multi-start: one launch per start on this stream; after the
-            // first, each keeps the results whose log-likelihood it beats
-            for (int k = 0; k < l.nstarts && e == hipSuccess; k++) {
-                for (int q = 0; q < 4; q++) K.initP[q] = l.initP[k][q];
-                K.keep = k == 0 ? 0 : (l.strict ? 2 : 1);
-                e = bsf::launch_fit(K, ks);
-            }
-        }
-        if (e != hipSuccess) break;
-        hipEvent_t ev = take_event(s);
-        if (!ev) {
-            e = hipErrorOutOfMemory;
-            break;
-        }
-        F.events.push_back(ev);
-        e = hipEventRecord(ev, ks);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s->s_out, ev, 0);
-    }
-    hipEvent_t ev_done = nullptr;
-    if (e == hipSuccess) {
-        ev_done = take_event(s);
-        if (!ev_done) e = hipErrorOutOfMemory;
-    }
-    if (e == hipSuccess) {
-        F.events.push_back(ev_done);
-        e = hipMemcpyAsync(h + in_bytes, d + in_bytes, total - in_bytes, hipMemcpyDeviceToHost, s->s_out);
-        if (e == hipSuccess) e = hipEventRecord(ev_done, s->s_out);
-    }
-    if (e != hipSuccess) {
-        // work already queued may still read the slot: wait for it before
-        // the slot can be handed out again
-        (void)hipStreamSynchronize(s->s_in);
-        for (int j = 0; j < s->nks; j++) (void)hipStreamSynchronize(s->ks[j]);
-        (void)hipStreamSynchronize(s->s_out);
-        for (hipEvent_t ev : F.events) s->event_pool.push_back(ev);
-        for (const auto &w : F.work) s->load[w.first] -= w.second;
-        return fail(BSF_ERR_DEVICE, "bsf: frame submission failed: %s", hipGetErrorString(e));
-    }
-    for (Launch &l : L)
-        for (size_t j = 0; j < l.groups.size(); j++) {
-            const bsf_roi_group &g = f->groups[l.groups[j]];
-            GroupOut go;
-            go.P = g.P;
-            go.CRLB = g.CRLB;
-            go.LL = g.LL;
-            go.count = g.count;
-            go.nv = l.nv;
-            go.start = l.start[j];
-            go.launch_n = l.n;
-            go.o_P = l.o_P;
-            go.o_CR = l.o_CR;
-            go.o_LL = l.o_LL;
-            F.outs.push_back(go);
-        }
-    F.ticket = s->next_ticket++;
-    *ticket = F.ticket;
-    s->inflight.push_back(std::move(F));
-    return BSF_OK;
+    return submit_eager(s, f, flags, ticket, L, f64);
 }
 
 int bsf_poll(bsf_session *s, unsigned long long ticket, int wait)

@@ -429,6 +429,37 @@ int bsf_poll(bsf_session *s, unsigned long long ticket, int wait);
 /* Occupancy of the session: frames in flight and arena bytes they hold. */
 int bsf_session_inflight(bsf_session *s, int *frames, size_t *arena_used);
 
+/* Graph mode: slots = 1..64 cuts the arena into that many equal slots, each
+ * holding one frame in flight (32 measured best for the streamed mix);
+ * 0 returns to per-call launches, the default.  Waits for the frames in
+ * flight.  A frame whose launches match a captured plan -- the same models,
+ * iterations, starts and optional inputs, with ROI counts and sizes within
+ * the plan's capacities, which are twice the counts of the frame that made
+ * it -- is submitted as one host-to-device copy, one hipGraphLaunch of the
+ * plan's graph for its slot (its launches read their ROI counts from device
+ * memory) and one device-to-host copy.  A frame no plan fits is captured
+ * anew; one that a slot cannot hold is launched call by call in as many
+ * slots as it needs (counted in `eager`).  Results are the same, bit for
+ * bit, in both modes.  Measured on the streamed mix, per-call launches
+ * sustain more frames and have the lower latency, hence the default.  No
+ * reference counterpart (the reference launches per MEX call,
+ * GPUmleFit_LM/mexFunction.cpp:596-601). */
+int bsf_session_set_graphs(bsf_session *s, int slots);
+/* Heterogeneous launches (on = 1; 0, the default, launches per model):
+ * a frame's single-start groups of fittypes 6, 7 and 8 -- up to four
+ * launches' worth, across models -- run as one launch in which every
+ * workgroup runs its ROI's fittype (4Pi and multi-start groups launch as
+ * before).  Results are the same, bit for bit.  That kernel holds every
+ * fittype's code and takes fittype 8's register footprint for all of them,
+ * so 3D and colour-4D ROIs run at half their own kernels' occupancy; measured
+ * on the streamed mix it is slower, hence the default.  Applies to frames
+ * launched call by call (not to graph replays).  No reference counterpart. */
+int bsf_session_set_hetero(bsf_session *s, int on);
+/* Graph mode counters: graphs captured, frames replayed, frames launched
+ * call by call in graph mode (each may be NULL). */
+int bsf_session_graph_stats(bsf_session *s, unsigned long long *captures, unsigned long long *replays,
+                            unsigned long long *eager);
+
 /* ------------------------------------------------ frames -> ROI stack
  * The GUI fitter's per-frame front-end (MultiD_fitter/simplefitter_spline.m:
  * 197-307 with MultiD_fitter/shared/maximumfindcall.m:4-21 and

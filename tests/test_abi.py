@@ -242,3 +242,12 @@ def test_tile_table_without_device():
     x = Extras(None, None, None, 3)
     assert lib.bsf_fit_device_ex(C.byref(req), C.byref(res), C.byref(x), None) == 1
     assert "tile" in lib.bsf_last_error().decode()
+
+
+def test_session_graph_mode_argument_errors():
+    import bsfit
+    from bsfit.session import _bind
+    lib = _bind(bsfit.load())
+    assert lib.bsf_session_set_graphs(None, 32) == 1
+    assert lib.bsf_session_graph_stats(None, None, None, None) == 1
+    assert lib.bsf_session_set_hetero(None, 1) == 1

@@ -186,6 +186,10 @@ def test_session_argument_errors(pools):
         with pytest.raises(bsfit.BsfError) as e:
             s.set_precision(3)
         assert e.value.code == 1
+        for bad in (-1, 65):
+            with pytest.raises(bsfit.BsfError) as e:
+                s.set_graphs(bad)
+            assert e.value.code == 1
         with pytest.raises(bsfit.BsfError) as e:        # several starts: fittypes 7 and 8 only
             s.submit([bsfit.Group(m, w.imstack, 10, starts=np.zeros((2, 2), np.float32))])
         assert e.value.code == 1
