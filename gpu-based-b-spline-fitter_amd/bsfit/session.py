@@ -212,7 +212,7 @@ class Session:
     def set_precision(self, precision):
         _check(self._lib.bsf_session_set_precision(self._h, int(precision)))
 
-    def set_graphs(self, slots=32):
+    def set_graphs(self, slots=16):
         """Graph mode (bsf_session_set_graphs) with `slots` arena slots:
         frames replay captured graphs of their launches; 0 (or False) returns
         to per-call launches.  Waits for the frames in flight."""

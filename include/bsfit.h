@@ -430,7 +430,7 @@ int bsf_poll(bsf_session *s, unsigned long long ticket, int wait);
 int bsf_session_inflight(bsf_session *s, int *frames, size_t *arena_used);
 
 /* Graph mode: slots = 1..64 cuts the arena into that many equal slots, each
- * holding one frame in flight (32 measured best for the streamed mix);
+ * holding one frame in flight (16 measured best for the streamed mix);
  * 0 returns to per-call launches, the default.  Waits for the frames in
  * flight.  A frame whose launches match a captured plan -- the same models,
  * iterations, starts and optional inputs, with ROI counts and sizes within
