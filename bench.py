@@ -36,7 +36,9 @@ for _p in (ROOT, PKG):
 FP32_PEAK = 157.3e12     # MI355X FP32 vector, FLOP/s
 METRIC = "ROI fits/sec per fittype (FP32+FP64-solve vs FP64); p50/p99 frame latency"
 TRAFFIC_FILES = {"vortex5d": os.path.join(ROOT, "profiles", "fit_kernel_vortex5d_mixed_traffic.md"),
-                 "colour4d": os.path.join(ROOT, "profiles", "fit_kernel_colour4d_traffic.md")}
+                 "colour4d": os.path.join(ROOT, "profiles", "fit_kernel_colour4d_traffic.md"),
+                 "3d": os.path.join(ROOT, "profiles", "fit_kernel_3d_traffic.md"),
+                 "4pi": os.path.join(ROOT, "profiles", "fit4pi_kernel_traffic.md")}
 TRAFFIC_FILE = TRAFFIC_FILES["colour4d"]
 
 
