@@ -46,6 +46,7 @@ class RoiRequest(C.Structure):
         ("mirror", C.c_int),
         ("psf_size", C.c_double),
         ("capacity", C.c_int),
+        ("agg_dist", C.c_double),
     ]
 
 
@@ -109,7 +110,7 @@ def peak_filter_psf(kind, psf, roi):
 
 
 def make_request(frames_ptr, pixel_type, shape, roi, filt_ptr, filt_shape, cutoff, offset, conversion,
-                 subtract_min, mirror, psf_size, first_frame, capacity):
+                 subtract_min, mirror, psf_size, first_frame, capacity, agg_dist=0.0):
     q = RoiRequest()
     q.frames = frames_ptr
     q.pixel_type = pixel_type
@@ -125,14 +126,16 @@ def make_request(frames_ptr, pixel_type, shape, roi, filt_ptr, filt_shape, cutof
     q.mirror = int(bool(mirror))
     q.psf_size = float(psf_size)
     q.capacity = int(capacity)
+    q.agg_dist = float(agg_dist)
     return q
 
 
 def find_rois(frames, roi, filt, cutoff, offset=0.0, conversion=1.0, subtract_min=False, mirror=False,
-              psf_size=0.0, first_frame=1, capacity=None, with_value=True):
+              psf_size=0.0, first_frame=1, capacity=None, with_value=True, agg_dist=0.0):
     """Run bsf_find_rois_host.  Returns (imstack (roi, roi, n) float32 F-order,
     peaks (n, 3) int32 [x, y, frame], value (n,) float32 or None, count);
-    n = min(count, capacity)."""
+    n = min(count, capacity).  agg_dist > 0 merges each frame's maxima closer
+    than it, as the GUI's p.agg (agg_dist = p.roifit * p.aggsz)."""
     lib = _bind()
     if frames.dtype == np.uint16:
         pt = PIXEL_U16
@@ -155,7 +158,7 @@ def find_rois(frames, roi, filt, cutoff, offset=0.0, conversion=1.0, subtract_mi
     peaks = np.zeros((ncap, 3), np.int32, order="F")
     value = np.zeros(ncap, np.float32) if with_value else None
     q = make_request(frames.ctypes.data, pt, frames.shape, roi, filt.ctypes.data, filt.shape, cutoff, offset,
-                     conversion, subtract_min, mirror, psf_size, first_frame, capacity)
+                     conversion, subtract_min, mirror, psf_size, first_frame, capacity, agg_dist)
     res = RoiResult(stack.ctypes.data, peaks.ctypes.data, value.ctypes.data if value is not None else None, 0)
     _check(lib.bsf_find_rois_host(C.byref(q), C.byref(res)))
     n = min(res.count, ncap)
@@ -164,7 +167,7 @@ def find_rois(frames, roi, filt, cutoff, offset=0.0, conversion=1.0, subtract_mi
 
 def fit_frames(frames, roi, filt, cutoff, fittype, iterations, coeff, offset=0.0, conversion=1.0,
                subtract_min=False, mirror=False, psf_size=0.0, first_frame=1, capacity=None, initP=None,
-               coeff3=None, precision=0):
+               coeff3=None, precision=0, agg_dist=0.0):
     """Run bsf_fit_frames_host: the ROIs of find_rois and their fits in one
     call.  Returns (imstack, peaks, value, P, CRLB, LL, count) with
     n = min(count, capacity) ROIs; P (n, NV+1) and CRLB (n, NV) as fit()."""
@@ -190,7 +193,7 @@ def fit_frames(frames, roi, filt, cutoff, fittype, iterations, coeff, offset=0.0
     c3 = _f32(coeff3) if coeff3 is not None else None
     ip = np.ascontiguousarray(np.ravel(initP), np.float32) if initP is not None else None
     q = make_request(frames.ctypes.data, pt, frames.shape, roi, filt.ctypes.data, filt.shape, cutoff, offset,
-                     conversion, subtract_min, mirror, psf_size, first_frame, capacity)
+                     conversion, subtract_min, mirror, psf_size, first_frame, capacity, agg_dist)
     f = _request(fittype, roi, ncap, iterations, coeff.ctypes.data, coeff.shape,
                  c3.ctypes.data if c3 is not None else None, ip, precision, None)
     r = RoiResult(stack.ctypes.data, peaks.ctypes.data, value.ctypes.data, 0)

@@ -27,6 +27,10 @@
 //                      else of the frame is written.
 //   count/scan/emit    order-preserving compaction of the mask words into
 //                      (x, y, frame) rows.
+//   aggregate_kernel   with aggregation (the GUI's p.agg): one workgroup per
+//                      frame merges the frame's maxima as aggre does and
+//                      applies the PSF-convolution border test after it;
+//                      gather_kernel then packs the frames' survivors.
 //   cut_kernel         one wave per ROI (grid-stride over the device count):
 //                      the window re-read from the raw frame and converted,
 //                      the filtered value recomputed in the same order.
@@ -376,6 +380,30 @@ struct CutArgs {
     const float *sub;
 };
 
+// The filtered value at pixel (row y, column x) of frame fr: the detect
+// kernel's sum for that pixel, term for term.
+template <typename T>
+__device__ float filtered_at(const T *fr, int nr, int nc, float offset, float conv, const float *h, int mr, int mc,
+                             const float *subp, int x, int y)
+{
+    const int ar = (mr - 1) / 2, ac = (mc - 1) / 2;
+    const float sub = subp ? *subp : 0.0f;
+    float acc = 0.0f;
+    for (int v = 0; v < mc; v++) {
+        const int gc = x - ac + v;
+        for (int u = 0; u < mr; u++) {
+            const int gr = y - ar + u;
+            float xv = 0.0f;
+            if (gc >= 0 && gc < nc && gr >= 0 && gr < nr) {
+                xv = photon(fr[(size_t)gc * nr + gr], offset, conv);
+                if (subp) xv = xv - sub;
+            }
+            acc = fmaf(h[v * mr + u], xv, acc);
+        }
+    }
+    return acc;
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void cut_kernel(CutArgs A)
 {
@@ -390,25 +418,176 @@ __global__ __launch_bounds__(64) void cut_kernel(CutArgs A)
             const int gc = A.mirror ? x + A.dx - c : x - A.dx + c;
             dst[e] = photon(fr[(size_t)gc * A.nr + (y - A.dx + r)], A.offset, A.conv);
         }
-        if (A.value && threadIdx.x == 0) {
-            // the detect kernel's sum for this pixel, term for term
-            const int ar = (A.mr - 1) / 2, ac = (A.mc - 1) / 2;
-            const float sub = A.sub ? A.sub[f] : 0.0f;
-            float acc = 0.0f;
-            for (int v = 0; v < A.mc; v++) {
-                const int gc = x - ac + v;
-                for (int u = 0; u < A.mr; u++) {
-                    const int gr = y - ar + u;
-                    float xv = 0.0f;
-                    if (gc >= 0 && gc < A.nc && gr >= 0 && gr < A.nr) {
-                        xv = photon(fr[(size_t)gc * A.nr + gr], A.offset, A.conv);
-                        if (A.sub) xv = xv - sub;
-                    }
-                    acc = fmaf(A.h[v * A.mr + u], xv, acc);
+        if (A.value && threadIdx.x == 0)
+            A.value[k] = filtered_at<T>(fr, A.nr, A.nc, A.offset, A.conv, A.h, A.mr, A.mc, A.sub ? A.sub + f : nullptr,
+                                        x, y);
+    }
+}
+
+// ---------------------------------------------------------- aggregation
+// aggre (MultiD_fitter/simplefitter_spline.m:281-283, 653-673) on each
+// frame's maxima [x, y, value] in scan order: while two maxima ii < jj lie
+// at a distance D with 0 < D < agg_dist, the first such pair in MATLAB's
+// column-major find order (smallest jj, then smallest ii) is replaced by
+// round(mean) of the two rows (x, y and value; round half away from zero) in
+// row ii, and row jj is deleted.  D == 0 never merges (the reference sets
+// those entries to NaN).  Coordinates are integers, so D < agg_dist is
+// d2 = dx^2 + dy^2 <= d2max with d2max the largest integer whose correctly
+// rounded square root is below agg_dist (agg_d2max, on the host).  After a
+// merge into ii the first close pair can only lie in column ii or later (the
+// pairs of earlier columns are unchanged), so the search resumes there.
+struct AggArgs {
+    const void *raw;
+    int nr, nc;
+    float offset, conv;
+    const float *h;
+    int mr, mc;
+    const float *sub;
+    int *cand;            // [3, ccap]: x, y, frame number of the maxima, frames in order
+    double *cval;         // [ccap] values (when requested), or NULL
+    const int *ccount;    // maxima found (at most ccap)
+    int ccap, first_frame;
+    long long d2max;
+    double psf_half;      // > 0: the PSF-convolution mode's border test, after the merge
+    int *fkept, *foff;    // per frame: maxima kept (both; foff is then scanned to offsets)
+    int *fseg;            // per frame: first index in cand
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void aggregate_kernel(AggArgs A)
+{
+    const int f = blockIdx.x, tid = threadIdx.x;
+    const int total = min(*A.ccount, A.ccap);
+    // the frame's run of maxima (the frame column is non-decreasing)
+    const int *fr = A.cand + 2 * (size_t)A.ccap;
+    const int key = A.first_frame + f;
+    int lo = 0, hi = total;
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (fr[m] < key) lo = m + 1;
+        else hi = m;
+    }
+    int end = lo, top = total;
+    while (end < top) {
+        const int m = (end + top) >> 1;
+        if (fr[m] <= key) end = m + 1;
+        else top = m;
+    }
+    int *xs = A.cand + lo, *ys = A.cand + A.ccap + lo;
+    double *vs = A.cval ? A.cval + lo : nullptr;
+    int n = end - lo;
+    if (vs) {
+        const T *frm = (const T *)A.raw + (size_t)f * A.nr * A.nc;
+        for (int k = tid; k < n; k += kThreads)
+            vs[k] = (double)filtered_at<T>(frm, A.nr, A.nc, A.offset, A.conv, A.h, A.mr, A.mc,
+                                           A.sub ? A.sub + f : nullptr, xs[k] - 1, ys[k] - 1);
+    }
+    __shared__ unsigned long long best;
+    int start = 1;
+    for (;;) {
+        if (tid == 0) best = ~0ull;
+        __syncthreads();
+        // each thread takes columns jj = start + tid, + 256, ... and stops at
+        // its first close pair (or once a smaller column has one)
+        for (int jj = start + tid; jj < n; jj += kThreads) {
+            if (((unsigned long long)jj << 32) > *(volatile unsigned long long *)&best) break;
+            const int xj = xs[jj], yj = ys[jj];
+            int hit = -1;
+            for (int ii = 0; ii < jj; ii++) {
+                const long long dxv = xs[ii] - xj, dyv = ys[ii] - yj;
+                const long long d2 = dxv * dxv + dyv * dyv;
+                if (d2 > 0 && d2 <= A.d2max) {
+                    hit = ii;
+                    break;
                 }
             }
-            A.value[k] = acc;
+            if (hit >= 0) {
+                atomicMin(&best, ((unsigned long long)jj << 32) | (unsigned)hit);
+                break;
+            }
         }
+        __syncthreads();
+        const unsigned long long b = best;
+        if (b == ~0ull) break;
+        const int jj = (int)(b >> 32), ii = (int)(b & 0xffffffffu);
+        if (tid == 0) {
+            xs[ii] = (int)round(((double)xs[ii] + (double)xs[jj]) / 2.0);
+            ys[ii] = (int)round(((double)ys[ii] + (double)ys[jj]) / 2.0);
+            if (vs) vs[ii] = round((vs[ii] + vs[jj]) / 2.0);
+        }
+        // delete row jj: rows jj+1.. move down by one, read then written per chunk
+        for (int k0 = jj; k0 < n - 1; k0 += kThreads) {
+            const int k = k0 + tid;
+            int tx = 0, ty = 0;
+            double tv = 0.0;
+            if (k < n - 1) {
+                tx = xs[k + 1];
+                ty = ys[k + 1];
+                if (vs) tv = vs[k + 1];
+            }
+            __syncthreads();
+            if (k < n - 1) {
+                xs[k] = tx;
+                ys[k] = ty;
+                if (vs) vs[k] = tv;
+            }
+        }
+        __syncthreads();
+        n--;
+        start = ii > 1 ? ii : 1;
+    }
+    // the PSF-convolution mode's border test (simplefitter_spline.m:284-295,
+    // rows used for both axes) on the merged maxima, kept in order in place
+    int kept = n;
+    if (A.psf_half > 0.0) {
+        const double hp = A.psf_half, lim = (double)A.nr - hp;
+        int base = 0;
+        for (int k0 = 0; k0 < n; k0 += kThreads) {
+            const int k = k0 + tid;
+            int tx = 0, ty = 0;
+            double tv = 0.0;
+            bool keep = false;
+            if (k < n) {
+                tx = xs[k];
+                ty = ys[k];
+                if (vs) tv = vs[k];
+                keep = !((double)tx < hp || (double)tx > lim || (double)ty < hp || (double)ty > lim);
+            }
+            int cnt;
+            const int pos = base + block_exclusive_scan(keep ? 1 : 0, &cnt);
+            if (keep) {
+                xs[pos] = tx;
+                ys[pos] = ty;
+                if (vs) vs[pos] = tv;
+            }
+            base += cnt;
+            __syncthreads();
+        }
+        kept = base;
+    }
+    if (tid == 0) {
+        A.fkept[f] = kept;
+        A.foff[f] = kept;
+        A.fseg[f] = lo;
+    }
+}
+
+// The frames' kept maxima, in frame order, into the outputs: peaks [cap, 3]
+// and value (float) at the frame's offset (fcount scanned to offsets).
+__global__ __launch_bounds__(kThreads) void gather_kernel(const int *__restrict__ cand, const double *__restrict__ cval,
+                                                          int ccap, const int *__restrict__ fkept,
+                                                          const int *__restrict__ foff, const int *__restrict__ fseg,
+                                                          int cap, int *__restrict__ peaks, float *__restrict__ value)
+{
+    const int f = blockIdx.x;
+    const int n = fkept[f], off = foff[f], lo = fseg[f];
+    for (int k = threadIdx.x; k < n; k += kThreads) {
+        const int o = off + k;
+        if (o >= cap) break;
+        peaks[o] = cand[lo + k];
+        peaks[cap + o] = cand[ccap + lo + k];
+        peaks[2 * cap + o] = cand[2 * (size_t)ccap + lo + k];
+        if (value) value[o] = (float)cval[lo + k];
     }
 }
 
@@ -416,13 +595,17 @@ __global__ void zero_count_kernel(int *count) { *count = 0; }
 
 struct WorkLayout {
     size_t sub, mask, bsum, total;
+    // with aggregation: every maximum of every frame (ccap: the most a frame
+    // of nr x nc can hold, strict maxima being never adjacent), their values,
+    // their count and the per-frame kept counts, offsets and first indices
+    size_t cand, cval, ccount, fkept, foff, fseg;
     long W;
-    int nb, wpc;
+    int nb, wpc, ccap;
 };
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-WorkLayout work_layout(int nr, int nc, int nframes)
+WorkLayout work_layout(int nr, int nc, int nframes, bool agg)
 {
     WorkLayout w;
     w.wpc = (nr + kTR - 1) / kTR;
@@ -432,7 +615,35 @@ WorkLayout work_layout(int nr, int nc, int nframes)
     w.mask = align256(sizeof(float) * (size_t)nframes);
     w.bsum = w.mask + align256(sizeof(uint64_t) * (size_t)w.W);
     w.total = w.bsum + align256(sizeof(int) * (size_t)(w.nb > 0 ? w.nb : 1));
+    w.ccap = 0;
+    w.cand = w.cval = w.ccount = w.fkept = w.foff = w.fseg = w.total;
+    if (agg) {
+        const size_t per = (size_t)((nr - 1) / 2) * (size_t)((nc - 1) / 2);
+        const size_t cc = per * (size_t)(nframes > 0 ? nframes : 0);
+        w.ccap = (int)cc;   // below 2^31 (the entry points check the bound)
+        const size_t nf = (size_t)(nframes > 0 ? nframes : 1);
+        w.cand = w.total;
+        w.cval = w.cand + align256(sizeof(int) * 3 * (cc ? cc : 1));
+        w.ccount = w.cval + align256(sizeof(double) * (cc ? cc : 1));
+        w.fkept = w.ccount + 256;
+        w.foff = w.fkept + align256(sizeof(int) * nf);
+        w.fseg = w.foff + align256(sizeof(int) * nf);
+        w.total = w.fseg + align256(sizeof(int) * nf);
+    }
     return w;
+}
+
+// The largest integer d2 with sqrt(d2) < thr (correctly rounded, as MATLAB
+// computes the distance of two integer points), or -1 when thr <= 0; capped
+// above the largest squared distance within an nr x nc frame.
+long long agg_d2max(double thr, int nr, int nc)
+{
+    if (!(thr > 0.0)) return -1;
+    const double lim = (double)nr * nr + (double)nc * nc;
+    const double t2 = thr * thr;
+    long long k = t2 > lim ? (long long)lim + 1 : (long long)t2 + 2;
+    while (k > 0 && !(std::sqrt((double)k) < thr)) k--;
+    return k;
 }
 
 using DetectFn = void (*)(DetectArgs);
@@ -448,7 +659,8 @@ DetectFn detect_fn(int mr, std::index_sequence<I...>)
 template <typename T>
 hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
 {
-    const WorkLayout w = work_layout(L.nr, L.nc, L.nframes);
+    const bool agg = L.agg_dist > 0.0;
+    const WorkLayout w = work_layout(L.nr, L.nc, L.nframes, agg);
     char *work = (char *)L.work;
     float *sub = L.subtract_min ? (float *)(work + w.sub) : nullptr;
     uint64_t *mask = (uint64_t *)(work + w.mask);
@@ -473,7 +685,9 @@ hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
     if ((double)cf > L.cutoff) cf = std::nextafter(cf, -HUGE_VALF);
     A.cutoff_f = cf;
     A.dx = L.S / 2;
-    A.psf_half = L.psf_size > 0.0 ? L.psf_size / 2.0 : 0.0;
+    // with aggregation the PSF border test follows the merge (aggregate_kernel)
+    const double psf_half = L.psf_size > 0.0 ? L.psf_size / 2.0 : 0.0;
+    A.psf_half = agg ? 0.0 : psf_half;
     A.mask = mask;
     A.wpc = w.wpc;
     const size_t lds = front_detect_lds_bytes(L.mr, L.mc);
@@ -490,9 +704,43 @@ hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(count_kernel, dim3((unsigned)w.nb), dim3(kThreads), 0, st, mask, w.W, bsum);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kThreads), 0, st, bsum, w.nb, L.count);
-    hipLaunchKernelGGL(emit_kernel, dim3((unsigned)w.nb), dim3(kThreads), 0, st, mask, w.W, bsum, w.wpc, L.nc,
-                       L.first_frame, L.cap, L.peaks);
+    if (!agg) {
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kThreads), 0, st, bsum, w.nb, L.count);
+        hipLaunchKernelGGL(emit_kernel, dim3((unsigned)w.nb), dim3(kThreads), 0, st, mask, w.W, bsum, w.wpc, L.nc,
+                           L.first_frame, L.cap, L.peaks);
+    } else {
+        // every maximum into the candidate list, merged per frame, packed
+        int *cand = (int *)(work + w.cand), *ccount = (int *)(work + w.ccount);
+        int *fkept = (int *)(work + w.fkept), *foff = (int *)(work + w.foff), *fseg = (int *)(work + w.fseg);
+        double *cval = L.value ? (double *)(work + w.cval) : nullptr;
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kThreads), 0, st, bsum, w.nb, ccount);
+        hipLaunchKernelGGL(emit_kernel, dim3((unsigned)w.nb), dim3(kThreads), 0, st, mask, w.W, bsum, w.wpc, L.nc,
+                           L.first_frame, w.ccap, cand);
+        AggArgs G;
+        G.raw = L.frames;
+        G.nr = L.nr;
+        G.nc = L.nc;
+        G.offset = L.offset;
+        G.conv = L.conv;
+        G.h = L.h;
+        G.mr = L.mr;
+        G.mc = L.mc;
+        G.sub = sub;
+        G.cand = cand;
+        G.cval = cval;
+        G.ccount = ccount;
+        G.ccap = w.ccap;
+        G.first_frame = L.first_frame;
+        G.d2max = agg_d2max(L.agg_dist, L.nr, L.nc);
+        G.psf_half = psf_half;
+        G.fkept = fkept;
+        G.foff = foff;
+        G.fseg = fseg;
+        hipLaunchKernelGGL(aggregate_kernel<T>, dim3((unsigned)L.nframes), dim3(kThreads), 0, st, G);
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kThreads), 0, st, foff, L.nframes, L.count);
+        hipLaunchKernelGGL(gather_kernel, dim3((unsigned)L.nframes), dim3(kThreads), 0, st, cand, cval, w.ccap, fkept,
+                           foff, fseg, L.cap, L.peaks, L.value);
+    }
     if (L.cap > 0) {
         CutArgs C;
         C.raw = L.frames;
@@ -508,7 +756,7 @@ hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
         C.dx = L.S / 2;
         C.mirror = L.mirror;
         C.stack = L.stack;
-        C.value = L.value;
+        C.value = agg ? nullptr : L.value;   // with aggregation the gather wrote the values
         C.h = L.h;
         C.mr = L.mr;
         C.mc = L.mc;
@@ -527,7 +775,7 @@ size_t front_detect_lds_bytes(int mr, int mc)
     return sizeof(float) * ((size_t)XC * (XR | 1) + (size_t)kFC * kFLD);
 }
 
-size_t front_work_bytes(int nr, int nc, int nframes) { return work_layout(nr, nc, nframes).total; }
+size_t front_work_bytes(int nr, int nc, int nframes, bool agg) { return work_layout(nr, nc, nframes, agg).total; }
 
 // Workspace of the allocation-free entry point (bsf_find_rois_device_noalloc):
 // one array per GPU for the life of the process.  Nothing in it needs to

@@ -46,11 +46,19 @@ int validate_rois(const bsf_roi_request *r, const bsf_roi_result *res)
         return fail(BSF_ERR_ARG, "bsf: NULL imstack or peaks output");
     if (!std::isfinite(r->offset) || !std::isfinite(r->conversion))
         return fail(BSF_ERR_ARG, "bsf: offset and conversion must be finite");
+    if (!(r->agg_dist >= 0.0) || !std::isfinite(r->agg_dist))
+        return fail(BSF_ERR_ARG, "bsf: agg_dist must be finite and >= 0 (0: no aggregation)");
     // strict maxima are never adjacent: at most ceil(nr/2) * ceil(nc/2) per frame
     const double most = (double)r->nframes * ((r->nrows + 1) / 2) * ((r->ncols + 1) / 2);
     if (most > (double)INT_MAX)
         return fail(BSF_ERR_ARG, "bsf: too many frames in one call (maxima could exceed 2^31)");
     return BSF_OK;
+}
+
+// workspace of a request (aggregation holds every maximum of the frames)
+size_t work_bytes_of(const bsf_roi_request *r)
+{
+    return bsf::front_work_bytes(r->nrows, r->ncols, r->nframes, r->agg_dist > 0.0);
 }
 
 bsf::FrontLaunch make_front(const bsf_roi_request *r, const void *frames, const float *h, float *stack, int *peaks,
@@ -74,6 +82,7 @@ bsf::FrontLaunch make_front(const bsf_roi_request *r, const void *frames, const 
     L.S = r->roi;
     L.mirror = r->mirror != 0;
     L.psf_size = r->psf_size;
+    L.agg_dist = r->agg_dist;
     L.cap = r->capacity;
     L.stack = stack;
     L.peaks = peaks;
@@ -198,7 +207,7 @@ int bsf_peak_filter_psf(int kind, const float *psf, const int *psf_dims, int roi
 size_t bsf_find_rois_work_bytes(const bsf_roi_request *r)
 {
     if (!r || r->nrows < 1 || r->ncols < 1 || r->nframes < 0) return 0;
-    return bsf::front_work_bytes(r->nrows, r->ncols, r->nframes);
+    return work_bytes_of(r);
 }
 
 int bsf_find_rois_device(const bsf_roi_request *r, bsf_roi_result *res, int *count, void *work, size_t work_bytes,
@@ -208,9 +217,9 @@ int bsf_find_rois_device(const bsf_roi_request *r, bsf_roi_result *res, int *cou
     int rc = validate_rois(r, res);
     if (rc) return rc;
     if (!count) return fail(BSF_ERR_ARG, "bsf: NULL count");
-    if (r->nframes > 0 && (!work || work_bytes < bsf::front_work_bytes(r->nrows, r->ncols, r->nframes)))
+    if (r->nframes > 0 && (!work || work_bytes < work_bytes_of(r)))
         return fail(BSF_ERR_ARG, "bsf: work buffer of %zu B is smaller than bsf_find_rois_work_bytes (%zu B)",
-                    work_bytes, bsf::front_work_bytes(r->nrows, r->ncols, r->nframes));
+                    work_bytes, work_bytes_of(r));
     if ((rc = check_gfx950())) return rc;
     bsf::FrontLaunch L = make_front(r, r->frames, r->filter, res->imstack, res->peaks, res->value, count, work);
     hipError_t e = bsf::launch_find_rois(L, (hipStream_t)stream);
@@ -224,7 +233,7 @@ int bsf_find_rois_device_noalloc(const bsf_roi_request *r, bsf_roi_result *res, 
     int rc = validate_rois(r, res);
     if (rc) return rc;
     if (!count) return fail(BSF_ERR_ARG, "bsf: NULL count");
-    const size_t need = bsf::front_work_bytes(r->nrows, r->ncols, r->nframes);
+    const size_t need = work_bytes_of(r);
     if (need > bsf::kFrontScratchBytes)
         return fail(BSF_ERR_NOMEM,
                     "bsf: %d frames of %d x %d need %zu B of workspace; the allocation-free entry point has %zu B "
@@ -250,7 +259,7 @@ int bsf_find_rois_host(const bsf_roi_request *r, bsf_roi_result *res)
     const size_t fbytes = px * (size_t)r->nrows * r->ncols * r->nframes;
     const size_t hbytes = sizeof(float) * (size_t)r->filter_rows * r->filter_cols;
     const size_t S2 = (size_t)r->roi * r->roi, cap = (size_t)r->capacity;
-    const size_t wbytes = bsf::front_work_bytes(r->nrows, r->ncols, r->nframes);
+    const size_t wbytes = work_bytes_of(r);
     void *d_frames = nullptr, *d_work = nullptr;
     float *d_h = nullptr, *d_stack = nullptr, *d_value = nullptr;
     int *d_peaks = nullptr, *d_count = nullptr;
@@ -297,9 +306,9 @@ int bsf_fit_frames_device(const bsf_roi_request *rq, const bsf_request *fq, bsf_
     int rc = validate_rois(rq, rois);
     if (rc) return rc;
     if (!fq || !count) return fail(BSF_ERR_ARG, "bsf: NULL fit request or count");
-    if (rq->nframes > 0 && (!work || work_bytes < bsf::front_work_bytes(rq->nrows, rq->ncols, rq->nframes)))
+    if (rq->nframes > 0 && (!work || work_bytes < work_bytes_of(rq)))
         return fail(BSF_ERR_ARG, "bsf: work buffer of %zu B is smaller than bsf_find_rois_work_bytes (%zu B)",
-                    work_bytes, bsf::front_work_bytes(rq->nrows, rq->ncols, rq->nframes));
+                    work_bytes, work_bytes_of(rq));
     // the fit reads the ROI stack in place: sz = roi, nfits = capacity (the
     // outputs' stride); only the first *count ROIs are fitted
     bsf_request q = *fq;
@@ -332,7 +341,7 @@ int bsf_fit_frames_host(const bsf_roi_request *rq, const bsf_request *fq, bsf_ro
     const size_t fbytes = px * (size_t)rq->nrows * rq->ncols * rq->nframes;
     const size_t hbytes = sizeof(float) * (size_t)rq->filter_rows * rq->filter_cols;
     const size_t S2 = (size_t)rq->roi * rq->roi, cap = (size_t)rq->capacity;
-    const size_t wbytes = bsf::front_work_bytes(rq->nrows, rq->ncols, rq->nframes);
+    const size_t wbytes = work_bytes_of(rq);
     size_t cn = 1;
     for (int a = 0; a < fq->coeff_ndim; a++) cn *= (size_t)(fq->coeff_dims[a] > 0 ? fq->coeff_dims[a] : 0);
     const size_t c3n = fq->fittype == 8 && fq->coeff_ndim >= 3

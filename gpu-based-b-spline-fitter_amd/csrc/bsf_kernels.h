@@ -115,6 +115,7 @@ struct FrontLaunch {
     double cutoff;
     int S, mirror;
     double psf_size;
+    double agg_dist;      // > 0: merge each frame's maxima closer than this (the GUI's aggre)
     int cap;
     float *stack;
     int *peaks;
@@ -123,7 +124,7 @@ struct FrontLaunch {
     void *work;
 };
 hipError_t launch_find_rois(const FrontLaunch &L, hipStream_t st);
-size_t front_work_bytes(int nr, int nc, int nframes);
+size_t front_work_bytes(int nr, int nc, int nframes, bool agg);
 // The allocation-free front-end's workspace: a file-scope device array of
 // kFrontScratchBytes per GPU; front_scratch returns the current device's
 // address of it (looked up once per device).

@@ -6,12 +6,13 @@
  *
  *   [imstack, peakcoordinates, value] = bsf_find_rois(frames, roifit, h,
  *       peakcutoff, offset, conversion, subtract_min, mirror, psf_size,
- *       first_frame, capacity)
+ *       first_frame, capacity, agg_dist)
  *
  * frames: uint16 or single [nrows, ncols, nframes]; h: single filter kernel
  * (fspecial-style, e.g. the GUI's hgauss / hdog); subtract_min: 1 for the
  * DoG and PSF background modes; psf_size: size(cal.PSF, 1) in the p.conv
- * mode, else 0.  Outputs: imstack single [roifit, roifit, n], peakcoordinates
+ * mode, else 0; agg_dist: p.roifit * p.aggsz when p.agg (maxima closer
+ * than it are merged as aggre merges them), else 0.  Outputs: imstack single [roifit, roifit, n], peakcoordinates
  * double [n, 3] (x, y, frame), value single [n, 1]; n = min(found, capacity).
  *
  * Build (MATLAB):  mex -I<repo>/include bsf_find_rois.c \
@@ -36,7 +37,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     if (nrhs < 6)
         mexErrMsgIdAndTxt("bsf_find_rois:WrongNoArgs",
                           "Inputs: frames, roifit, h, peakcutoff, offset, conversion[, subtract_min, mirror, "
-                          "psf_size, first_frame, capacity]");
+                          "psf_size, first_frame, capacity, agg_dist]");
     memset(&q, 0, sizeof q);
     if (mxGetClassID(prhs[0]) == mxUINT16_CLASS)
         q.pixel_type = BSF_PIXEL_U16;
@@ -62,6 +63,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     q.subtract_min = nrhs > 6 ? (int)mxGetScalar(prhs[6]) : 0;
     q.mirror = nrhs > 7 ? (int)mxGetScalar(prhs[7]) : 0;
     q.psf_size = nrhs > 8 ? mxGetScalar(prhs[8]) : 0.0;
+    q.agg_dist = nrhs > 11 ? mxGetScalar(prhs[11]) : 0.0;
     q.first_frame = nrhs > 9 ? (int)mxGetScalar(prhs[9]) : 1;
     cap = nrhs > 10 ? (int)mxGetScalar(prhs[10]) : q.nframes * 2000;
     q.capacity = cap;

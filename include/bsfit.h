@@ -470,8 +470,15 @@ int bsf_session_graph_stats(bsf_session *s, unsigned long long *captures, unsign
  *   maxima = pixels of impf above all 8 neighbours, interior pixels, in
  *            column-major scan order, x = column + 1, y = row + 1
  *   kept   where impf > cutoff, dx < x <= ncols - dx, dx < y <= nrows - dx
- *            (:277-280; dx = floor(roi/2)), and with psf_size > 0 where
- *            psf_size/2 <= x, y <= nrows - psf_size/2 (:284-295)
+ *            (:277-280; dx = floor(roi/2))
+ *   merged with agg_dist > 0 (p.agg), per frame, as aggre (:281-283,
+ *            653-673): while two maxima ii < jj (scan order) lie at a
+ *            distance 0 < D < agg_dist, the first such pair in column-major
+ *            order over (ii, jj) (smallest jj, then smallest ii) becomes
+ *            round(mean) of the two in row ii (x, y and value; half away
+ *            from zero) and row jj is deleted
+ *   kept   with psf_size > 0 where psf_size/2 <= x, y <= nrows - psf_size/2
+ *            (:284-295)
  *   ROI k  = imphot(y-dx:y+dx, x-dx:x+dx), columns reversed if mirror
  *            (:296-307); peaks(k, :) = [x, y, frame] as peakcoordinates.
  * Frames are [nrows, ncols, nframes] column-major; frame f is numbered
@@ -497,12 +504,18 @@ typedef struct bsf_roi_request {
     int mirror;             /* p.mirror */
     double psf_size;        /* PSF-convolution mode: size(cal.PSF, 1); 0 = off */
     int capacity;           /* ROIs the outputs hold */
+    double agg_dist;        /* p.agg: p.roifit * p.aggsz, maxima closer than this are
+                               merged (aggre); 0 = off.  Aggregation enlarges the
+                               workspace (bsf_find_rois_work_bytes: ~20 B per 4 pixels
+                               of all frames) */
 } bsf_roi_request;
 
 typedef struct bsf_roi_result {
     float *imstack;         /* [roi, roi, capacity] photons */
     int *peaks;             /* [capacity, 3] column-major: x, y, frame number */
-    float *value;           /* [capacity] filtered value at the maximum, or NULL */
+    float *value;           /* [capacity] filtered value at the maximum (with
+                               aggregation, round(mean) where maxima merged, as the
+                               reference's maxima column 3), or NULL */
     int count;              /* bsf_find_rois_host: ROIs found; when above capacity
                                only the first capacity are written */
 } bsf_roi_result;
@@ -522,8 +535,9 @@ int bsf_find_rois_device(const bsf_roi_request *req, bsf_roi_result *res, int *c
  * forbid allocation).  No counterpart in the reference.  The workspace is a
  * device array inside the library, 16 MiB per GPU, held until the process
  * ends: requests needing more than that (bsf_find_rois_work_bytes; about
- * 8 B per 64 pixels of all frames, e.g. up to 120 frames of 1024 x 1024)
- * return BSF_ERR_NOMEM.  The call allocates nothing.  Calls that overlap on
+ * 8 B per 64 pixels of all frames, e.g. up to 120 frames of 1024 x 1024;
+ * with aggregation about 5 B per pixel, e.g. 3 such frames) return
+ * BSF_ERR_NOMEM.  The call allocates nothing.  Calls that overlap on
  * one GPU share that array and race: serialise them (one stream, or events). */
 int bsf_find_rois_device_noalloc(const bsf_roi_request *req, bsf_roi_result *res, int *count, void *stream);
 

@@ -22,5 +22,5 @@ from .loader import (  # noqa: F401
     cholesky, lu_eval, matinv, com2d, maxmin2d, eval_pass, fit_4pi, nv_4pi,
     ref_available, ref_bspline, ref_delta_tables, ref_deriv_eval,
     ref_cholesky, ref_lu_eval, ref_matinv, NV_OF, D_OF,
-    photons, filter2, find_maxima, find_rois, planted,
+    photons, filter2, find_maxima, find_rois, aggregate, planted,
 )

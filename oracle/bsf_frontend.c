@@ -11,6 +11,8 @@
  *        scanned column by column (x = column + 1, y = row + 1, value)
  *   MultiD_fitter/simplefitter_spline.m:277-280  value > peakcutoff, dx < x <= ncols - dx,
  *                                            dx < y <= nrows - dx
+ *   MultiD_fitter/simplefitter_spline.m:281-283, 653-673  with p.agg: aggre merges maxima
+ *                                            closer than p.roifit * p.aggsz (orc_aggregate)
  *   MultiD_fitter/simplefitter_spline.m:284-295  PSF-convolution mode: drop maxima within
  *                                            size(PSF,1)/2 of the border (nrows used for
  *                                            both axes, as the reference does)
@@ -108,12 +110,50 @@ long orc_find_maxima(const float *F, int nr, int nc, double cutoff, int dx, doub
     return n;
 }
 
+/* aggre (MultiD_fitter/simplefitter_spline.m:653-673) on one frame's maxima
+ * [x, y, value] (the reference's maxgood, double): D(ii, jj) is the distance
+ * between maxima ii < jj, D == 0 counts as NaN (never merged); while some
+ * D < thr, the first such pair in MATLAB's column-major find order (smallest
+ * jj, then smallest ii) becomes round(mean) of the two rows in row ii (x, y
+ * and value alike; MATLAB's round is half away from zero, as C's) and row jj
+ * is deleted.  Returns the new count.  Literal O(n^2) search per merge. */
+long orc_aggregate(int *xs, int *ys, double *vs, long n, double thr)
+{
+    for (;;) {
+        long fi = -1, fj = -1;
+        for (long jj = 1; jj < n && fj < 0; jj++)
+            for (long ii = 0; ii < jj; ii++) {
+                const double dxv = (double)xs[ii] - (double)xs[jj], dyv = (double)ys[ii] - (double)ys[jj];
+                const double D = sqrt(dxv * dxv + dyv * dyv);
+                if (D != 0.0 && D < thr) {
+                    fi = ii;
+                    fj = jj;
+                    break;
+                }
+            }
+        if (fj < 0) return n;
+        xs[fi] = (int)round(((double)xs[fi] + (double)xs[fj]) / 2.0);
+        ys[fi] = (int)round(((double)ys[fi] + (double)ys[fj]) / 2.0);
+        vs[fi] = round((vs[fi] + vs[fj]) / 2.0);
+        for (long k = fj; k + 1 < n; k++) {
+            xs[k] = xs[k + 1];
+            ys[k] = ys[k + 1];
+            vs[k] = vs[k + 1];
+        }
+        n--;
+    }
+}
+
 /* The whole front-end over nframes frames.  Outputs: stack [S, S, cap],
  * peaks [cap, 3] column-major (x, y, frame number), vals [cap] or NULL.
+ * With agg_dist > 0 (p.agg: p.roifit * p.aggsz) each frame's maxima go
+ * through orc_aggregate before the PSF-convolution mode's border test, in
+ * the reference's order (simplefitter_spline.m:281-295); vals then holds the
+ * aggregated value column, round(mean) for merged maxima.
  * Returns the number of ROIs found (which may exceed cap). */
 long orc_find_rois(const void *raw, int pixel_type, int nr, int nc, int nframes, int first_frame, float offset,
                    float conv, const float *h, int mr, int mc, int subtract_min, double cutoff, int S, int mirror,
-                   double psf_size, long cap, float *stack, int *peaks, float *vals)
+                   double psf_size, long cap, float *stack, int *peaks, float *vals, double agg_dist)
 {
     const long np = (long)nr * nc;
     float *ph = (float *)malloc(sizeof(float) * np);
@@ -122,7 +162,9 @@ long orc_find_rois(const void *raw, int pixel_type, int nr, int nc, int nframes,
     int *xs = (int *)malloc(sizeof(int) * (np / 4 + 8));
     int *ys = (int *)malloc(sizeof(int) * (np / 4 + 8));
     float *vs = (float *)malloc(sizeof(float) * (np / 4 + 8));
+    double *vd = (double *)malloc(sizeof(double) * (np / 4 + 8));
     const int dx = S / 2;
+    const int agg = agg_dist > 0.0;
     long total = 0;
     for (int f = 0; f < nframes; f++) {
         const char *fr = (const char *)raw + (size_t)f * np * (pixel_type == 0 ? 2 : 4);
@@ -135,7 +177,24 @@ long orc_find_rois(const void *raw, int pixel_type, int nr, int nc, int nframes,
             memcpy(X, ph, sizeof(float) * np);
         }
         orc_filter2(X, nr, nc, h, mr, mc, F);
-        const long nf = orc_find_maxima(F, nr, nc, cutoff, dx, psf_size, xs, ys, vs, np / 4 + 8);
+        long nf = orc_find_maxima(F, nr, nc, cutoff, dx, agg ? 0.0 : psf_size, xs, ys, vs, np / 4 + 8);
+        for (long q = 0; q < nf; q++) vd[q] = (double)vs[q];
+        if (agg) {
+            nf = orc_aggregate(xs, ys, vd, nf, agg_dist);
+            if (psf_size > 0) {   /* the border test after the merge, in place */
+                const double hp = psf_size / 2.0;
+                long m = 0;
+                for (long q = 0; q < nf; q++) {
+                    if ((double)xs[q] < hp || (double)xs[q] > nr - hp || (double)ys[q] < hp || (double)ys[q] > nr - hp)
+                        continue;
+                    xs[m] = xs[q];
+                    ys[m] = ys[q];
+                    vd[m] = vd[q];
+                    m++;
+                }
+                nf = m;
+            }
+        }
         for (long q = 0; q < nf; q++, total++) {
             if (total >= cap) continue;
             const int x = xs[q] - 1, y = ys[q] - 1;
@@ -147,7 +206,7 @@ long orc_find_rois(const void *raw, int pixel_type, int nr, int nc, int nframes,
             peaks[total] = xs[q];
             peaks[cap + total] = ys[q];
             peaks[2 * cap + total] = first_frame + f;
-            if (vals) vals[total] = vs[q];
+            if (vals) vals[total] = (float)vd[q];
         }
     }
     free(ph);
@@ -156,5 +215,6 @@ long orc_find_rois(const void *raw, int pixel_type, int nr, int nc, int nframes,
     free(xs);
     free(ys);
     free(vs);
+    free(vd);
     return total;
 }

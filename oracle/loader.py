@@ -383,8 +383,11 @@ def _frontend():
         lib.orc_find_maxima.restype = C.c_long
         lib.orc_find_rois.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                                       C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
-                                      C.c_int, C.c_double, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]
+                                      C.c_int, C.c_double, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_double]
         lib.orc_find_rois.restype = C.c_long
+        lib.orc_aggregate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_double]
+        lib.orc_aggregate.restype = C.c_long
         lib._fe_bound = True
     return lib
 
@@ -428,8 +431,19 @@ def find_maxima(F, cutoff, dx, psf_size=0.0):
     return np.stack([xs[:n], ys[:n], vs[:n].astype(np.float64)], 1)
 
 
+def aggregate(maxima, thr):
+    """aggre on one frame's maxima (n, 3) [x, y, value] (bsf_frontend.c:
+    orc_aggregate): the merged (m, 3) array."""
+    m = np.asarray(maxima, np.float64).reshape(-1, 3)
+    xs = np.ascontiguousarray(m[:, 0], np.int32)
+    ys = np.ascontiguousarray(m[:, 1], np.int32)
+    vs = np.ascontiguousarray(m[:, 2], np.float64)
+    n = _frontend().orc_aggregate(xs.ctypes.data, ys.ctypes.data, vs.ctypes.data, len(m), float(thr))
+    return np.stack([xs[:n].astype(np.float64), ys[:n].astype(np.float64), vs[:n]], 1)
+
+
 def find_rois(frames, roi, filt, cutoff, offset=0.0, conversion=1.0, subtract_min=False, mirror=False,
-              psf_size=0.0, first_frame=1, capacity=None):
+              psf_size=0.0, first_frame=1, capacity=None, agg_dist=0.0):
     """The whole front-end (bsf_frontend.c: orc_find_rois); same arguments and
     returns as bsfit.frames.find_rois."""
     frames = np.asfortranarray(frames)
@@ -445,6 +459,6 @@ def find_rois(frames, roi, filt, cutoff, offset=0.0, conversion=1.0, subtract_mi
     n = _frontend().orc_find_rois(frames.ctypes.data, _pixels(frames), nr, nc, nf, int(first_frame), float(offset),
                                   float(conversion), h.ctypes.data, h.shape[0], h.shape[1], int(bool(subtract_min)),
                                   float(cutoff), int(roi), int(bool(mirror)), float(psf_size), capacity,
-                                  stack.ctypes.data, peaks.ctypes.data, value.ctypes.data)
+                                  stack.ctypes.data, peaks.ctypes.data, value.ctypes.data, float(agg_dist))
     k = min(n, capacity)
     return stack[:, :, :k], np.asfortranarray(peaks[:k]), value[:k], n

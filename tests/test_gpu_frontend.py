@@ -202,3 +202,111 @@ def test_random_configurations_match_oracle(seed):
     print("seed %d: %dx%dx%d %s kernel %dx%d roi %d sub %d mirror %d psf %g cap %d" %
           (seed, nr, nc, nf, np.dtype(dt).name, mr, mc, roi, sub, mirror, psf, cap), flush=True)
     _compare(FR.find_rois(frames, roi, h, **args), oracle.find_rois(frames, roi, h, **args))
+
+
+AGG_CASES = [
+    # (nrows, ncols, nframes, dtype, kernel, subtract_min, mirror, psf_size, roi, cutoff, emitters, agg_dist)
+    (128, 128, 3, np.uint16, ("dog", 1.0), True, False, 0.0, 13, 8.0, 60, 13.0),
+    (160, 96, 2, np.float32, ("gauss", 1.0), False, True, 0.0, 11, 35.0, 50, 11.0),
+    (96, 96, 3, np.uint16, ("rand", (5, 8)), True, False, 27.0, 9, 12.0, 40, 8.0),
+    (70, 130, 2, np.uint16, ("dog", 1.0), True, False, 0.0, 7, 4.0, 30, 5.0),   # distance 5 exactly: kept
+    (64, 64, 2, np.uint16, ("dog", 1.0), True, False, 0.0, 7, 2.0, 20, 1.0),    # below any distance: no merges
+    (90, 90, 2, np.uint16, ("rand", (3, 3)), False, False, 0.0, 5, 0.0, 40, 500.0),   # one maximum per frame
+]
+
+
+@pytest.mark.parametrize("case", range(len(AGG_CASES)))
+def test_aggregation_matches_oracle(case):
+    """With agg_dist (the GUI's p.agg) the frames' maxima merge as aggre
+    merges them, before the PSF border test: counts, peak rows, ROIs and the
+    value column equal the oracle's bit for bit."""
+    nr, nc, nf, dt, kind, sub, mirror, psf, roi, cutoff, em, thr = AGG_CASES[case]
+    rng = np.random.default_rng(300 + case)
+    frames, _ = gen.make_frames(nf, nr, nc, em, 900 + case, dtype=dt)
+    h = _kernel(kind, rng)
+    args = dict(cutoff=cutoff, offset=100.0, conversion=0.5, subtract_min=sub, mirror=mirror, psf_size=psf,
+                first_frame=2, capacity=20000)
+    plain = FR.find_rois(frames, roi, h, **args)
+    got = FR.find_rois(frames, roi, h, agg_dist=thr, **args)
+    want = oracle.find_rois(frames, roi, h, agg_dist=thr, **args)
+    print("agg case %d: %d maxima, %d after aggregation at %g" % (case, plain[3], got[3], thr), flush=True)
+    _compare(got, want)
+    if thr <= 1.0:
+        _compare(got, plain)              # strict maxima are at least 2 apart
+    elif case < 3:
+        assert got[3] < plain[3], "the case should merge some maxima"
+
+
+def test_aggregation_capacity_device_and_fits():
+    """Aggregation through the capacity limit, the device entry point (its
+    larger workspace) and frames -> fits."""
+    frames, _ = gen.make_frames(3, 128, 128, 70, 4242)
+    h = FR.peak_filter("dog", 1.0)
+    args = dict(cutoff=8.0, offset=100.0, conversion=0.5, subtract_min=True, agg_dist=13.0)
+    full = FR.find_rois(frames, 13, h, capacity=5000, **args)
+    part = FR.find_rois(frames, 13, h, capacity=9, **args)
+    assert part[3] == full[3] > 9
+    _compare(part, oracle.find_rois(frames, 13, h, capacity=9, **args))
+    # device entry point: the workspace grows with aggregation
+    lib = FR._bind()
+    hip = Hip()
+    cap, roi = 64, 13
+    d_fr, d_h = hip.put(frames), hip.put(h)
+    q0 = FR.make_request(d_fr, FR.PIXEL_U16, frames.shape, roi, d_h, h.shape, 8.0, 100.0, 0.5, True, False, 0.0, 1,
+                         cap)
+    q = FR.make_request(d_fr, FR.PIXEL_U16, frames.shape, roi, d_h, h.shape, 8.0, 100.0, 0.5, True, False, 0.0, 1,
+                        cap, 13.0)
+    w0, wb = lib.bsf_find_rois_work_bytes(C.byref(q0)), lib.bsf_find_rois_work_bytes(C.byref(q))
+    assert wb >= w0 + 3 * 63 * 63 * 20
+    d_w, d_st, d_pk, d_v, d_n = (hip.alloc(wb), hip.alloc(roi * roi * cap * 4), hip.alloc(cap * 12),
+                                 hip.alloc(cap * 4), hip.alloc(4))
+    try:
+        res = FR.RoiResult(d_st, d_pk, d_v, 0)
+        rc = lib.bsf_find_rois_device(C.byref(q), C.byref(res), d_n, d_w, wb - 8, None)
+        assert rc == 1                     # refused before any launch
+        rc = lib.bsf_find_rois_device(C.byref(q), C.byref(res), d_n, d_w, wb, None)
+        assert rc == 0, lib.bsf_last_error()
+        assert hip.h.hipDeviceSynchronize() == 0
+        n = int(hip.get(d_n, 1, np.int32)[0])
+        k = min(n, cap)
+        pk = hip.get(d_pk, cap * 3, np.int32).reshape(cap, 3, order="F")[:k]
+        st = hip.get(d_st, roi * roi * cap, np.float32).reshape(roi, roi, cap, order="F")[:, :, :k]
+        v = hip.get(d_v, cap, np.float32)[:k]
+    finally:
+        hip.free(d_fr, d_h, d_w, d_st, d_pk, d_v, d_n)
+    want = FR.find_rois(frames, roi, h, capacity=cap, **args)
+    assert n == want[3] == full[3]
+    np.testing.assert_array_equal(pk, want[1])
+    np.testing.assert_array_equal(st, want[0])
+    np.testing.assert_array_equal(v, want[2])
+    # frames -> ROIs -> fits with aggregation equals fitting the aggregated ROIs
+    w = gen.config(0, N=4)
+    st, pk, val, P, CR, LL, n = FR.fit_frames(frames, 13, h, 8.0, 6, 50, w.coeff, offset=100.0, conversion=0.5,
+                                              subtract_min=True, capacity=5000, agg_dist=13.0)
+    assert n == full[3]
+    np.testing.assert_array_equal(pk, full[1])
+    P2, CR2, LL2 = bsfit.fit(6, full[0], 50, w.coeff)
+    np.testing.assert_array_equal(P, P2)
+    np.testing.assert_array_equal(CR, CR2)
+    np.testing.assert_array_equal(LL, LL2)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_aggregation_matches_oracle(seed):
+    """Random frames, kernels, ROI sizes, PSF borders and thresholds with
+    aggregation, against the oracle bit for bit."""
+    rng = np.random.default_rng(9100 + seed)
+    nr, nc = int(rng.integers(40, 200)), int(rng.integers(40, 200))
+    nf = int(rng.integers(1, 4))
+    dt = np.uint16 if rng.random() < 0.6 else np.float32
+    roi = int(rng.choice(np.arange(3, 16, 2)))
+    psf = float(rng.choice([0.0, 13.0, 27.0]))
+    frames, _ = gen.make_frames(nf, nr, nc, max(2, nr * nc // 400), 9500 + seed, dtype=dt)
+    h = FR.peak_filter("dog", float(rng.uniform(0.8, 1.6)))
+    thr = float(rng.choice([2.0, 2.5, np.sqrt(8.0), 7.0, roi * rng.uniform(0.5, 2.0)]))
+    args = dict(cutoff=float(rng.uniform(2, 20)), offset=100.0, conversion=0.5, subtract_min=True,
+                mirror=bool(rng.random() < 0.3), psf_size=psf, first_frame=1, capacity=int(rng.integers(1, 3000)),
+                agg_dist=thr)
+    print("seed %d: %dx%dx%d roi %d psf %g agg %g cap %d" % (seed, nr, nc, nf, roi, psf, thr, args["capacity"]),
+          flush=True)
+    _compare(FR.find_rois(frames, roi, h, **args), oracle.find_rois(frames, roi, h, **args))

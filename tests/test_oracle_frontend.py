@@ -223,3 +223,121 @@ def test_psf_kernels():
     p64 = 5.0 * psf.astype(np.float64)
     mean_k = (p64 - p64.mean(0, keepdims=True)).mean(2)
     np.testing.assert_allclose(FR.peak_filter_psf("mean", psf, 13), mean_k.astype(np.float32), rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------------ aggregation
+def _aggre_matlab(data, thr):
+    """aggre (MultiD_fitter/simplefitter_spline.m:653-673) as MATLAB runs it:
+    the distance matrix of the upper triangle with its zeros set to NaN, the
+    first true entry of D < thr in column-major order, round (half away from
+    zero) of the mean into row ii and row jj deleted."""
+    data = np.array(data, np.float64).reshape(-1, 3)
+    while True:
+        n = len(data)
+        D = np.zeros((max(n - 1, 0), n))
+        for ii in range(n - 1):
+            for jj in range(ii + 1, n):
+                D[ii, jj] = np.sqrt((data[ii, 0] - data[jj, 0]) ** 2 + (data[ii, 1] - data[jj, 1]) ** 2)
+        D[D == 0] = np.nan
+        with np.errstate(invalid="ignore"):
+            agind = D < thr
+        hits = np.argwhere(agind.T)          # (jj, ii) in column-major order
+        if len(hits) == 0:
+            return data
+        jj, ii = hits[0]
+        m = (data[ii] + data[jj]) / 2
+        data[ii] = np.sign(m) * np.floor(np.abs(m) + 0.5)
+        data = np.delete(data, jj, 0)
+
+
+def test_aggregate_known_answers():
+    # distance 5: merged below a threshold above 5, kept at exactly 5 (D < thr)
+    two = [[10, 10, 5.0], [13, 14, 8.0]]
+    np.testing.assert_array_equal(oracle.aggregate(two, 5.5), [[12, 12, 7.0]])
+    np.testing.assert_array_equal(oracle.aggregate(two, 5.0), two)
+    # coincident maxima are never merged (D == 0 is NaN in the reference)
+    same = [[10, 10, 1.0], [10, 10, 2.0]]
+    np.testing.assert_array_equal(oracle.aggregate(same, 100.0), same)
+    # the first pair in column-major order wins: A-B (column 2) before B-C
+    # (column 3); the merged point (2) is then 4 from C
+    row = [[1, 5, 1.0], [4, 5, 1.0], [7, 5, 1.0]]
+    np.testing.assert_array_equal(oracle.aggregate(row, 3.5), [[3, 5, 1.0], [7, 5, 1.0]])
+    # a merge that lands next to a third maximum merges again: A-C first
+    # (column 3; A-B is 2 apart), round(1.5) = 2, then (2, 1) with B: round(2.5)
+    tri = [[1, 1, 0.0], [3, 1, 0.0], [2, 1, 3.0]]
+    np.testing.assert_array_equal(oracle.aggregate(tri, 1.5), [[3, 1, 1.0]])
+    # half-way values round away from zero, negative ones too
+    neg = [[1, 1, -3.0], [2, 1, -4.0]]
+    np.testing.assert_array_equal(oracle.aggregate(neg, 2.0), [[2, 1, -4.0]])
+    assert oracle.aggregate(np.zeros((0, 3)), 3.0).shape == (0, 3)
+    np.testing.assert_array_equal(oracle.aggregate([[4, 4, 1.0]], 3.0), [[4, 4, 1.0]])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_aggregate_matches_matlab_semantics(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 40))
+    side = int(rng.integers(6, 40))
+    data = np.stack([rng.integers(1, side, n), rng.integers(1, side, n),
+                     rng.normal(20, 5, n).astype(np.float32).astype(np.float64)], 1)
+    if seed % 2:
+        data[n // 2] = data[0]                    # a coincident pair
+    thr = float(rng.uniform(1.0, 8.0)) if seed != 5 else 3.0   # 3.0: distances equal to the threshold
+    np.testing.assert_array_equal(oracle.aggregate(data, thr), _aggre_matlab(data, thr))
+
+
+def test_find_rois_aggregates_before_the_psf_border_test():
+    """find_rois with agg_dist is the composition in the reference's order:
+    maxima (cutoff and dx borders), aggre, then the PSF border test, then the
+    cut (simplefitter_spline.m:277-307)."""
+    frames, _ = gen.make_frames(7, 96, 96, 40, 2)
+    h = np.ones((3, 3), np.float32) / 9
+    args = dict(cutoff=55.0, offset=100, conversion=0.5)
+    roi, thr, psf = 9, 11.0, 30.0
+    st, pk, val, n = oracle.find_rois(frames, roi, h, capacity=2000, psf_size=psf, agg_dist=thr, **args)
+    want_pk, want_st, want_v = [], [], []
+    for f in range(frames.shape[2]):
+        ph = oracle.photons(frames[:, :, f], 100, 0.5)
+        F = oracle.filter2(h, ph)
+        m = _aggre_matlab(oracle.find_maxima(F, 55.0, roi // 2), thr)
+        hp = psf / 2
+        keep = ~((m[:, 0] < hp) | (m[:, 0] > 96 - hp) | (m[:, 1] < hp) | (m[:, 1] > 96 - hp))
+        for x, y, v in m[keep]:
+            x, y = int(x) - 1, int(y) - 1
+            want_pk.append((x + 1, y + 1, f + 1))
+            want_st.append(ph[y - roi // 2:y + roi // 2 + 1, x - roi // 2:x + roi // 2 + 1])
+            want_v.append(v)
+    assert n == len(want_pk) and n > 0
+    merged = n < len(oracle.find_rois(frames, roi, h, capacity=2000, psf_size=psf, **args)[1])
+    assert merged, "the case should merge some maxima"
+    np.testing.assert_array_equal(pk, np.array(want_pk))
+    np.testing.assert_array_equal(np.transpose(st, (2, 0, 1)), np.array(want_st))
+    np.testing.assert_array_equal(val, np.array(want_v, np.float32))
+
+
+def test_aggregate_planted_mistake_is_caught():
+    """A search in row-major order (first ii, then jj) -- the plausible
+    mistake -- gives another result on the three-in-a-row case."""
+    def rowmajor(data, thr):
+        data = [list(r) for r in data]
+        while True:
+            hit = None
+            for ii in range(len(data)):
+                for jj in range(ii + 1, len(data)):
+                    d = np.hypot(data[ii][0] - data[jj][0], data[ii][1] - data[jj][1])
+                    if 0 < d < thr:
+                        hit = (ii, jj)
+                        break
+                if hit:
+                    break
+            if not hit:
+                return np.array(data, np.float64)
+            ii, jj = hit
+            data[ii] = [np.floor((a + b) / 2 + 0.5) for a, b in zip(data[ii], data[jj])]
+            del data[jj]
+    # column-major merges 7-9 first (column 3) and then 3-5; row-major
+    # merges 3-7 first and chains everything into one maximum
+    case = [[3, 5, 1.0], [7, 5, 1.0], [9, 5, 1.0], [5, 5, 1.0]]
+    np.testing.assert_array_equal(oracle.aggregate(case, 3.5), [[4, 5, 1.0], [8, 5, 1.0]])
+    assert not np.array_equal(rowmajor(case, 3.5), oracle.aggregate(case, 3.5))
+    np.testing.assert_array_equal(_aggre_matlab(case, 3.5), oracle.aggregate(case, 3.5))
