@@ -510,7 +510,10 @@ def main():
             os.environ["GPU_MAX_HW_QUEUES"] = str(need)
         serve(args)
         return
-
+    # the host-to-host figure streams frames through a session with 4 kernel
+    # streams and 2 copy streams: one hardware queue each (HIP's default is 4)
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 6:
+        os.environ["GPU_MAX_HW_QUEUES"] = "6"
     fit_bench(args)
 
 
@@ -680,26 +683,45 @@ class FitCase:
                              "f_alg": r["f_alg"][int(w.S)], "mean_evals": r["mean_evals"]}
         return out
 
-    def host_to_host(self, steps):
-        """The same ROIs host -> host through a serving session: one frame of
-        one group per ROI size (pack into pinned memory, one copy in, one
-        launch of mixed sizes, one copy out, results scattered to the
-        caller's arrays)."""
+    def host_to_host(self, steps, frames=8, kernel_streams=4):
+        """The same ROIs host -> host through a serving session, pipelined as a
+        streaming caller would: `frames` frames, each a share of every ROI
+        size group (pack into the frame's pinned slot, one copy in, one launch
+        of mixed sizes, one copy out, results scattered to the caller's
+        arrays), all submitted before the first poll, so one frame's packing
+        and copies overlap the launches of the others on `kernel_streams`
+        streams (with the session's two copy streams, one hardware queue
+        each: fit_bench raises HIP's default of four to six)."""
         import bsfit
         if self.ft == 104:
             return None
         need = sum(w.imstack.nbytes for w in self.groups) + self.N * (2 * self.NV + 2) * 4 + self.N * 16
-        with bsfit.Session(arena_bytes=int(need * 1.25) + (16 << 20), kernel_streams=1,
+        parts = []
+        for i in range(frames):
+            fr = []
+            for w in self.groups:
+                a, b = i * w.N // frames, (i + 1) * w.N // frames
+                fr.append((w.imstack[:, :, a:b], w))
+            parts.append(fr)
+        with bsfit.Session(arena_bytes=int(need * 1.5) + (64 << 20), kernel_streams=kernel_streams,
                            precision=self.prec) as s:
             m = s.register_model(self.ft, self.coeff, self.coeff3)
-            groups = [bsfit.Group(m, w.imstack, w.iterations, initP=w.initP) for w in self.groups]
-            s.poll(s.submit(groups), wait=True)
+
+            def step():
+                tickets = [s.submit([bsfit.Group(m, im, w.iterations, initP=w.initP) for im, w in fr])
+                           for fr in parts]
+                for t in tickets:
+                    s.poll(t, wait=True)
+            for _ in range(2):
+                step()
             t0 = time.perf_counter()
             for _ in range(steps):
-                s.poll(s.submit(groups), wait=True)
+                step()
             dt = (time.perf_counter() - t0) / steps
-        return {"fits_per_s": self.N / dt, "ms_per_step": dt * 1e3, "steps": steps,
-                "path": "bsf_submit + bsf_poll: host arrays -> pinned arena -> H2D -> one launch -> D2H -> host arrays"}
+        return {"fits_per_s": self.N / dt, "ms_per_step": dt * 1e3, "steps": steps, "frames": frames,
+                "kernel_streams": kernel_streams,
+                "path": "bsf_submit x %d frames, then bsf_poll each: host arrays -> pinned arena -> H2D -> one "
+                        "launch of mixed sizes per frame -> D2H -> host arrays" % frames}
 
 
 def fit_bench(args):
