@@ -507,7 +507,9 @@ typedef struct bsf_roi_request {
     double agg_dist;        /* p.agg: p.roifit * p.aggsz, maxima closer than this are
                                merged (aggre); 0 = off.  Aggregation enlarges the
                                workspace (bsf_find_rois_work_bytes: ~20 B per 4 pixels
-                               of all frames) */
+                               of all frames) and, as the reference's loop, searches
+                               O(n^2) pairs per merge for a frame of n maxima: meant
+                               for the tens to thousands a frame has above a cutoff */
 } bsf_roi_request;
 
 typedef struct bsf_roi_result {
