@@ -529,13 +529,19 @@ def tiles_bench(args):
     from bsfit import gen
     nu, rep = 8192, 16
     N = nu * rep
-    for ft in (6, 7):
-        c = gen.model_coeffs("astig3d" if ft == 6 else "color4d")
+    for ft in (6, 7, 8):
+        c3 = None
+        if ft == 8:
+            c, c3 = gen.model_coeffs("dipole5d")
+        else:
+            c = gen.model_coeffs("astig3d" if ft == 6 else "color4d")
         NV = bsfit.num_params(ft)
-        dc = torch.from_numpy(np.asfortranarray(c, np.float32).ravel(order="F").copy()).cuda()
-        sizes = (args.roi_size,) if args.roi_size else (5, 7, 9, 11, 13, 15, 17, 21)
+        up = lambda a: torch.from_numpy(np.asfortranarray(a, np.float32).ravel(order="F").copy()).cuda()
+        dc = up(c)
+        dc3 = up(c3) if c3 is not None else None
+        sizes = (args.roi_size,) if args.roi_size else ((5, 7, 9, 11, 13, 15, 17, 21) if ft != 8 else (11, 13, 15, 17))
         for S in sizes:
-            w = gen.make_rois(ft, nu, S, 5000 + S, c)
+            w = gen.make_rois(ft, nu, S, 5000 + S, c, c3)
             im = np.asfortranarray(w.imstack, np.float32).ravel(order="F")
             data = torch.from_numpy(np.tile(im, rep)).cuda()
             P = torch.empty((NV + 1) * N, device="cuda")
@@ -544,7 +550,8 @@ def tiles_bench(args):
             for prec in (bsfit.PREC_MIX, bsfit.PREC_F64):
                 default = bsfit.fit_tile(ft, prec, S)
                 t, ms = bsfit.tune(ft, data.data_ptr(), S, N, 50, dc.data_ptr(), c.shape, P.data_ptr(),
-                                   CR.data_ptr(), LL.data_ptr(), initP=w.initP, reps=args.steps)
+                                   CR.data_ptr(), LL.data_ptr(), initP=w.initP, reps=args.steps,
+                                   coeff3_ptr=dc3.data_ptr() if dc3 is not None else None, precision=prec)
                 bsfit.set_fit_tile(ft, prec, S, 0)
                 print(json.dumps({"workload": "tiles", "fittype": ft, "precision": "f64" if prec else "mix",
                                   "roi_size": S, "rois": N, "ms_tile1": round(ms[0], 4), "ms_tile2": round(ms[1], 4),

@@ -248,8 +248,11 @@ __device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> 
     const E off = (E)(A.nd[0] - S) / E(2);
     const E colc = (E)((S + 1) / 2) - cth[0];
     const E rowc = (E)((S + 1) / 2) - cth[1];
-    if (lane < 8 * D) {
-        const int a = lane >> 3, f = lane & 7, idx = (f & 1) + 1;
+    // 8 factors per axis: lanes 0 .. 8D-1 of the ROI (fittype 8 in a
+    // two-ROI wave has 32 lanes for its 40: the first 8 take two)
+#pragma unroll
+    for (int fl = lane; fl < 8 * D; fl += 64 / R) {
+        const int a = fl >> 3, f = fl & 7, idx = (f & 1) + 1;
         E c = rowc;
         c = (a == 1) ? colc : c;
 #pragma unroll
@@ -259,7 +262,7 @@ __device__ __forceinline__ void make_geo(const Dev &A, const E *cth, Geo<E, FT> 
         //   4,5: beta2(dd + idx - 1/2); 6,7: beta2(dd - idx + 1/2)
         const E x3 = (f < 2) ? d + (E)(idx - 1) : d - (E)idx;
         const E x2 = (f < 6) ? dd + ((E)idx - E(0.5)) : dd - ((E)idx - E(0.5));
-        L.fac[lane] = (f < 4) ? bsp3s<E>(x3) : bsp2s<E>(x2);
+        L.fac[fl] = (f < 4) ? bsp3s<E>(x3) : bsp2s<E>(x2);
     }
     // per row / column taps; the sequence is non-decreasing except where a
     // coordinate equals n exactly (tap n+1, the maximum; all later ones are n)
@@ -478,14 +481,17 @@ __device__ __forceinline__ void stage_a(const Dev &A, const Geo<E, FT> &g, E *pl
             planes[3 * plane_stride + pq] = du;
             planes[4 * plane_stride + pq] = zb;
             planes[5 * plane_stride + pq] = ub;
-            // the 3D model (coeff3) shares the xy window and the z taps
+            // the 3D model (coeff3) shares the xy window and the z taps (the
+            // ROI's z tap: wave-uniform, or per lane in a two-ROI wave)
             const __amdgpu_buffer_rsrc_t r3 =
                 __builtin_amdgcn_make_buffer_rsrc((void *)A.coeff3, (short)0, A.c3bytes, 0x00020000);
-            const int tap3 = __builtin_amdgcn_readfirstlane((g.knx[0] - 2) * s2);
+            int tap3 = (g.knx[0] - 2) * s2, vo3 = vo - vtap;
+            if constexpr (R == 1) tap3 = __builtin_amdgcn_readfirstlane(tap3);
+            else vo3 += tap3, tap3 = 0;
             E v3 = E(0), dz3 = E(0);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                E cc = (E)bload(r3, vo, tap3 + t * s2);
+                E cc = (E)bload(r3, vo3, tap3 + t * s2);
                 v3 += z.W[t] * cc;
                 dz3 += z.Dv[t] * cc;
             }
@@ -1098,6 +1104,39 @@ __device__ __forceinline__ void wave_sum_lds2(const E (&v0)[V], const E (&v1)[V]
             t += dpp_move<kQuadXor2>(t);
             if (vv < C && q4 == 0) tot[k0 + vv] = t;
         }
+        __syncthreads();
+    }
+}
+
+// wave_sum_lds for a wave that fits two ROIs with one sum per lane (fittype
+// 8: lane l of a half holds the sums of the half's pixels l, l + 32, ...):
+// each half reduces its 32 lanes, 8 values at a time (the chunk transposed
+// to LDS, lane 4v+q adds entries 8q..8q+7 of value v, the quads combine).
+// The totals land in scratch[16*65 + k], as wave_sum_lds leaves them.
+template <typename E, int V>
+__device__ __forceinline__ void wave_sum_lds_half(const E (&v)[V], E *scratch, int hl)
+{
+    E *tot = scratch + kRedChunk * 65;
+    const int vi = hl >> 2, q4 = hl & 3;
+#pragma unroll
+    for (int k0 = 0; k0 < V; k0 += 8) {
+#pragma unroll
+        for (int k = k0; k < V && k < k0 + 8; k++) scratch[(k - k0) * 33 + hl] = v[k];
+        __syncthreads();
+        const int C = (V - k0) < 8 ? (V - k0) : 8;
+        E s0 = E(0), s1 = E(0);
+        if (vi < C) {
+            const E *row = scratch + vi * 33 + q4 * 8;
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                s0 += row[j];
+                s1 += row[j + 1];
+            }
+        }
+        E t = s0 + s1;
+        t += dpp_move<kQuadXor1>(t);
+        t += dpp_move<kQuadXor2>(t);
+        if (vi < C && q4 == 0) tot[k0 + vi] = t;
         __syncthreads();
     }
 }

@@ -68,15 +68,19 @@ __device__ __forceinline__ void roi_init4(const float *d, int S, E *v)
 // per-ROI work that does not spread over lanes (geometry, the damped solve,
 // the LM control, the reductions' tails) runs once for both.  A half's lane
 // holds the pixel sums of two lanes of the one-ROI kernel and reduces them in
-// its order, so both forms give the same bits.  (The geometry takes 8 lanes
-// per spline axis: fittype 8's five axes need 40, so it runs with R = 1.)
+// its order, so both forms give the same bits -- except FP32 fittype 8,
+// whose lanes keep one set of sums over the half's pixels (HALF_SUMS below).
 // The fit of the ROIs of workgroup `block` of `nblocks`; fit_kernel runs it
 // for its own workgroup, the heterogeneous kernel (bsf_fit_hetero.hip) for
 // the workgroup's place within its launch entry.
 template <int FT, typename E, int SC, int R = 1>
 __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblocks)
 {
-    static_assert(R == 1 || (R == 2 && FT != 8), "two ROIs per wave: fittypes 6 and 7");
+    static_assert(R == 1 || (R == 2 && (FT != 8 || sizeof(E) == 4)), "two ROIs per wave: fittypes 6, 7, FP32 8");
+    // fittype 8 in a two-ROI wave keeps one set of pixel sums per lane (the
+    // half's pixels l, l + 32, ...): a second set, for the one-ROI order,
+    // does not fit its registers, so its sums are not the one-ROI kernel's bits
+    constexpr bool HALF_SUMS = R == 2 && FT == 8;
     constexpr int NV = Ft<FT>::NV;
     constexpr int NH = NV * (NV + 1) / 2;
     constexpr int NRED = 1 + NV + NH;
@@ -269,6 +273,11 @@ __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblock
             }
             __syncthreads();   // planes are dead; the scratch reuses them
             wave_sum_lds<E, NRED>(acc, s_red, lane);
+        } else if constexpr (HALF_SUMS) {
+#pragma nounroll
+            for (int p = hl; p < S2; p += L) pixel_terms(p, first, acc);
+            __syncthreads();   // planes are dead; the scratch reuses them
+            wave_sum_lds_half<E, NRED>(acc, s_red, hl);
         } else {
             // pixels of the one-ROI kernel's lanes hl (acc) and hl + 32 (acc1)
             E acc1[NRED];
@@ -283,7 +292,9 @@ __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblock
             wave_sum_lds2<E, NRED>(acc, acc1, s_red, hl);
         }
         const E *tot = s_red + kRedChunk * 65;
-        if (hl < NV + NH) s_jh[hl] = (double)tot[1 + hl];
+        // J and H (NV + NH values: 44 for fittype 8, more than a half-wave's lanes)
+#pragma unroll
+        for (int k = hl; k < NV + NH; k += L) s_jh[k] = (double)tot[1 + k];
         return tot[0];
     };
 
@@ -371,7 +382,7 @@ __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblock
             th = th - upd;
             sumUpdate = E(0);
 #pragma unroll
-            for (int l = 0; l < NV; l++) sumUpdate += fabs(readlane_e<E>(upd, l));
+            for (int l = 0; l < NV; l++) sumUpdate += fabs(roi_read<R, E>(upd, l, hb));
         } else {
             th = th - upd / (E(1) + fabs(upd / clampv));
         }
@@ -426,10 +437,10 @@ __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblock
     } else {
         // the last evaluation's geometry again (x, y, z, w, u unchanged in
         // cth; recomputing it is cheaper than keeping it live in registers)
-        cth[2] = readlane_e<E>(th, 2);
-        cth[3] = readlane_e<E>(th, 3);
+        cth[2] = roi_read<R, E>(th, 2, hb);
+        cth[3] = roi_read<R, E>(th, 3, hb);
         __syncthreads();
-        make_geo<E, FT>(A, cth, g, lane, gl, SC != 0);
+        make_geo<E, FT, R>(A, cth, g, hl, gl, SC != 0, hb);
     }
     __syncthreads();
     stage_a<E, FT, R>(A, g, s_pl, plane_stride, hl);
@@ -479,6 +490,11 @@ __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblock
         }
         __syncthreads();
         wave_sum_lds<E, NF>(acc, s_red, lane);
+    } else if constexpr (HALF_SUMS) {
+#pragma nounroll
+        for (int p = hl; p < S2; p += L) fisher_terms(p, acc);
+        __syncthreads();
+        wave_sum_lds_half<E, NF>(acc, s_red, hl);
     } else {
         E acc1[NF];
 #pragma unroll
@@ -534,7 +550,8 @@ __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblock
 }
 
 template <int FT, typename E, int SC, int R = 1>
-__global__ __launch_bounds__(64) void fit_kernel(Dev A)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FT == 8 && R == 2 ? 2 : 1)))
+void fit_kernel(Dev A)
 {
     fit_roi<FT, E, SC, R>(A, (int)blockIdx.x, (int)gridDim.x);
 }

@@ -15,8 +15,8 @@ size_t lds_bytes(int fittype, int f64, int S) { return fit_lds_bytes(fittype, f6
 
 // --------------------------------------------------------------- tiles
 namespace {
-// recorded tiles, [fittype 6 / 7][FP64][ROI size]; 0: the default
-std::atomic<int> g_tile[2][2][kTileMaxS + 1];
+// recorded tiles, [fittype 6 / 7 / 8][FP64][ROI size]; 0: the default
+std::atomic<int> g_tile[3][2][kTileMaxS + 1];
 
 // measured on MI355X (131,072 ROIs per launch, both tiles' bsf_tune
 // timings): two ROIs per wave win for FP32 3D fits of up to 9 x 9 pixels
@@ -27,22 +27,26 @@ std::atomic<int> g_tile[2][2][kTileMaxS + 1];
 int default_tile(int fittype, int f64, int S) { return fittype == 6 && !f64 && S <= 9 ? 2 : 1; }
 }  // namespace
 
+// two ROIs per wave: fittypes 6 and 7, and fittype 8 in FP32 (in FP64 its
+// registers and LDS already hold one ROI per SIMD)
 int tile_possible(int fittype, int f64, int S, int tile)
 {
     if (tile == 1) return 1;
-    return tile == 2 && (fittype == 6 || fittype == 7) && 2 * lds_bytes(fittype, f64, S) <= kTileLdsMax;
+    // (a half-wave's 32 lanes hold the tap rows of at most 32 ROI rows)
+    return tile == 2 && (fittype == 6 || fittype == 7 || (fittype == 8 && !f64)) && S <= 32 &&
+           2 * lds_bytes(fittype, f64, S) <= kTileLdsMax;
 }
 
 int fit_tile(int fittype, int f64, int S)
 {
-    if ((fittype != 6 && fittype != 7) || S < 1 || S > kTileMaxS) return 1;
+    if (fittype < 6 || fittype > 8 || S < 1 || S > kTileMaxS) return 1;
     const int t = g_tile[fittype - 6][f64 ? 1 : 0][S].load(std::memory_order_relaxed);
     return t ? t : default_tile(fittype, f64, S);
 }
 
 int set_fit_tile(int fittype, int f64, int S, int tile)
 {
-    if ((fittype != 6 && fittype != 7) || S < 1 || S > kTileMaxS || tile < 0 || tile > 2) return -1;
+    if (fittype < 6 || fittype > 8 || S < 1 || S > kTileMaxS || tile < 0 || tile > 2) return -1;
     if (tile && !tile_possible(fittype, f64, S, tile)) return -1;
     g_tile[fittype - 6][f64 ? 1 : 0][S].store(tile, std::memory_order_relaxed);
     return 0;
@@ -62,6 +66,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
         case 13: return launch_t<6, double, 0, 2>(A, nb, st);
         case 14: return sized_kernel(7, sized) ? launch_sized7(A, nb, st, sized, 2) : launch_t<7, float, 0, 2>(A, nb, st);
         case 15: return launch_t<7, double, 0, 2>(A, nb, st);
+        case 16: return launch_t<8, float, 0, 2>(A, nb, st);
         default: return hipErrorInvalidValue;
         }
     }

@@ -53,8 +53,8 @@ hipError_t launch_fit_hetero(const FitLaunch *L, int n, hipStream_t st);
 // ROIs per wave of the fit kernel, per (fittype, precision, ROI size) of a
 // launch (its largest size with a ROI table): the measured default, or the
 // tile set_fit_tile / the autotuner recorded.  A launch whose FitLaunch.tile
-// is 0 takes it from here.  Fittype 8, device-side counts and tiles whose
-// LDS (two regions) exceed kTileLdsMax run one ROI per wave.
+// is 0 takes it from here.  FP64 fittype 8, device-side counts and tiles
+// whose LDS (two regions) exceed kTileLdsMax run one ROI per wave.
 constexpr int kTileMaxS = 64;
 constexpr size_t kTileLdsMax = 64 * 1024;
 int fit_tile(int fittype, int f64, int S);

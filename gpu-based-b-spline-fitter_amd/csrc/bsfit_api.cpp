@@ -260,9 +260,10 @@ int bsf_tune(const bsf_request *r, bsf_result *res, int reps, int *tile, float *
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     float best = 0.f;
+    int last = 1;
     for (int t = 1; t <= 2 && e == hipSuccess; t++) {
         if (!bsf::tile_possible(r->fittype, f64, r->sz, t)) continue;
-        L.tile = t;
+        L.tile = last = t;
         e = launch_all(L, st);   // warm-up: code object load, caches
         if (e == hipSuccess) e = hipEventRecord(e0, st);
         for (int k = 0; k < reps && e == hipSuccess; k++) e = launch_all(L, st);
@@ -275,10 +276,17 @@ int bsf_tune(const bsf_request *r, bsf_result *res, int reps, int *tile, float *
         if (ms) ms[t - 1] = t_ms;
         if (t == 1 || t_ms < best) best = t_ms, *tile = t;
     }
+    // res holds the fits of the chosen tile (fittype 8's two tiles sum the
+    // pixels in different orders)
+    if (e == hipSuccess && *tile != last) {
+        L.tile = *tile;
+        e = launch_all(L, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (e != hipSuccess) return fail(BSF_ERR_DEVICE, "bsf: tuning launch failed: %s", hipGetErrorString(e));
-    if (r->fittype == 6 || r->fittype == 7) bsf::set_fit_tile(r->fittype, f64, r->sz, *tile);
+    bsf::set_fit_tile(r->fittype, f64, r->sz, *tile);
     return BSF_OK;
 }
 

@@ -147,7 +147,8 @@ typedef struct bsf_roi {
  *           rejected step, and the final Fisher pass)
  * and the launch's tile (ROIs per wavefront, see bsf_tune): 0 takes the
  * recorded choice for (fittype, precision, sz), 1 or 2 forces it (2 is
- * taken by fittypes 6 and 7 only; elsewhere the launch runs with 1). */
+ * taken by fittypes 6, 7 and FP32 fittype 8 up to sz 32 where two ROIs'
+ * LDS fits 64 KB; elsewhere the launch runs with 1). */
 typedef struct bsf_fit_extras {
     const bsf_roi *rois;
     unsigned char *status;
@@ -172,9 +173,12 @@ int bsf_fit_device_ex(const bsf_request *req, bsf_result *res, const bsf_fit_ext
  * `stream` (waited for); records the fastest for (fittype, precision, sz)
  * in the process-wide table that later fits of that kind use, and returns
  * it in *tile; ms (2 floats, may be NULL) receives each tile's mean launch
- * time in milliseconds (0 where the tile is not possible).  Both tiles give
- * the same results bit for bit; res holds them afterwards.  Fittype 8 and
- * the 4Pi fitter have one tile. */
+ * time in milliseconds (0 where the tile is not possible); res holds the
+ * fits of the chosen tile afterwards.  For fittypes 6 and 7 both tiles give
+ * the same results bit for bit.  A two-ROI fittype-8 wave (FP32 only) keeps
+ * one set of pixel sums per lane, added in another order than the one-ROI
+ * kernel's, so its fits differ in the last bits (same accuracy against the
+ * FP64 oracle).  FP64 fittype 8 and the 4Pi fitter have one tile. */
 int bsf_tune(const bsf_request *req, bsf_result *res, int reps, int *tile, float *ms, void *stream);
 
 /* The recorded tile for (fittype, precision, sz) (1 or 2), and setting it

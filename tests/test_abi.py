@@ -207,27 +207,28 @@ def test_roi_table_layout():
 
 def test_tile_table_without_device():
     """The tile table (bsf_fit_tile / bsf_set_fit_tile) is host state:
-    fittypes 6 and 7 take tiles 1 and 2, fittype 8 only 1, tile 0 restores
+    fittypes 6, 7 and FP32 fittype 8 take tiles 1 and 2, FP64 fittype 8
+    only 1 (and no tile 2 above ROI size 32), tile 0 restores
     the default; bsf_tune and bsf_fit_device_ex check their arguments before
     any device work."""
     import bsfit
     from bsfit.api import Extras, Request, Result, _request
     lib = bsfit.load()
-    for ft in (6, 7):
-        for prec in (bsfit.PREC_MIX, bsfit.PREC_F64):
-            d = bsfit.fit_tile(ft, prec, 13)
-            assert d in (1, 2)
-            try:
-                bsfit.set_fit_tile(ft, prec, 13, 2)
-                assert bsfit.fit_tile(ft, prec, 13) == 2
-                bsfit.set_fit_tile(ft, prec, 13, 1)
-                assert bsfit.fit_tile(ft, prec, 13) == 1
-            finally:
-                bsfit.set_fit_tile(ft, prec, 13, 0)
-            assert bsfit.fit_tile(ft, prec, 13) == d
-    assert bsfit.fit_tile(8, bsfit.PREC_MIX, 13) == 1
-    for bad in ((8, bsfit.PREC_MIX, 13, 2), (6, bsfit.PREC_MIX, 13, 3), (6, bsfit.PREC_MIX, 0, 1),
-                (6, 7, 13, 1), (7, bsfit.PREC_MIX, 65, 1)):
+    for ft, prec in ((6, bsfit.PREC_MIX), (6, bsfit.PREC_F64), (7, bsfit.PREC_MIX), (7, bsfit.PREC_F64),
+                     (8, bsfit.PREC_MIX)):
+        d = bsfit.fit_tile(ft, prec, 13)
+        assert d in (1, 2)
+        try:
+            bsfit.set_fit_tile(ft, prec, 13, 2)
+            assert bsfit.fit_tile(ft, prec, 13) == 2
+            bsfit.set_fit_tile(ft, prec, 13, 1)
+            assert bsfit.fit_tile(ft, prec, 13) == 1
+        finally:
+            bsfit.set_fit_tile(ft, prec, 13, 0)
+        assert bsfit.fit_tile(ft, prec, 13) == d
+    assert bsfit.fit_tile(8, bsfit.PREC_MIX, 13) == 1 and bsfit.fit_tile(8, bsfit.PREC_F64, 13) == 1
+    for bad in ((8, bsfit.PREC_F64, 13, 2), (8, bsfit.PREC_MIX, 33, 2), (6, bsfit.PREC_MIX, 13, 3),
+                (6, bsfit.PREC_MIX, 0, 1), (6, 7, 13, 1), (7, bsfit.PREC_MIX, 65, 1)):
         with pytest.raises(bsfit.BsfError) as e:
             bsfit.set_fit_tile(*bad)
         assert e.value.code == 1
