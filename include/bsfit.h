@@ -178,7 +178,12 @@ int bsf_fit_device_ex(const bsf_request *req, bsf_result *res, const bsf_fit_ext
  * the same results bit for bit.  A two-ROI fittype-8 wave (FP32 only) keeps
  * one set of pixel sums per lane, added in another order than the one-ROI
  * kernel's, so its fits differ in the last bits (same accuracy against the
- * FP64 oracle).  FP64 fittype 8 and the 4Pi fitter have one tile. */
+ * FP64 oracle); a launch takes the tile recorded for its largest ROI size,
+ * so with tile 2 recorded for some fittype-8 sizes and not others a ROI's
+ * bits depend on the largest size it is launched with (sessions: on its
+ * frame).  Measured on MI355X one ROI per wave is faster for fittype 8 at
+ * every size, the default.  FP64 fittype 8 and the 4Pi fitter have one
+ * tile. */
 int bsf_tune(const bsf_request *req, bsf_result *res, int reps, int *tile, float *ms, void *stream);
 
 /* The recorded tile for (fittype, precision, sz) (1 or 2), and setting it
