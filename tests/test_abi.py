@@ -137,7 +137,8 @@ def test_front_end_argument_errors_and_host_kernels():
     assert h.shape == (7, 7)
     fr = np.zeros((32, 32, 1), np.uint16)
     for kw, code in ((dict(roi=12), 3), (dict(roi=53), 3), (dict(roi=7, filt=np.zeros((52, 3), np.float32)), 4),
-                     (dict(roi=7, capacity=-1), 1)):
+                     (dict(roi=7, capacity=-1), 1), (dict(roi=7, agg_dist=-1.0), 1),
+                     (dict(roi=7, agg_dist=float("nan")), 1), (dict(roi=7, agg_dist=float("inf")), 1)):
         args = dict(filt=h, cutoff=1.0, capacity=4)
         args.update(kw)
         roi = args.pop("roi")
@@ -153,6 +154,17 @@ def test_front_end_argument_errors_and_host_kernels():
         with pytest.raises(bsfit.BsfError) as e:
             FR.find_rois(fr, 7, h, 1.0, capacity=4)
         assert e.value.code == 6
+    # aggregation's workspace: every maximum a frame can hold (strict maxima
+    # are never adjacent: ceil((n-2)/2) per interior row and column), 12 B of
+    # peak row and 8 B of value each, plus per-frame counts
+    lib = FR._bind()
+    for nr, nc, nf in ((32, 32, 1), (101, 67, 3), (1024, 1024, 2)):
+        q0 = FR.make_request(0, FR.PIXEL_U16, (nr, nc, nf), 7, 0, h.shape, 1.0, 0.0, 1.0, False, False, 0.0, 1, 4)
+        q1 = FR.make_request(0, FR.PIXEL_U16, (nr, nc, nf), 7, 0, h.shape, 1.0, 0.0, 1.0, False, False, 0.0, 1, 4,
+                             13.0)
+        w0, w1 = lib.bsf_find_rois_work_bytes(C.byref(q0)), lib.bsf_find_rois_work_bytes(C.byref(q1))
+        most = nf * ((nr - 1) // 2) * ((nc - 1) // 2)
+        assert w1 - w0 >= 20 * most and w1 - w0 <= 20 * most + 12 * nf + 2048, (nr, nc, nf, w0, w1)
 
 
 def test_oversized_coefficient_arrays_are_refused():

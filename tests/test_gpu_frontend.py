@@ -272,6 +272,16 @@ def test_aggregation_capacity_device_and_fits():
         pk = hip.get(d_pk, cap * 3, np.int32).reshape(cap, 3, order="F")[:k]
         st = hip.get(d_st, roi * roi * cap, np.float32).reshape(roi, roi, cap, order="F")[:, :, :k]
         v = hip.get(d_v, cap, np.float32)[:k]
+        # the allocation-free entry point (its 16 MiB workspace holds these frames' maxima)
+        lib.bsf_find_rois_device_noalloc.argtypes = [C.POINTER(FR.RoiRequest), C.POINTER(FR.RoiResult), C.c_void_p,
+                                                     C.c_void_p]
+        lib.bsf_find_rois_device_noalloc.restype = C.c_int
+        rc = lib.bsf_find_rois_device_noalloc(C.byref(q), C.byref(res), d_n, None)
+        assert rc == 0, lib.bsf_last_error()
+        assert hip.h.hipDeviceSynchronize() == 0
+        assert int(hip.get(d_n, 1, np.int32)[0]) == n
+        np.testing.assert_array_equal(hip.get(d_pk, cap * 3, np.int32).reshape(cap, 3, order="F")[:k], pk)
+        np.testing.assert_array_equal(hip.get(d_v, cap, np.float32)[:k], v)
     finally:
         hip.free(d_fr, d_h, d_w, d_st, d_pk, d_v, d_n)
     want = FR.find_rois(frames, roi, h, capacity=cap, **args)
