@@ -549,8 +549,12 @@ __device__ __forceinline__ void fit_roi(Dev A, const int block, const int nblock
     }
 }
 
-template <int FT, typename E, int SC, int R = 1>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FT == 8 && R == 2 ? 2 : 1)))
+// W2: registers for two waves per SIMD (256) where the kernel would
+// otherwise take more and run one (FP64 fittype 8: 306 registers, or 256
+// with 49 VGPRs spilled, most of them outside the pixel loops; the launcher
+// picks by ROI size); the fittype-8 two-ROI tile always takes it.
+template <int FT, typename E, int SC, int R = 1, int W2 = 0>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FT == 8 && R == 2) || W2 ? 2 : 1)))
 void fit_kernel(Dev A)
 {
     fit_roi<FT, E, SC, R>(A, (int)blockIdx.x, (int)gridDim.x);
@@ -595,11 +599,11 @@ inline Dev fit_dev(const FitLaunch &L)
 }
 
 // nrois ROIs in R-ROI waves (one wave per workgroup, R LDS regions)
-template <int FT, typename E, int SC = 0, int R = 1>
+template <int FT, typename E, int SC = 0, int R = 1, int W2 = 0>
 inline hipError_t launch_t(const Dev &A, int nrois, hipStream_t st)
 {
     const size_t lds = R * lds_bytes(FT, sizeof(E) == 8, A.S);   // A.S = largest ROI size of the launch
-    hipLaunchKernelGGL((fit_kernel<FT, E, SC, R>), dim3((nrois + R - 1) / R), dim3(64), lds, st, A);
+    hipLaunchKernelGGL((fit_kernel<FT, E, SC, R, W2>), dim3((nrois + R - 1) / R), dim3(64), lds, st, A);
     return hipGetLastError();
 }
 

@@ -52,6 +52,21 @@ int set_fit_tile(int fittype, int f64, int S, int tile)
     return 0;
 }
 
+// FP64 fittype 8 runs with registers for two waves per SIMD where its LDS
+// lets a CU (160 KB) hold enough of its waves for the spills to pay: five
+// with the one-pass stage B, six with the two-pass one (whose pixel loop
+// holds the column sums and spills more).  Measured on MI355X (40,000 ROIs
+// per size, ms per launch, two-wave against one-wave registers, same bits):
+// S = 9: 20.0 / 27.0, 11: 29.3 / 33.3, 12: 38.4 / 36.2 (five waves per CU),
+// 13: 42.8 / 35.5 (four), 15: 38.8 / 47.4, 16: 50.2 / 50.1, 17: 50.4 /
+// 53.5; 100,000 ROIs of sizes 11-17 in one launch: 100.3 / 103.3.
+constexpr size_t kCuLds = 160 * 1024;
+inline bool f64_two_waves(int S)
+{
+    const size_t waves = stage_b_split(8, true, S) ? 6 : 5;
+    return waves * lds_bytes(8, 1, S) <= kCuLds;
+}
+
 hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
 {
     const Dev A = fit_dev(L);
@@ -76,7 +91,7 @@ hipError_t launch_fit(const FitLaunch &L, hipStream_t st)
     case 14: return sized_kernel(7, sized) ? launch_sized7(A, nb, st, sized) : launch_t<7, float>(A, nb, st);
     case 15: return sized_kernel_f64(7, sized) ? launch_sized7d(A, nb, st, sized) : launch_t<7, double>(A, nb, st);
     case 16: return launch_t<8, float>(A, nb, st);
-    case 17: return launch_t<8, double>(A, nb, st);
+    case 17: return f64_two_waves(L.S) ? launch_t<8, double, 0, 1, 1>(A, nb, st) : launch_t<8, double>(A, nb, st);
     default: return hipErrorInvalidValue;
     }
 }
