@@ -366,13 +366,15 @@ def make_rois(fittype, N, S, seed, coeff, coeff3=None, noise=True, iterations=50
                     truth=truth, initP=initP, iterations=iterations, mu=mu)
 
 
-def config(idx, N=None, S=None, noise=True, seed=None):
+def config(idx, N=None, S=None, noise=True, seed=None, bg_per_channel=False):
     """Workloads for the documented configurations.
 
     0: 3D astigmatic, 27x27x61 grid, S=13, N=10,000, fittype 6
     1: colour 4D, 27x27x61x3 grid, S=15, N=100,000, fittype 7
     2: 4Pi, four 13x13 channels (phase offsets 0, pi/2, pi, 3pi/2), 27x27x61x18
-       grid (16 phase samples + 2 wrap samples), N=200,000, all shared (NV=6)
+       grid (16 phase samples + 2 wrap samples), N=200,000, all shared (NV=6);
+       bg_per_channel: a background of its own per channel, drawn and fitted
+       per channel (x, y, photons, z, phase shared: NV=9)
     3: vortex 5D, 21x21x21x17x9 grid (+21^3 averaged model), S in
        {11,13,15,17}, N=100,000, fittype 8 (one Workload per S group)
     """
@@ -382,7 +384,8 @@ def config(idx, N=None, S=None, noise=True, seed=None):
     if idx == 1:
         return make_rois(7, N or 100000, S or 15, seed, model_coeffs("color4d"), noise=noise)
     if idx == 2:
-        return make_rois_4pi(N or 200000, S or 13, seed, model_coeffs("4pi"), noise=noise)
+        shared = (1, 1, 1, 0, 1, 1) if bg_per_channel else (1, 1, 1, 1, 1, 1)
+        return make_rois_4pi(N or 200000, S or 13, seed, model_coeffs("4pi"), shared=shared, noise=noise)
     if idx == 3:
         c5, c3 = model_coeffs("dipole5d")
         Ntot = N or 100000
