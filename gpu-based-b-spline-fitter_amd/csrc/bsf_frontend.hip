@@ -436,6 +436,12 @@ __global__ __launch_bounds__(64) void cut_kernel(CutArgs A)
 // rounded square root is below agg_dist (agg_d2max, on the host).  After a
 // merge into ii the first close pair can only lie in column ii or later (the
 // pairs of earlier columns are unchanged), so the search resumes there.
+// Rows keep their first index for the whole merge (a deleted row is marked,
+// x = -1, and dropped when the survivors are packed), so "ii < jj" compares
+// indices; each frame's maxima sit in lists of cs x cs cells (cs >= the
+// largest |dx| or |dy| of a close pair), so the first ii of column jj is the
+// smallest index in the 3 x 3 cells around it instead of a scan of every
+// earlier row.
 struct AggArgs {
     const void *raw;
     int nr, nc;
@@ -451,7 +457,47 @@ struct AggArgs {
     double psf_half;      // > 0: the PSF-convolution mode's border test, after the merge
     int *fkept, *foff;    // per frame: maxima kept (both; foff is then scanned to offsets)
     int *fseg;            // per frame: first index in cand
+    int *cnext;           // [ccap]: the next maximum in the same cell, -1 ends the list
+    int *chead;           // [frames, gx, gy]: the first maximum of each cell, -1: none
+    int cs, gx, gy;       // cell size; cells along x (columns) and y (rows)
 };
+
+// The cell heads are built with atomics, which act at L2, so every access
+// to them goes to L2 (agent-scope atomic loads and stores) and none can see
+// a stale line in the CU's vector cache.
+__device__ __forceinline__ int head_load(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void head_store(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Smallest index below `bound` among the maxima of the 3 x 3 cells around
+// (x, y) whose squared distance d2 from it is in (0, d2max]; bound if none.
+__device__ __forceinline__ int first_close(const AggArgs &A, const int *xs, const int *ys, const int *cnext,
+                                           const int *chead, int x, int y, int bound)
+{
+    const int cx = x / A.cs, cy = y / A.cs;
+    const int a0 = cx > 0 ? cx - 1 : 0, a1 = cx + 1 < A.gx ? cx + 1 : A.gx - 1;
+    const int b0 = cy > 0 ? cy - 1 : 0, b1 = cy + 1 < A.gy ? cy + 1 : A.gy - 1;
+    for (int a = a0; a <= a1; a++)
+        for (int b = b0; b <= b1; b++)
+            for (int k = head_load(&chead[a * A.gy + b]); k >= 0; k = cnext[k]) {
+                if (k >= bound) continue;
+                const long long dxv = xs[k] - x, dyv = ys[k] - y;
+                const long long d2 = dxv * dxv + dyv * dyv;
+                if (d2 > 0 && d2 <= A.d2max) bound = k;
+            }
+    return bound;
+}
+
+// Take maximum k out of the list of cell c (one thread).
+__device__ __forceinline__ void cell_unlink(int *cnext, int *chead, int c, int k)
+{
+    int q = head_load(&chead[c]);
+    if (q == k) {
+        head_store(&chead[c], cnext[k]);
+        return;
+    }
+    while (q >= 0 && cnext[q] != k) q = cnext[q];   // (k is always in the list)
+    if (q >= 0) cnext[q] = cnext[k];
+}
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void aggregate_kernel(AggArgs A)
@@ -482,26 +528,30 @@ __global__ __launch_bounds__(kThreads) void aggregate_kernel(AggArgs A)
             vs[k] = (double)filtered_at<T>(frm, A.nr, A.nc, A.offset, A.conv, A.h, A.mr, A.mc,
                                            A.sub ? A.sub + f : nullptr, xs[k] - 1, ys[k] - 1);
     }
+    // every maximum into the list of its cell (no merge possible: d2max < 1)
+    const bool merge = A.d2max >= 1;
+    int *cnext = A.cnext + lo;
+    int *chead = A.chead + (size_t)f * A.gx * A.gy;
+    auto cell = [&](int x, int y) { return (x / A.cs) * A.gy + y / A.cs; };
+    if (merge) {
+        for (int c = tid; c < A.gx * A.gy; c += kThreads) head_store(&chead[c], -1);
+        __syncthreads();
+        for (int k = tid; k < n; k += kThreads) cnext[k] = atomicExch(&chead[cell(xs[k], ys[k])], k);
+        __syncthreads();
+    }
     __shared__ unsigned long long best;
     int start = 1;
-    for (;;) {
+    while (merge) {
         if (tid == 0) best = ~0ull;
         __syncthreads();
         // each thread takes columns jj = start + tid, + 256, ... and stops at
         // its first close pair (or once a smaller column has one)
         for (int jj = start + tid; jj < n; jj += kThreads) {
             if (((unsigned long long)jj << 32) > *(volatile unsigned long long *)&best) break;
-            const int xj = xs[jj], yj = ys[jj];
-            int hit = -1;
-            for (int ii = 0; ii < jj; ii++) {
-                const long long dxv = xs[ii] - xj, dyv = ys[ii] - yj;
-                const long long d2 = dxv * dxv + dyv * dyv;
-                if (d2 > 0 && d2 <= A.d2max) {
-                    hit = ii;
-                    break;
-                }
-            }
-            if (hit >= 0) {
+            const int xj = xs[jj];
+            if (xj < 0) continue;   // merged away
+            const int hit = first_close(A, xs, ys, cnext, chead, xj, ys[jj], jj);
+            if (hit < jj) {
                 atomicMin(&best, ((unsigned long long)jj << 32) | (unsigned)hit);
                 break;
             }
@@ -511,36 +561,31 @@ __global__ __launch_bounds__(kThreads) void aggregate_kernel(AggArgs A)
         if (b == ~0ull) break;
         const int jj = (int)(b >> 32), ii = (int)(b & 0xffffffffu);
         if (tid == 0) {
-            xs[ii] = (int)round(((double)xs[ii] + (double)xs[jj]) / 2.0);
-            ys[ii] = (int)round(((double)ys[ii] + (double)ys[jj]) / 2.0);
+            const int xo = xs[ii], yo = ys[ii];
+            const int xn = (int)round(((double)xo + (double)xs[jj]) / 2.0);
+            const int yn = (int)round(((double)yo + (double)ys[jj]) / 2.0);
             if (vs) vs[ii] = round((vs[ii] + vs[jj]) / 2.0);
-        }
-        // delete row jj: rows jj+1.. move down by one, read then written per chunk
-        for (int k0 = jj; k0 < n - 1; k0 += kThreads) {
-            const int k = k0 + tid;
-            int tx = 0, ty = 0;
-            double tv = 0.0;
-            if (k < n - 1) {
-                tx = xs[k + 1];
-                ty = ys[k + 1];
-                if (vs) tv = vs[k + 1];
+            cell_unlink(cnext, chead, cell(xs[jj], ys[jj]), jj);
+            xs[jj] = -1;
+            const int co = cell(xo, yo), cn = cell(xn, yn);
+            if (co != cn) {
+                cell_unlink(cnext, chead, co, ii);
+                cnext[ii] = head_load(&chead[cn]);
+                head_store(&chead[cn], ii);
             }
-            __syncthreads();
-            if (k < n - 1) {
-                xs[k] = tx;
-                ys[k] = ty;
-                if (vs) vs[k] = tv;
-            }
+            xs[ii] = xn;
+            ys[ii] = yn;
         }
         __syncthreads();
-        n--;
         start = ii > 1 ? ii : 1;
     }
-    // the PSF-convolution mode's border test (simplefitter_spline.m:284-295,
-    // rows used for both axes) on the merged maxima, kept in order in place
+    // the survivors packed in order, with the PSF-convolution mode's border
+    // test (simplefitter_spline.m:284-295, rows used for both axes) after the
+    // merge
     int kept = n;
-    if (A.psf_half > 0.0) {
+    if (merge || A.psf_half > 0.0) {
         const double hp = A.psf_half, lim = (double)A.nr - hp;
+        const bool psf = hp > 0.0;
         int base = 0;
         for (int k0 = 0; k0 < n; k0 += kThreads) {
             const int k = k0 + tid;
@@ -551,7 +596,8 @@ __global__ __launch_bounds__(kThreads) void aggregate_kernel(AggArgs A)
                 tx = xs[k];
                 ty = ys[k];
                 if (vs) tv = vs[k];
-                keep = !((double)tx < hp || (double)tx > lim || (double)ty < hp || (double)ty > lim);
+                keep = tx >= 0 &&
+                       !(psf && ((double)tx < hp || (double)tx > lim || (double)ty < hp || (double)ty > lim));
             }
             int cnt;
             const int pos = base + block_exclusive_scan(keep ? 1 : 0, &cnt);
@@ -593,46 +639,6 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(const int *__restrict_
 
 __global__ void zero_count_kernel(int *count) { *count = 0; }
 
-struct WorkLayout {
-    size_t sub, mask, bsum, total;
-    // with aggregation: every maximum of every frame (ccap: the most a frame
-    // of nr x nc can hold, strict maxima being never adjacent), their values,
-    // their count and the per-frame kept counts, offsets and first indices
-    size_t cand, cval, ccount, fkept, foff, fseg;
-    long W;
-    int nb, wpc, ccap;
-};
-
-size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-
-WorkLayout work_layout(int nr, int nc, int nframes, bool agg)
-{
-    WorkLayout w;
-    w.wpc = (nr + kTR - 1) / kTR;
-    w.W = (long)nframes * nc * w.wpc;
-    w.nb = (int)((w.W + kCB - 1) / kCB);
-    w.sub = 0;
-    w.mask = align256(sizeof(float) * (size_t)nframes);
-    w.bsum = w.mask + align256(sizeof(uint64_t) * (size_t)w.W);
-    w.total = w.bsum + align256(sizeof(int) * (size_t)(w.nb > 0 ? w.nb : 1));
-    w.ccap = 0;
-    w.cand = w.cval = w.ccount = w.fkept = w.foff = w.fseg = w.total;
-    if (agg) {
-        const size_t per = (size_t)((nr - 1) / 2) * (size_t)((nc - 1) / 2);
-        const size_t cc = per * (size_t)(nframes > 0 ? nframes : 0);
-        w.ccap = (int)cc;   // below 2^31 (the entry points check the bound)
-        const size_t nf = (size_t)(nframes > 0 ? nframes : 1);
-        w.cand = w.total;
-        w.cval = w.cand + align256(sizeof(int) * 3 * (cc ? cc : 1));
-        w.ccount = w.cval + align256(sizeof(double) * (cc ? cc : 1));
-        w.fkept = w.ccount + 256;
-        w.foff = w.fkept + align256(sizeof(int) * nf);
-        w.fseg = w.foff + align256(sizeof(int) * nf);
-        w.total = w.fseg + align256(sizeof(int) * nf);
-    }
-    return w;
-}
-
 // The largest integer d2 with sqrt(d2) < thr (correctly rounded, as MATLAB
 // computes the distance of two integer points), or -1 when thr <= 0; capped
 // above the largest squared distance within an nr x nc frame.
@@ -644,6 +650,74 @@ long long agg_d2max(double thr, int nr, int nc)
     long long k = t2 > lim ? (long long)lim + 1 : (long long)t2 + 2;
     while (k > 0 && !(std::sqrt((double)k) < thr)) k--;
     return k;
+}
+
+// The merge's cells: cs x cs pixels, cs at least the largest |dx| of a
+// close pair (the integer square root of d2max) and at least 4 (the lists'
+// heads then take at most a quarter of a byte per pixel).
+struct AggGrid {
+    long long d2max;
+    int cs, gx, gy;
+};
+
+AggGrid agg_grid(double thr, int nr, int nc)
+{
+    AggGrid g;
+    g.d2max = agg_d2max(thr, nr, nc);
+    long long r = g.d2max > 0 ? (long long)std::sqrt((double)g.d2max) : 0;
+    while (r * r > g.d2max) r--;
+    while ((r + 1) * (r + 1) <= g.d2max) r++;
+    g.cs = (int)(r > 4 ? r : 4);
+    g.gx = nc / g.cs + 1;
+    g.gy = nr / g.cs + 1;
+    return g;
+}
+
+struct WorkLayout {
+    size_t sub, mask, bsum, total;
+    // with aggregation: every maximum of every frame (ccap: the most a frame
+    // of nr x nc can hold, strict maxima being never adjacent), their values,
+    // their count, the per-frame kept counts, offsets and first indices, and
+    // the merge's cell lists (a link per maximum, gx x gy heads per frame)
+    size_t cand, cval, ccount, fkept, foff, fseg, cnext, chead;
+    long W;
+    int nb, wpc, ccap;
+    AggGrid grid;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+WorkLayout work_layout(int nr, int nc, int nframes, double agg_dist)
+{
+    const bool agg = agg_dist > 0.0;
+    WorkLayout w;
+    w.wpc = (nr + kTR - 1) / kTR;
+    w.W = (long)nframes * nc * w.wpc;
+    w.nb = (int)((w.W + kCB - 1) / kCB);
+    w.sub = 0;
+    w.mask = align256(sizeof(float) * (size_t)nframes);
+    w.bsum = w.mask + align256(sizeof(uint64_t) * (size_t)w.W);
+    w.total = w.bsum + align256(sizeof(int) * (size_t)(w.nb > 0 ? w.nb : 1));
+    w.ccap = 0;
+    w.cand = w.cval = w.ccount = w.fkept = w.foff = w.fseg = w.cnext = w.chead = w.total;
+    w.grid = AggGrid{-1, 4, 1, 1};
+    if (agg) {
+        w.grid = agg_grid(agg_dist, nr, nc);
+        const size_t per = (size_t)((nr - 1) / 2) * (size_t)((nc - 1) / 2);
+        const size_t cc = per * (size_t)(nframes > 0 ? nframes : 0);
+        w.ccap = (int)cc;   // below 2^31 (the entry points check the bound)
+        const size_t nf = (size_t)(nframes > 0 ? nframes : 1);
+        w.cand = w.total;
+        w.cval = w.cand + align256(sizeof(int) * 3 * (cc ? cc : 1));
+        w.ccount = w.cval + align256(sizeof(double) * (cc ? cc : 1));
+        w.fkept = w.ccount + 256;
+        w.foff = w.fkept + align256(sizeof(int) * nf);
+        w.fseg = w.foff + align256(sizeof(int) * nf);
+        w.cnext = w.fseg + align256(sizeof(int) * nf);
+        w.chead = w.cnext + align256(sizeof(int) * (cc ? cc : 1));
+        w.total = w.chead + align256(sizeof(int) * nf * (size_t)w.grid.gx * (size_t)w.grid.gy);
+    }
+    return w;
 }
 
 using DetectFn = void (*)(DetectArgs);
@@ -660,7 +734,7 @@ template <typename T>
 hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
 {
     const bool agg = L.agg_dist > 0.0;
-    const WorkLayout w = work_layout(L.nr, L.nc, L.nframes, agg);
+    const WorkLayout w = work_layout(L.nr, L.nc, L.nframes, L.agg_dist);
     char *work = (char *)L.work;
     float *sub = L.subtract_min ? (float *)(work + w.sub) : nullptr;
     uint64_t *mask = (uint64_t *)(work + w.mask);
@@ -731,7 +805,12 @@ hipError_t launch_typed(const FrontLaunch &L, hipStream_t st)
         G.ccount = ccount;
         G.ccap = w.ccap;
         G.first_frame = L.first_frame;
-        G.d2max = agg_d2max(L.agg_dist, L.nr, L.nc);
+        G.d2max = w.grid.d2max;
+        G.cnext = (int *)(work + w.cnext);
+        G.chead = (int *)(work + w.chead);
+        G.cs = w.grid.cs;
+        G.gx = w.grid.gx;
+        G.gy = w.grid.gy;
         G.psf_half = psf_half;
         G.fkept = fkept;
         G.foff = foff;
@@ -775,7 +854,10 @@ size_t front_detect_lds_bytes(int mr, int mc)
     return sizeof(float) * ((size_t)XC * (XR | 1) + (size_t)kFC * kFLD);
 }
 
-size_t front_work_bytes(int nr, int nc, int nframes, bool agg) { return work_layout(nr, nc, nframes, agg).total; }
+size_t front_work_bytes(int nr, int nc, int nframes, double agg_dist)
+{
+    return work_layout(nr, nc, nframes, agg_dist).total;
+}
 
 // Workspace of the allocation-free entry point (bsf_find_rois_device_noalloc):
 // one array per GPU for the life of the process.  Nothing in it needs to

@@ -58,7 +58,7 @@ int validate_rois(const bsf_roi_request *r, const bsf_roi_result *res)
 // workspace of a request (aggregation holds every maximum of the frames)
 size_t work_bytes_of(const bsf_roi_request *r)
 {
-    return bsf::front_work_bytes(r->nrows, r->ncols, r->nframes, r->agg_dist > 0.0);
+    return bsf::front_work_bytes(r->nrows, r->ncols, r->nframes, r->agg_dist);
 }
 
 bsf::FrontLaunch make_front(const bsf_roi_request *r, const void *frames, const float *h, float *stack, int *peaks,

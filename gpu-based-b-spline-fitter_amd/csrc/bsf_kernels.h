@@ -103,7 +103,7 @@ size_t coeff_work_bytes(const int *dims, int ndim);
 // pixel_type 0 = uint16, 1 = float32); h [mr, mc] the correlation kernel
 // (device); outputs on the device: stack [S, S, cap], peaks [cap, 3] (x, y,
 // frame), value [cap] or NULL, count (one int: the ROIs found, possibly more
-// than cap); work is device scratch of front_work_bytes(nr, nc, nframes).
+// than cap); work is device scratch of front_work_bytes(nr, nc, nframes, agg_dist).
 struct FrontLaunch {
     const void *frames;
     int pixel_type;
@@ -124,7 +124,7 @@ struct FrontLaunch {
     void *work;
 };
 hipError_t launch_find_rois(const FrontLaunch &L, hipStream_t st);
-size_t front_work_bytes(int nr, int nc, int nframes, bool agg);
+size_t front_work_bytes(int nr, int nc, int nframes, double agg_dist);   // agg_dist <= 0: no merge
 // The allocation-free front-end's workspace: a file-scope device array of
 // kFrontScratchBytes per GPU; front_scratch returns the current device's
 // address of it (looked up once per device).

@@ -515,10 +515,11 @@ typedef struct bsf_roi_request {
     int capacity;           /* ROIs the outputs hold */
     double agg_dist;        /* p.agg: p.roifit * p.aggsz, maxima closer than this are
                                merged (aggre); 0 = off.  Aggregation enlarges the
-                               workspace (bsf_find_rois_work_bytes: ~20 B per 4 pixels
-                               of all frames) and, as the reference's loop, searches
-                               O(n^2) pairs per merge for a frame of n maxima: meant
-                               for the tens to thousands a frame has above a cutoff */
+                               workspace (bsf_find_rois_work_bytes: ~24 B per 4 pixels
+                               of all frames, plus at most 1/4 B per pixel of cell
+                               heads); the search for the next pair looks only at the
+                               maxima of the neighbouring cells, so dense frames (a
+                               low cutoff) stay linear in their maxima per merge */
 } bsf_roi_request;
 
 typedef struct bsf_roi_result {
@@ -547,7 +548,7 @@ int bsf_find_rois_device(const bsf_roi_request *req, bsf_roi_result *res, int *c
  * device array inside the library, 16 MiB per GPU, held until the process
  * ends: requests needing more than that (bsf_find_rois_work_bytes; about
  * 8 B per 64 pixels of all frames, e.g. up to 120 frames of 1024 x 1024;
- * with aggregation about 5 B per pixel, e.g. 3 such frames) return
+ * with aggregation about 6.3 B per pixel, e.g. 2 such frames) return
  * BSF_ERR_NOMEM.  The call allocates nothing.  Calls that overlap on
  * one GPU share that array and race: serialise them (one stream, or events). */
 int bsf_find_rois_device_noalloc(const bsf_roi_request *req, bsf_roi_result *res, int *count, void *stream);

@@ -156,15 +156,22 @@ def test_front_end_argument_errors_and_host_kernels():
         assert e.value.code == 6
     # aggregation's workspace: every maximum a frame can hold (strict maxima
     # are never adjacent: ceil((n-2)/2) per interior row and column), 12 B of
-    # peak row and 8 B of value each, plus per-frame counts
+    # peak row, 8 B of value and a 4 B cell link each, per-frame counts, and
+    # per frame the heads of the merge's cells: cs x cs pixels, cs the
+    # integer square root of the largest d2 with sqrt(d2) < agg_dist, at
+    # least 4 (agg_dist 13: d2 <= 168, cs 12; 2.5: d2 <= 6, cs 4)
     lib = FR._bind()
     for nr, nc, nf in ((32, 32, 1), (101, 67, 3), (1024, 1024, 2)):
         q0 = FR.make_request(0, FR.PIXEL_U16, (nr, nc, nf), 7, 0, h.shape, 1.0, 0.0, 1.0, False, False, 0.0, 1, 4)
-        q1 = FR.make_request(0, FR.PIXEL_U16, (nr, nc, nf), 7, 0, h.shape, 1.0, 0.0, 1.0, False, False, 0.0, 1, 4,
-                             13.0)
-        w0, w1 = lib.bsf_find_rois_work_bytes(C.byref(q0)), lib.bsf_find_rois_work_bytes(C.byref(q1))
+        w0 = lib.bsf_find_rois_work_bytes(C.byref(q0))
         most = nf * ((nr - 1) // 2) * ((nc - 1) // 2)
-        assert w1 - w0 >= 20 * most and w1 - w0 <= 20 * most + 12 * nf + 2048, (nr, nc, nf, w0, w1)
+        for dist, cs in ((13.0, 12), (2.5, 4)):
+            q1 = FR.make_request(0, FR.PIXEL_U16, (nr, nc, nf), 7, 0, h.shape, 1.0, 0.0, 1.0, False, False, 0.0, 1,
+                                 4, dist)
+            w1 = lib.bsf_find_rois_work_bytes(C.byref(q1))
+            heads = 4 * nf * (nc // cs + 1) * (nr // cs + 1)
+            lo = 24 * most + heads
+            assert lo <= w1 - w0 <= lo + 12 * nf + 2304, (nr, nc, nf, dist, w0, w1)
 
 
 def test_oversized_coefficient_arrays_are_refused():

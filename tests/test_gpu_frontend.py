@@ -301,6 +301,55 @@ def test_aggregation_capacity_device_and_fits():
     np.testing.assert_array_equal(LL, LL2)
 
 
+@pytest.mark.parametrize("thr,psf", [(4.0, 0.0), (6.5, 27.0), (20.0, 0.0)])
+def test_dense_aggregation_matches_oracle(thr, psf):
+    """Every strict maximum of noisy frames (a cutoff far below the noise):
+    ~670 per frame, merging in chains that carry merged maxima across the
+    merge's cells (4, 6 and 19 pixels wide); counts, rows, ROIs and values
+    equal the oracle's bit for bit."""
+    frames, _ = gen.make_frames(2, 128, 144, 20, 7100, dtype=np.uint16)
+    h = FR.peak_filter("dog", 1.0)
+    args = dict(cutoff=-1e9, offset=100.0, conversion=0.5, subtract_min=True, psf_size=psf, first_frame=1,
+                capacity=20000)
+    plain = FR.find_rois(frames, 9, h, **args)
+    got = FR.find_rois(frames, 9, h, agg_dist=thr, **args)
+    want = oracle.find_rois(frames, 9, h, agg_dist=thr, **args)
+    print("dense agg %g psf %g: %d maxima, %d after aggregation" % (thr, psf, plain[3], got[3]), flush=True)
+    assert plain[3] > 500 and got[3] < plain[3]    # (the PSF border test leaves 820 of the 1,345)
+    _compare(got, want)
+
+
+def test_dense_aggregation_large_frames():
+    """Two 1024 x 1024 frames with every strict maximum kept (tens of
+    thousands per frame) and agg_dist 5: too many maxima for the oracle's
+    literal search, so the result is checked for what aggre's loop leaves --
+    no two maxima of a frame at a distance 0 < D < 5, each survivor within
+    the frame, no more survivors than maxima -- and the merge, whose next
+    pair is searched among the neighbouring cells only, ends in seconds."""
+    import time
+    frames, _ = gen.make_frames(2, 1024, 1024, 400, 7200, dtype=np.uint16)
+    h = FR.peak_filter("dog", 1.0)
+    args = dict(cutoff=-1e9, offset=100.0, conversion=0.5, subtract_min=True, first_frame=1, capacity=400000)
+    plain = FR.find_rois(frames, 3, h, **args)
+    t0 = time.perf_counter()
+    got = FR.find_rois(frames, 3, h, agg_dist=5.0, **args)
+    dt = time.perf_counter() - t0
+    n = got[3]
+    print("large dense agg: %d maxima, %d after aggregation, %.2f s" % (plain[3], n, dt), flush=True)
+    assert plain[3] > 50000 and 0 < n < plain[3]
+    assert dt < 20.0
+    pk = got[1][:n]
+    offs = [(a, b) for a in range(-4, 5) for b in range(-4, 5) if 0 < a * a + b * b <= 24]
+    for f in (1, 2):
+        q = pk[pk[:, 2] == f]
+        x, y = q[:, 0], q[:, 1]
+        assert x.min() >= 1 and x.max() <= 1024 and y.min() >= 1 and y.max() <= 1024
+        occ = np.zeros((1024 + 10, 1024 + 10), bool)
+        occ[x + 4, y + 4] = True
+        for a, b in offs:
+            assert not occ[x + 4 + a, y + 4 + b].any(), (f, a, b)
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_random_aggregation_matches_oracle(seed):
     """Random frames, kernels, ROI sizes, PSF borders and thresholds with
