@@ -621,8 +621,9 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
 // Where it is used: FP32 fittypes 6/7 always (a kernel that also holds the
 // one-pass form needs more than 128 VGPRs for fittype 7); FP64 fittypes 6/7
 // up to S = 15 (2 waves per SIMD: the extra LDS leaves occupancy as it is);
-// FP64 fittype 8 up to S = 13 (1 wave per SIMD; the kernel holding both
-// forms also needs 310 instead of 382 registers, which speeds up every S).
+// FP64 fittype 8 up to S = 13 (its LDS and registers hold one or two waves
+// per SIMD either way; the kernel holding both forms also needed 310
+// instead of 382 registers, which sped up every S).
 // FP32 fittype 8 keeps one pass: two gained nothing at S <= 13 and the
 // kernel holding both lost 3 % at S >= 15.  Elsewhere pixel_model does one
 // pass.  (Keeping the sums in registers and writing them over the planes

@@ -28,7 +28,8 @@ int default_tile(int fittype, int f64, int S) { return fittype == 6 && !f64 && S
 }  // namespace
 
 // two ROIs per wave: fittypes 6 and 7, and fittype 8 in FP32 (in FP64 its
-// registers and LDS already hold one ROI per SIMD)
+// registers and LDS hold one or two waves of one ROI per SIMD, no room for
+// a second ROI per wave)
 int tile_possible(int fittype, int f64, int S, int tile)
 {
     if (tile == 1) return 1;
