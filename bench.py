@@ -757,13 +757,25 @@ def fit_bench(args):
             for prec in ("mix", "f64"):
                 if m == model and prec == args.precision and args.n is None and args.roi_size is None:
                     rr = r
+                    groups = head.groups
                 else:
                     c = FitCase(m, precision=prec)
                     rr = c.run(3, 2)
+                    groups = c.groups
                     del c
                     torch.cuda.empty_cache()
                 per[m][prec] = {"fits_per_s": rr["fits_per_s"], "kernel_fits_per_s": rr["kernel_fits_per_s"],
                                 "frac_fp32": rr["frac"], "mean_evals": rr["mean_evals"]}
+                # the reference CPU fitter (restated by the oracle) on a bounded
+                # sample of the same ROIs, for the 3D and colour-4D
+                # configurations (the 4Pi fitter has no CPU path in the
+                # reference; vortex-5D is the line's own cpu_baseline)
+                if prec == "mix" and m in ("3d", "colour4d") and not args.no_cpu:
+                    th = cpu_threads()
+                    v_all, samp = cpu_baseline(groups, th, budget_s=4.0)
+                    v_one, _ = cpu_baseline(groups, 1, budget_s=2.0)
+                    per[m]["cpu"] = {"fits_per_s": v_all, "cores": th, "fits_per_s_1core": v_one, "kind": "port",
+                                     "sample": samp}
             print("bench: %s MIX %.3g fits/s (frac %.3f), F64 %.3g fits/s" % (
                 m, per[m]["mix"]["fits_per_s"], per[m]["mix"]["frac_fp32"], per[m]["f64"]["fits_per_s"]),
                 file=sys.stderr, flush=True)
