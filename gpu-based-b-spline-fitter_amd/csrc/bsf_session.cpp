@@ -582,8 +582,8 @@ int submit_eager(bsf_session *s, const bsf_frame *f, int flags, unsigned long lo
             if (het.empty() || het.back().size() == (size_t)bsf::kHeteroMax) het.emplace_back();
             het.back().push_back(i);
         }
-        for (auto &h : het)
-            for (size_t q = 1; q < h.size(); q++) L[h[0]].cost += L[h[q]].cost, L[h[q]].cost = -1;
+        for (auto &grp : het)
+            for (size_t q = 1; q < grp.size(); q++) L[grp[0]].cost += L[grp[q]].cost, L[grp[q]].cost = -1;
     }
     for (int i : ord) {
         if (e != hipSuccess) break;
@@ -597,16 +597,16 @@ int submit_eager(bsf_session *s, const bsf_frame *f, int flags, unsigned long lo
         l.stream = j;
         hipStream_t ks = s->ks[j];
         e = hipStreamWaitEvent(ks, ev_in, 0);
-        const std::vector<int> *h = nullptr;
+        const std::vector<int> *grp = nullptr;
         for (const auto &x : het)
-            if (x[0] == i && x.size() > 1) h = &x;
-        if (e == hipSuccess && h) {
+            if (x[0] == i && x.size() > 1) grp = &x;
+        if (e == hipSuccess && grp) {
             bsf::FitLaunch K[bsf::kHeteroMax];
-            for (size_t q = 0; q < h->size(); q++) {
-                const Launch &m = L[(*h)[q]];
+            for (size_t q = 0; q < grp->size(); q++) {
+                const Launch &m = L[(*grp)[q]];
                 K[q] = fit_launch(s, m, f64, d, (const float *)(d + m.o_data), nullptr);
             }
-            e = bsf::launch_fit_hetero(K, (int)h->size(), ks);
+            e = bsf::launch_fit_hetero(K, (int)grp->size(), ks);
         } else if (e == hipSuccess) {
             e = launch_one(s, l, f64, d, (const float *)(d + l.o_data), nullptr, ks);
         }
