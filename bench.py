@@ -818,9 +818,13 @@ def fit_bench(args):
                              "157.3 TFLOP/s (counts packed FP32, which this build does not issue); achieved = "
                              "sum over ROIs of the evaluations each fit ran (kernel counter, mean %.2f) x "
                              "F_alg(S) (%s flop per ROI-evaluation) / kernel time (HIP events on the launch "
-                             "stream); traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                             "copy-calibrated)" % (r["mean_evals"], ", ".join(
-                                 "S=%d: %d" % (S, f) for S, f in sorted(r["f_alg"].items())))},
+                             "stream); traffic = bytes per launch beyond L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                             "copy-calibrated)%s" % (r["mean_evals"], ", ".join(
+                                 "S=%d: %d" % (S, f) for S, f in sorted(r["f_alg"].items())),
+                                 "; for vortex-5D these are the coefficient gathers that miss an XCD's 4 MB L2 "
+                                 "(the 13.3 MB 5D array is re-gathered every evaluation), served by the Infinity "
+                                 "Cache or HBM, against 93.6 MB of algorithmic bytes (ROI data, outputs, the "
+                                 "coefficient arrays once)" if model == "vortex5d" else "")},
         "cpu_baseline": cpu,
     }
     out.update(extra)
