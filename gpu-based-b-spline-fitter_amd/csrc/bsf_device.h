@@ -621,9 +621,8 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
 // Where it is used: FP32 fittypes 6/7 always (a kernel that also holds the
 // one-pass form needs more than 128 VGPRs for fittype 7); FP64 fittypes 6/7
 // up to S = 15 (2 waves per SIMD: the extra LDS leaves occupancy as it is);
-// FP64 fittype 8 up to S = 13 (its LDS and registers hold one or two waves
-// per SIMD either way; the kernel holding both forms also needed 310
-// instead of 382 registers, which sped up every S).
+// FP64 fittype 8 nowhere (kSplit8MaxF64), though its kernel holds the form
+// for the registers it saves.
 // FP32 fittype 8 keeps one pass: two gained nothing at S <= 13 and the
 // kernel holding both lost 3 % at S >= 15.  Elsewhere pixel_model does one
 // pass.  (Keeping the sums in registers and writing them over the planes
@@ -631,9 +630,18 @@ __device__ __forceinline__ void pixel_model(const Geo<E, FT> &g, const E *planes
 template <int FT>
 struct ColSums { static constexpr int NHP = FT == 8 ? 8 : 4; };
 
+// FP64 fittype 8 takes the two-pass form up to this ROI size: none.  Its
+// kernel still holds that form (the size is a run-time value): with it the
+// compiler allocates 306 instead of 382 registers, and 256 with 49 VGPRs
+// spilled instead of 155 in the two-wave variant.  Measured with the
+// two-wave registers (40,000 ROIs, ms per launch, one pass / two passes,
+// same bits): S = 9 20.2 / 20.0, 11 23.0 / 29.3, 12 27.5 / 36.4, 13 23.7 /
+// 35.3.
+constexpr int kSplit8MaxF64 = 0;
+
 __host__ __device__ inline bool stage_b_split(int ft, bool f64, int S)
 {
-    if (ft == 8) return f64 && S <= 13;
+    if (ft == 8) return f64 && S <= kSplit8MaxF64;
     return !f64 || S <= 15;
 }
 

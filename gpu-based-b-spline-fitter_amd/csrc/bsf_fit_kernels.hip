@@ -55,12 +55,12 @@ int set_fit_tile(int fittype, int f64, int S, int tile)
 
 // FP64 fittype 8 runs with registers for two waves per SIMD where its LDS
 // lets a CU (160 KB) hold enough of its waves for the spills to pay: five
-// with the one-pass stage B, six with the two-pass one (whose pixel loop
-// holds the column sums and spills more).  Measured on MI355X (40,000 ROIs
-// per size, ms per launch, two-wave against one-wave registers, same bits):
-// S = 9: 20.0 / 27.0, 11: 29.3 / 33.3, 12: 38.4 / 36.2 (five waves per CU),
-// 13: 42.8 / 35.5 (four), 15: 38.8 / 47.4, 16: 50.2 / 50.1, 17: 50.4 /
-// 53.5; 100,000 ROIs of sizes 11-17 in one launch: 100.3 / 103.3.
+// with the one-pass stage B (up to S = 17), six with the two-pass one
+// (which fittype 8 no longer takes, kSplit8MaxF64).  Measured on MI355X
+// (40,000 ROIs per size, ms per launch, two-wave against one-wave
+// registers, same bits): S = 9: 20.0 / 27.0, 11: 29.3 / 33.3, 15: 38.8 /
+// 47.4, 16: 50.2 / 50.1, 17: 50.4 / 53.5 (S = 9 and 11 with the two-pass
+// form); 100,000 ROIs of sizes 11-17 in one launch: 100.3 / 103.3.
 constexpr size_t kCuLds = 160 * 1024;
 inline bool f64_two_waves(int S)
 {
