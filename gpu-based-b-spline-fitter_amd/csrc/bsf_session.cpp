@@ -74,6 +74,7 @@ struct Launch {
     std::vector<int> groups;   // launch order (largest ROI size first)
     std::vector<int> start;    // first launch row of each group
     size_t o_desc = 0, o_data = 0, o_dT = 0, o_iz = 0, o_ip = 0;   // input section
+    size_t o_end = 0;                                             // its end
     size_t o_P = 0, o_CR = 0, o_LL = 0;                           // output section
     double cost = 0;
     int stream = 0;
@@ -535,6 +536,7 @@ int submit_eager(bsf_session *s, const bsf_frame *f, int flags, unsigned long lo
         o += l.has_iz ? up((size_t)l.n * 4) : 0;
         l.o_ip = o;
         o += l.has_ip ? up((size_t)l.n * 4) : 0;
+        l.o_end = o;
     }
     const size_t in_bytes = o;
     for (Launch &l : L) {
@@ -555,23 +557,39 @@ int submit_eager(bsf_session *s, const bsf_frame *f, int flags, unsigned long lo
     int rc = admit(s, take, flags, &off);
     if (rc) return rc;
     char *h = s->h_arena + off, *d = s->d_arena + off;
-    for (Launch &l : L) pack_launch(f, l, l, h, l.o_data, 0);
+    // The longest launch first: its inputs are packed, copied in and the
+    // launch queued before the other launches' inputs are packed (and copied
+    // in one piece), so the frame's longest kernel starts while the rest of
+    // the frame is still being packed.  A heterogeneous launch needs several
+    // launches' inputs, so in that mode the whole input section is packed
+    // and copied first.
+    const bool piped = !s->hetero;
+    if (!piped)
+        for (Launch &l : L) pack_launch(f, l, l, h, l.o_data, 0);
 
     Frame F;
     F.off = off;
     F.bytes = take;
-    hipEvent_t ev_in = take_event(s);
-    if (!ev_in) return fail(BSF_ERR_DEVICE, "bsf: hipEventCreate failed");
-    F.events.push_back(ev_in);
-    hipError_t e = hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s->s_in);
-    if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
-    // largest launches first, each to the kernel stream with the least queued
+    hipEvent_t ev_in = nullptr;
+    hipError_t e = hipSuccess;
+    if (!piped) {
+        ev_in = take_event(s);
+        if (!ev_in) return fail(BSF_ERR_DEVICE, "bsf: hipEventCreate failed");
+        F.events.push_back(ev_in);
+        e = hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s->s_in);
+        if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
+    }
+    // longest launches first, each to the kernel stream with the least queued
     // work (launches of earlier frames not yet delivered included), so the
-    // big launches of consecutive frames run side by side
+    // big launches of consecutive frames run side by side.  A launch's length
+    // is its cost, a 4Pi launch's 1.6 times that: with the same work per ROI
+    // its one wave per ROI takes the channels in turn (a lone frame's 4Pi
+    // launch of ~400 ROIs ran 1.43 ms against 0.87 ms for its vortex-5D one).
+    auto length = [&](const Launch &l) { return l.cost * (s->models[l.model].ft == BSF_FT_4PI ? 1.6 : 1.0); };
     std::vector<int> &ord = s->order;
     ord.resize(L.size());
     for (size_t i = 0; i < L.size(); i++) ord[i] = (int)i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return L[a].cost > L[b].cost; });
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return length(L[a]) > length(L[b]); });
     // heterogeneous mode: the single-start launches of fittypes 6-8 go
     // together, up to kHeteroMax per launch (the first of them carries it)
     std::vector<std::vector<int>> het;
@@ -589,6 +607,29 @@ int submit_eager(bsf_session *s, const bsf_frame *f, int flags, unsigned long lo
         if (e != hipSuccess) break;
         Launch &l = L[i];
         if (l.cost < 0) continue;   // part of a heterogeneous launch
+        if (piped && (i == ord[0] || (ord.size() > 1 && i == ord[1]))) {
+            // the longest launch's inputs, then (at the second) everyone else's
+            ev_in = take_event(s);
+            if (!ev_in) {
+                e = hipErrorOutOfMemory;
+                break;
+            }
+            F.events.push_back(ev_in);
+            if (i == ord[0]) {
+                pack_launch(f, l, l, h, l.o_data, 0);
+                e = hipMemcpyAsync(d + l.o_desc, h + l.o_desc, l.o_end - l.o_desc, hipMemcpyHostToDevice, s->s_in);
+            } else {
+                for (size_t q = 1; q < ord.size(); q++) pack_launch(f, L[ord[q]], L[ord[q]], h, L[ord[q]].o_data, 0);
+                // everything but the first launch's inputs: before and after them
+                const Launch &a = L[ord[0]];
+                if (a.o_desc > 0) e = hipMemcpyAsync(d, h, a.o_desc, hipMemcpyHostToDevice, s->s_in);
+                if (e == hipSuccess && a.o_end < in_bytes)
+                    e = hipMemcpyAsync(d + a.o_end, h + a.o_end, in_bytes - a.o_end, hipMemcpyHostToDevice, s->s_in);
+            }
+            if (e == hipSuccess) e = hipEventRecord(ev_in, s->s_in);
+            if (e != hipSuccess) break;
+        }
+        hipEvent_t ev_l = ev_in;
         int j = 0;
         for (int q = 1; q < s->nks; q++)
             if (s->load[q] < s->load[j]) j = q;
@@ -596,7 +637,7 @@ int submit_eager(bsf_session *s, const bsf_frame *f, int flags, unsigned long lo
         F.work.emplace_back(j, l.cost);
         l.stream = j;
         hipStream_t ks = s->ks[j];
-        e = hipStreamWaitEvent(ks, ev_in, 0);
+        e = hipStreamWaitEvent(ks, ev_l, 0);
         const std::vector<int> *grp = nullptr;
         for (const auto &x : het)
             if (x[0] == i && x.size() > 1) grp = &x;
